@@ -1,0 +1,740 @@
+/*
+ * Host side of the C ABI (include/orb_capi.h): context, parameter tables,
+ * pyramid geometry, frame-set memory and launch sequencing.
+ *
+ * The reference keeps parameters in module-global __constant__ symbols
+ * (orbd.cu:32-41) and reallocates its scratch on every size change
+ * (orb.cpp:145-191); here a context owns the tables and a cached frame set
+ * (geometry, job tables and HBM buffers for a list of image sizes), so a
+ * repeated call with the same sizes allocates nothing and syncs nothing.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "orb_capi.h"
+#include "orb_internal.h"
+
+using namespace orbk;
+
+namespace
+{
+
+const int32_t k_pattern31[1024] = {
+#include "orb_pattern31.inc"
+};
+
+std::string g_create_error;
+
+int align_up(int a, int b) { return (a % b != 0) ? (a - a % b + b) : a; }
+
+/* Pyramid geometry.  Scale 2 follows orb.cpp:145-191: levels
+   min(noctaves, (int)log2f(min(w,h)/80) + 1) capped at MAX_OCTAVE (5),
+   w_l = w_{l-1} >> 1, NMS radius 2^l.  Another scale s in (1,2):
+   w_l = floor(w / s^l + 0.5) while the short side is >= 16 (at most 8 levels),
+   NMS radius max(1, floor(s^l + 0.5)).  Pitch alignUp(w_l, 128), levels
+   contiguous. */
+struct Geom {
+    int L = 0, ref = 1;
+    double scale = 2.0;
+    int w[kMaxLevels] = {}, h[kMaxLevels] = {}, p[kMaxLevels] = {}, radius[kMaxLevels] = {};
+    long long off[kMaxLevels] = {};
+    double sc[kMaxLevels] = {};
+    long long total = 0;
+};
+
+bool compute_geom(int w, int h, int noctaves, double scale, Geom& g)
+{
+    g = Geom();
+    if (w <= 0 || h <= 0 || noctaves < 1) return false;
+    if (!(scale > 1.0 && scale <= 2.0)) return false;
+    g.scale = scale;
+    g.ref = scale == 2.0;
+    int L;
+    if (g.ref) {
+        const int m = (w < h ? w : h) / 80;
+        if (m < 1) return false;
+        L = (int)log2f((float)m) + 1;
+        if (L > noctaves) L = noctaves;
+        if (L > kRefMaxLevels) L = kRefMaxLevels;
+        for (int l = 0; l < L; l++) {
+            g.w[l] = l == 0 ? w : g.w[l - 1] >> 1;
+            g.h[l] = l == 0 ? h : g.h[l - 1] >> 1;
+            g.sc[l] = (double)(1 << l);
+            g.radius[l] = 1 << l;
+        }
+    } else {
+        if ((w < h ? w : h) < 16) return false;
+        L = 0;
+        double sc = 1.0;
+        for (int l = 0; l < noctaves && l < kMaxLevels; l++) {
+            if (l > 0) sc *= scale;
+            const int wl = (int)floor(w / sc + 0.5), hl = (int)floor(h / sc + 0.5);
+            if ((wl < hl ? wl : hl) < 16) break;
+            g.w[l] = wl;
+            g.h[l] = hl;
+            g.sc[l] = sc;
+            const int r = (int)floor(sc + 0.5);
+            g.radius[l] = r < 1 ? 1 : r;
+            L++;
+        }
+    }
+    g.L = L;
+    long long off = 0;
+    for (int l = 0; l < L; l++) {
+        g.p[l] = align_up(g.w[l], 128);
+        g.off[l] = off;
+        off += (long long)g.h[l] * g.p[l];
+    }
+    g.total = off;
+    return L > 0;
+}
+
+/* bilinear source position of level index i along an axis of S source and D
+   level pixels: (i0 << 8) | frac, centre-aligned, 8 fractional bits */
+int resize_tab(int i, int S, int D)
+{
+    long long num = ((long long)(2 * i + 1) * S - D) * 256;
+    long long pos = num / (2LL * D);
+    if (pos < 0) pos = 0;
+    if (pos > (long long)(S - 1) * 256) pos = (long long)(S - 1) * 256;
+    return (int)pos;
+}
+
+size_t round256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct FrameSet {
+    std::vector<int> ws, hs;
+    int noctaves = 0, border = 0, cap = 0;
+    double scale = 0;
+    int nframes = 0;
+    std::vector<FrameDev> hframes;
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    FrameDev* d_frames = nullptr;
+    Job* d_pyr = nullptr;
+    Job* d_det = nullptr;
+    int pyr_njobs = 0, pyr_blocks = 0, det_njobs = 0, det_blocks = 0;
+    int* d_nms_frame = nullptr;
+    int nms_blocks = 0;
+    Candidate* d_cand = nullptr;
+    int* d_blk_count = nullptr;
+    int* d_blk_out = nullptr;
+    int* d_blk_eq = nullptr;
+    SelectState* d_sel = nullptr;
+    uint32_t* d_epochs = nullptr;
+    int* d_tabs = nullptr;
+};
+
+} // namespace
+
+struct orb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool inited = false;
+    orb_params prm{};
+    int2* d_pattern = nullptr;
+    float gauss[4] = {};
+    float harris_scale = 0.f;
+    int umax[32] = {};
+    FrameSet fs;
+    uint32_t* d_match_partial = nullptr;
+    size_t match_partial_cap = 0;
+    std::string err;
+    /* stage timing: events recorded around launches while prof_mask != 0 */
+    unsigned prof_mask = 0;
+    std::vector<hipEvent_t> ev_pool;
+    struct Rec { int stage; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+};
+
+namespace
+{
+
+int fail(orb_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_create_error = buf;
+    return code;
+}
+
+int hip_fail(orb_ctx* c, hipError_t e, const char* what)
+{
+    return fail(c, e == hipErrorOutOfMemory ? ORB_E_NOMEM : ORB_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+/* setUmax, orbd.cu:66-92 */
+void make_umax(int patch_size, int* umax)
+{
+    const int half = patch_size / 2;
+    umax[half + 1] = 0;
+    const float v = half * sqrtf(2.f) / 2;
+    const int vmax = (int)floorf(v + 1);
+    const int vmin = (int)ceilf(v);
+    for (int i = 0; i <= vmax; i++) umax[i] = (int)roundf(sqrtf((float)(half * half - i * i)));
+    for (int i = half, v0 = 0; i >= vmin; --i) {
+        while (umax[v0] == umax[v0 + 1]) ++v0;
+        umax[i] = v0;
+        ++v0;
+    }
+}
+
+/* setGaussianKernel, orbd.cu:413-433 */
+void make_gauss(float k[4])
+{
+    const float svar = -1.f / (2.f * 2.f * 2.f);
+    float sum = 0.f;
+    for (int i = 0; i <= 3; i++) {
+        k[i] = expf((float)(i * i) * svar);
+        sum = sum + (k[i] + (i == 0 ? 0.f : k[i]));
+    }
+    sum = 1.f / sum;
+    for (int i = 0; i <= 3; i++) k[i] *= sum;
+}
+
+/* setPattern for patch 31, orbd.cu:95-410: the table verbatim for WTA_K 2;
+   for WTA_K 3/4, 128 tuples of distinct points drawn with glibc
+   srand(0x12345678)/rand(), as the reference does. */
+void make_pattern(int wta_k, int32_t* xy)
+{
+    if (wta_k == 2) {
+        memcpy(xy, k_pattern31, sizeof(k_pattern31));
+        return;
+    }
+    memset(xy, 0, 1024 * sizeof(int32_t));
+    srand(0x12345678);
+    for (int i = 0; i < 128; i++) {
+        for (int k = 0; k < wta_k; k++) {
+            for (;;) {
+                const int idx = rand() % 512;
+                const int px = k_pattern31[2 * idx], py = k_pattern31[2 * idx + 1];
+                int k1;
+                for (k1 = 0; k1 < k; k1++) {
+                    const int q = wta_k * i + k1;
+                    if (px == xy[2 * q] && py == xy[2 * q + 1]) break;
+                }
+                if (k1 == k) {
+                    xy[2 * (wta_k * i + k)] = px;
+                    xy[2 * (wta_k * i + k) + 1] = py;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+void free_frameset(FrameSet& fs)
+{
+    if (fs.arena) (void)hipFree(fs.arena);
+    fs = FrameSet();
+}
+
+/* Build geometry, job tables and buffers for a list of frame sizes. */
+int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int>& hs)
+{
+    FrameSet& fs = c->fs;
+    const orb_params& P = c->prm;
+    const int border = P.edge_threshold < 3 ? 3 : P.edge_threshold;
+    if (fs.arena && fs.ws == ws && fs.hs == hs && fs.noctaves == P.noctaves && fs.scale == P.scale_factor &&
+        fs.border == border && fs.cap == P.max_pts)
+        return ORB_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    free_frameset(fs);
+    const int nf = (int)ws.size();
+    std::vector<Geom> geo(nf);
+    for (int f = 0; f < nf; f++) {
+        if (ws[f] > 16384 || hs[f] > 16384)
+            return fail(c, ORB_E_CAPACITY, "frame %d: %dx%d exceeds 16384", f, ws[f], hs[f]);
+        if (!compute_geom(ws[f], hs[f], P.noctaves, P.scale_factor, geo[f]))
+            return fail(c, ORB_E_ARG, "frame %d: %dx%d too small for the pyramid (scale %g)", f, ws[f], hs[f],
+                        P.scale_factor);
+    }
+    /* sizes of every sub-buffer */
+    std::vector<size_t> pyr_off(nf), key_off(nf);
+    std::vector<int> guard(nf), key_guard_rows(nf), tab_base(nf);
+    size_t pyr_bytes = 0, key_bytes = 0;
+    int tab_ints = 0;
+    std::vector<Job> pyr_jobs, det_jobs;
+    std::vector<int> nms_frame;
+    int pyr_blocks = 0, det_blocks = 0, nms_blocks = 0;
+    fs.hframes.assign(nf, FrameDev());
+    for (int f = 0; f < nf; f++) {
+        const Geom& g = geo[f];
+        FrameDev& F = fs.hframes[f];
+        memset(&F, 0, sizeof(F));
+        F.w0 = g.w[0];
+        F.h0 = g.h[0];
+        F.p0 = g.p[0];
+        F.nlevels = g.L;
+        F.ref_mode = g.ref;
+        F.border = border;
+        F.cap = P.max_pts;
+        F.total = g.total;
+        int rmax = 1;
+        for (int l = 0; l < g.L; l++) {
+            F.lw[l] = g.w[l];
+            F.lh[l] = g.h[l];
+            F.lp[l] = g.p[l];
+            F.loff[l] = g.off[l];
+            F.radius[l] = g.radius[l];
+            F.inv_sc[l] = (float)(1.0 / g.sc[l]);
+            F.tab_off[l] = tab_ints;
+            if (!g.ref && l > 0) tab_ints += 2 * (g.w[l] + g.h[l]);
+            if (g.radius[l] > rmax) rmax = g.radius[l];
+            /* pyramid tiles cover the whole pitch (padding written too) */
+            const int ptx = g.p[l] / kPyrTW, pty = (g.h[l] + kPyrTH - 1) / kPyrTH;
+            pyr_jobs.push_back(Job{f, l, ptx, pyr_blocks});
+            pyr_blocks += ptx * pty;
+            if (g.w[l] > 6 && g.h[l] > 6) {
+                const int dtx = (g.w[l] - 6 + kDetTW - 1) / kDetTW, dty = (g.h[l] - 6 + kDetTH - 1) / kDetTH;
+                det_jobs.push_back(Job{f, l, dtx, det_blocks});
+                det_blocks += dtx * dty;
+            }
+        }
+        /* flat-pyramid guard: covers the deepest stencil / patch overhang
+           (Harris +-4, describe +-18 rows/cols) beyond either end */
+        guard[f] = 32 * g.p[0];
+        pyr_off[f] = pyr_bytes;
+        pyr_bytes += round256((size_t)g.total + 2 * (size_t)guard[f]);
+        key_guard_rows[f] = rmax + 1;
+        key_off[f] = key_bytes;
+        key_bytes += round256(sizeof(unsigned long long) * (size_t)(g.h[0] + 2 * key_guard_rows[f]) * g.p[0]);
+        const int rows = g.h[0] - 2 * border, span = g.w[0] - 2 * border;
+        F.nms_first = nms_blocks;
+        F.nms_rows = rows > 0 && span > 0 ? rows : 0;
+        F.nms_segs = rows > 0 && span > 0 ? (span + kNmsSeg - 1) / kNmsSeg : 0;
+        for (int b = 0; b < F.nms_rows * F.nms_segs; b++) nms_frame.push_back(f);
+        nms_blocks += F.nms_rows * F.nms_segs;
+    }
+    /* one arena for everything, zeroed once */
+    size_t off = 0;
+    auto carve = [&](size_t bytes) { size_t o = off; off += round256(bytes ? bytes : 1); return o; };
+    const size_t o_frames = carve(sizeof(FrameDev) * nf);
+    const size_t o_pyr_jobs = carve(sizeof(Job) * pyr_jobs.size());
+    const size_t o_det_jobs = carve(sizeof(Job) * det_jobs.size());
+    const size_t o_nms_frame = carve(sizeof(int) * nms_frame.size());
+    const size_t o_cand = carve(sizeof(Candidate) * (size_t)nms_blocks * (kNmsSeg / 2));
+    const size_t o_cnt = carve(sizeof(int) * nms_blocks);
+    const size_t o_out = carve(sizeof(int) * nms_blocks);
+    const size_t o_eq = carve(sizeof(int) * nms_blocks);
+    const size_t o_sel = carve(sizeof(SelectState) * nf);
+    const size_t o_epochs = carve(sizeof(uint32_t) * nf);
+    const size_t o_tabs = carve(sizeof(int) * (size_t)tab_ints);
+    const size_t o_pyr = carve(2 * pyr_bytes);
+    const size_t o_keys = carve(key_bytes);
+    const size_t total = off;
+    void* arena = nullptr;
+    hipError_t e = hipMalloc(&arena, total);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(frame set)");
+    fs.arena = arena;
+    fs.arena_bytes = total;
+    char* base = (char*)arena;
+    HIP_TRY(c, hipMemsetAsync(arena, 0, total, c->stream));
+    fs.d_frames = (FrameDev*)(base + o_frames);
+    fs.d_pyr = (Job*)(base + o_pyr_jobs);
+    fs.d_det = (Job*)(base + o_det_jobs);
+    fs.d_nms_frame = (int*)(base + o_nms_frame);
+    fs.d_cand = (Candidate*)(base + o_cand);
+    fs.d_blk_count = (int*)(base + o_cnt);
+    fs.d_blk_out = (int*)(base + o_out);
+    fs.d_blk_eq = (int*)(base + o_eq);
+    fs.d_sel = (SelectState*)(base + o_sel);
+    fs.d_epochs = (uint32_t*)(base + o_epochs);
+    fs.d_tabs = (int*)(base + o_tabs);
+    uint8_t* pyr = (uint8_t*)(base + o_pyr);
+    unsigned long long* keys = (unsigned long long*)(base + o_keys);
+    /* coordinate tables for scale != 2 */
+    std::vector<int> tabs((size_t)tab_ints);
+    for (int f = 0; f < nf; f++) {
+        const Geom& g = geo[f];
+        FrameDev& F = fs.hframes[f];
+        F.raw = pyr + pyr_off[f] + guard[f];
+        F.blur = pyr + pyr_bytes + pyr_off[f] + guard[f];
+        F.keys = (unsigned long long*)((char*)keys + key_off[f]) + (size_t)key_guard_rows[f] * g.p[0];
+        if (g.ref) continue;
+        for (int l = 1; l < g.L; l++) {
+            int* t = tabs.data() + F.tab_off[l];
+            for (int x = 0; x < g.w[l]; x++) t[x] = resize_tab(x, g.w[0], g.w[l]);
+            for (int y = 0; y < g.h[l]; y++) t[g.w[l] + y] = resize_tab(y, g.h[0], g.h[l]);
+            for (int x = 0; x < g.w[l]; x++) t[g.w[l] + g.h[l] + x] = (int)floor(x * g.sc[l] + 0.5);
+            for (int y = 0; y < g.h[l]; y++) t[2 * g.w[l] + g.h[l] + y] = (int)floor(y * g.sc[l] + 0.5);
+        }
+    }
+    std::vector<uint32_t> epochs(nf, 1u);
+    HIP_TRY(c, hipMemcpyAsync(fs.d_pyr, pyr_jobs.data(), sizeof(Job) * pyr_jobs.size(), hipMemcpyHostToDevice, c->stream));
+    if (!det_jobs.empty())
+        HIP_TRY(c, hipMemcpyAsync(fs.d_det, det_jobs.data(), sizeof(Job) * det_jobs.size(), hipMemcpyHostToDevice, c->stream));
+    if (!nms_frame.empty())
+        HIP_TRY(c, hipMemcpyAsync(fs.d_nms_frame, nms_frame.data(), sizeof(int) * nms_frame.size(), hipMemcpyHostToDevice, c->stream));
+    if (tab_ints)
+        HIP_TRY(c, hipMemcpyAsync(fs.d_tabs, tabs.data(), sizeof(int) * tabs.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(fs.d_epochs, epochs.data(), sizeof(uint32_t) * nf, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    fs.ws = ws;
+    fs.hs = hs;
+    fs.noctaves = P.noctaves;
+    fs.scale = P.scale_factor;
+    fs.border = border;
+    fs.cap = P.max_pts;
+    fs.nframes = nf;
+    fs.pyr_njobs = (int)pyr_jobs.size();
+    fs.pyr_blocks = pyr_blocks;
+    fs.det_njobs = (int)det_jobs.size();
+    fs.det_blocks = det_blocks;
+    fs.nms_blocks = nms_blocks;
+    /* force the first per-call upload */
+    for (auto& F : fs.hframes) F.img = nullptr;
+    return ORB_OK;
+}
+
+/* Events for the stages of one call (null when timing is off). */
+const StageEvents* take_events(orb_ctx* c, StageEvents& ev, unsigned stages)
+{
+    const unsigned m = c->prof_mask & stages;
+    if (!m) return nullptr;
+    for (int s = 0; s < kNumStages; s++) {
+        ev.on[s] = (m >> s) & 1u;
+        if (!ev.on[s]) continue;
+        for (int k = 0; k < 2; k++) {
+            hipEvent_t e = nullptr;
+            if (!c->ev_pool.empty()) {
+                e = c->ev_pool.back();
+                c->ev_pool.pop_back();
+            } else if (hipEventCreate(&e) != hipSuccess) {
+                ev.on[s] = false;
+                break;
+            }
+            (k == 0 ? ev.start[s] : ev.stop[s]) = e;
+        }
+        if (ev.on[s]) c->recs.push_back({s, ev.start[s], ev.stop[s]});
+    }
+    return &ev;
+}
+
+Tables make_tables(const orb_ctx* c)
+{
+    Tables t;
+    memset(&t, 0, sizeof(t));
+    t.tabs = c->fs.d_tabs;
+    t.pattern = c->d_pattern;
+    t.k0 = c->gauss[0];
+    t.k1 = c->gauss[1];
+    t.k2 = c->gauss[2];
+    t.k3 = c->gauss[3];
+    t.harris_scale = c->harris_scale;
+    t.threshold = c->prm.fast_threshold;
+    t.score_type = c->prm.score_type;
+    t.wta_k = c->prm.wta_k;
+    t.half_patch = c->prm.patch_size / 2;
+    for (int i = 0; i < 17; i++) t.umax[i] = c->umax[i];
+    return t;
+}
+
+int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, const int* hs, const int* pitches,
+               orb::OrbPoint* const* pts, uint8_t* const* desc, int* d_counts, int compute_desc)
+{
+    if (!c->inited) return fail(c, ORB_E_STATE, "orb_init has not been called");
+    if (nf <= 0) return fail(c, ORB_E_ARG, "nframes must be positive");
+    std::vector<int> vw(ws, ws + nf), vh(hs, hs + nf);
+    for (int f = 0; f < nf; f++) {
+        if (!imgs[f] || !pts[f] || !d_counts) return fail(c, ORB_E_ARG, "frame %d: null pointer", f);
+        if (compute_desc && !desc[f]) return fail(c, ORB_E_ARG, "frame %d: null descriptor buffer", f);
+        if (pitches[f] < ws[f]) return fail(c, ORB_E_ARG, "frame %d: pitch %d < width %d", f, pitches[f], ws[f]);
+    }
+    int rc = build_frameset(c, vw, vh);
+    if (rc != ORB_OK) return rc;
+    FrameSet& fs = c->fs;
+    bool changed = false;
+    for (int f = 0; f < nf; f++) {
+        FrameDev& F = fs.hframes[f];
+        if (F.img != imgs[f] || F.ip != pitches[f] || F.pts != pts[f] || F.desc != (compute_desc ? desc[f] : nullptr) ||
+            F.count != d_counts + f) {
+            F.img = imgs[f];
+            F.ip = pitches[f];
+            F.pts = pts[f];
+            F.desc = compute_desc ? desc[f] : nullptr;
+            F.count = d_counts + f;
+            changed = true;
+        }
+    }
+    if (changed) {
+        HIP_TRY(c, hipMemcpyAsync(fs.d_frames, fs.hframes.data(), sizeof(FrameDev) * nf, hipMemcpyHostToDevice,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    DetectLaunch d;
+    memset(&d, 0, sizeof(d));
+    d.frames = fs.d_frames;
+    d.nframes = nf;
+    d.pyr_jobs = fs.d_pyr;
+    d.pyr_njobs = fs.pyr_njobs;
+    d.pyr_blocks = fs.pyr_blocks;
+    d.det_jobs = fs.d_det;
+    d.det_njobs = fs.det_njobs;
+    d.det_blocks = fs.det_blocks;
+    d.nms_blocks = fs.nms_blocks;
+    d.nms_block_frame = fs.d_nms_frame;
+    d.cand = fs.d_cand;
+    d.blk_count = fs.d_blk_count;
+    d.blk_out = fs.d_blk_out;
+    d.blk_eq = fs.d_blk_eq;
+    d.sel = fs.d_sel;
+    d.epochs = fs.d_epochs;
+    d.desc_blocks_per_frame = compute_desc ? (c->prm.max_pts + kDescPerBlock - 1) / kDescPerBlock : 0;
+    d.t = make_tables(c);
+    StageEvents ev;
+    memset(&ev, 0, sizeof(ev));
+    hipError_t e = launch_detect(d, compute_desc, c->stream, take_events(c, ev, 0x3fu));
+    if (e != hipSuccess) return hip_fail(c, e, "detect launch");
+    return ORB_OK;
+}
+
+int ensure_partial(orb_ctx* c, size_t words)
+{
+    if (words <= c->match_partial_cap) return ORB_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->d_match_partial) (void)hipFree(c->d_match_partial);
+    c->d_match_partial = nullptr;
+    c->match_partial_cap = 0;
+    HIP_TRY(c, hipMalloc((void**)&c->d_match_partial, words * sizeof(uint32_t)));
+    c->match_partial_cap = words;
+    return ORB_OK;
+}
+
+int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8_t* db, int ndb, const int* d_ndb,
+              int method, int* idx1, int* d1, int* idx2, int* d2, orb::OrbPoint* pts)
+{
+    if (nq < 0 || ndb < 0) return fail(c, ORB_E_ARG, "negative descriptor count");
+    if (ndb >= (1 << 23)) return fail(c, ORB_E_CAPACITY, "database of %d entries exceeds 2^23-1", ndb);
+    if (nq == 0) return ORB_OK;
+    if (!q || (ndb > 0 && !db)) return fail(c, ORB_E_ARG, "null descriptor pointer");
+    if (method != ORB_MATCH_POPCOUNT && method != ORB_MATCH_MFMA) return fail(c, ORB_E_ARG, "unknown method %d", method);
+    MatchLaunch m;
+    memset(&m, 0, sizeof(m));
+    const size_t words = match_partial_words(nq, ndb, method, &m.nchunks, &m.chunk);
+    int rc = ensure_partial(c, words);
+    if (rc != ORB_OK) return rc;
+    m.q = q;
+    m.nq = nq;
+    m.d_nq = d_nq;
+    m.db = db;
+    m.ndb = ndb;
+    m.d_ndb = d_ndb;
+    m.partial = c->d_match_partial;
+    m.idx1 = idx1;
+    m.d1 = d1;
+    m.idx2 = idx2;
+    m.d2 = d2;
+    m.pts = pts;
+    StageEvents ev;
+    memset(&ev, 0, sizeof(ev));
+    const StageEvents* pev = take_events(c, ev, 0xc0u);
+    hipError_t e = method == ORB_MATCH_MFMA ? launch_match_mfma(m, c->stream, pev)
+                                            : launch_match_popcount(m, c->stream, pev);
+    if (e != hipSuccess) return hip_fail(c, e, method == ORB_MATCH_MFMA ? "MFMA match" : "popcount match");
+    return ORB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int orb_ctx_create(orb_ctx** out, int device)
+{
+    if (!out) return fail(nullptr, ORB_E_ARG, "null context pointer");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, ORB_E_HIP, "no HIP device available");
+    if (device < 0 || device >= n) return fail(nullptr, ORB_E_ARG, "device %d out of range [0,%d)", device, n);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, ORB_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    orb_ctx* c = new orb_ctx();
+    c->device = device;
+    e = hipMalloc((void**)&c->d_pattern, 512 * sizeof(int2));
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, ORB_E_NOMEM, "hipMalloc(pattern): %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return ORB_OK;
+}
+
+int orb_ctx_destroy(orb_ctx* c)
+{
+    if (!c) return ORB_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_frameset(c->fs);
+    for (auto& r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->d_pattern) (void)hipFree(c->d_pattern);
+    if (c->d_match_partial) (void)hipFree(c->d_match_partial);
+    delete c;
+    return ORB_OK;
+}
+
+int orb_set_stream(orb_ctx* c, void* stream)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    c->stream = (hipStream_t)stream;
+    return ORB_OK;
+}
+
+int orb_init(orb_ctx* c, const orb_params* p)
+{
+    if (!c || !p) return fail(c, ORB_E_ARG, "null argument");
+    if (p->noctaves < 1) return fail(c, ORB_E_ARG, "noctaves must be >= 1");
+    if (p->wta_k < 2 || p->wta_k > 4) return fail(c, ORB_E_ARG, "wta_k must be 2, 3 or 4");
+    if (p->score_type != orb::HARRIS_SCORE && p->score_type != orb::FAST_SCORE)
+        return fail(c, ORB_E_ARG, "unknown score type %d", p->score_type);
+    if (p->patch_size != 31) return fail(c, ORB_E_ARG, "patch_size must be 31");
+    if (p->fast_threshold < 0 || p->fast_threshold > 255) return fail(c, ORB_E_ARG, "fast_threshold out of [0,255]");
+    if (p->max_pts <= 0) return fail(c, ORB_E_ARG, "max_pts must be positive");
+    if (p->max_pts > (1 << 22)) return fail(c, ORB_E_CAPACITY, "max_pts above 2^22");
+    if (!(p->scale_factor > 1.0 && p->scale_factor <= 2.0)) return fail(c, ORB_E_ARG, "scale_factor must be in (1,2]");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->prm = *p;
+    make_umax(p->patch_size, c->umax);
+    make_gauss(c->gauss);
+    {
+        const float s = 1.f / (4 * 7 * 255.f); /* setScaleSqSq, orbd.cu:436-441 */
+        c->harris_scale = s * s * s * s;
+    }
+    int32_t pat[1024];
+    make_pattern(p->wta_k, pat);
+    HIP_TRY(c, hipMemcpyAsync(c->d_pattern, pat, sizeof(pat), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->inited = true;
+    return ORB_OK;
+}
+
+int orb_num_levels(const orb_ctx* c, int w, int h)
+{
+    if (!c || !c->inited) return 0;
+    Geom g;
+    if (!compute_geom(w, h, c->prm.noctaves, c->prm.scale_factor, g)) return 0;
+    return g.L;
+}
+
+int orb_detect_and_compute(orb_ctx* c, const uint8_t* img, int w, int h, int pitch, orb::OrbPoint* pts,
+                           uint8_t* desc, int* d_count, int compute_desc)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    return run_frames(c, 1, &img, &w, &h, &pitch, &pts, &desc, d_count, compute_desc);
+}
+
+int orb_detect_and_compute_batch(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, const int* hs,
+                                 const int* pitches, orb::OrbPoint* const* pts, uint8_t* const* desc, int* d_counts,
+                                 int compute_desc)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (!imgs || !ws || !hs || !pitches || !pts || (compute_desc && !desc))
+        return fail(c, ORB_E_ARG, "null frame array");
+    return run_frames(c, nf, imgs, ws, hs, pitches, pts, desc, d_counts, compute_desc);
+}
+
+int orb_match(orb_ctx* c, orb::OrbPoint* pts1, const uint8_t* desc1, int n1, const uint8_t* desc2, int n2)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (n1 > 0 && !pts1) return fail(c, ORB_E_ARG, "null point array");
+    return run_match(c, desc1, n1, nullptr, desc2, n2, nullptr, ORB_MATCH_POPCOUNT, nullptr, nullptr, nullptr,
+                     nullptr, pts1);
+}
+
+int orb_match_dcount(orb_ctx* c, orb::OrbPoint* pts1, const uint8_t* desc1, const int* d_n1, int cap1,
+                     const uint8_t* desc2, const int* d_n2, int cap2)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (!pts1 || !d_n1 || !d_n2) return fail(c, ORB_E_ARG, "null pointer");
+    return run_match(c, desc1, cap1, d_n1, desc2, cap2, d_n2, ORB_MATCH_POPCOUNT, nullptr, nullptr, nullptr,
+                     nullptr, pts1);
+}
+
+int orb_match_top2(orb_ctx* c, const uint8_t* q, int nq, const uint8_t* db, int ndb, int method, int* idx1, int* d1,
+                   int* idx2, int* d2)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    return run_match(c, q, nq, nullptr, db, ndb, nullptr, method, idx1, d1, idx2, d2, nullptr);
+}
+
+int orb_sync(orb_ctx* c)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return ORB_OK;
+}
+
+const char* orb_last_error(const orb_ctx* c)
+{
+    return c ? c->err.c_str() : g_create_error.c_str();
+}
+
+size_t orb_workspace_bytes(const orb_ctx* c)
+{
+    return c ? c->fs.arena_bytes + c->match_partial_cap * sizeof(uint32_t) : 0;
+}
+
+int orb_profile_enable(orb_ctx* c, unsigned stage_mask)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    c->prof_mask = stage_mask & ((1u << ORB_NUM_STAGES) - 1u);
+    return ORB_OK;
+}
+
+int orb_profile_read(orb_ctx* c, double* ms_sum, int* counts)
+{
+    if (!c || !ms_sum || !counts) return fail(c, ORB_E_ARG, "null argument");
+    for (int s = 0; s < ORB_NUM_STAGES; s++) {
+        ms_sum[s] = 0.0;
+        counts[s] = 0;
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ms_sum[r.stage] += ms;
+            counts[r.stage] += 1;
+        }
+        c->ev_pool.push_back(r.a);
+        c->ev_pool.push_back(r.b);
+    }
+    c->recs.clear();
+    return ORB_OK;
+}
+
+int orb_debug_sincos(orb_ctx* c, const float* in, int n, float* s, float* co)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    hipError_t e = launch_debug_sincos(in, n, s, co, c->stream);
+    return e == hipSuccess ? ORB_OK : hip_fail(c, e, "debug sincos");
+}
+
+int orb_debug_atan2(orb_ctx* c, const float* yx, int n, float* out)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    hipError_t e = launch_debug_atan2(yx, n, out, c->stream);
+    return e == hipSuccess ? ORB_OK : hip_fail(c, e, "debug atan2");
+}
+
+} // extern "C"

@@ -1,0 +1,150 @@
+/*
+ * Internal layout shared by the host orchestration (orb_capi.cpp) and the
+ * gfx950 kernels (orb_kernels.hip, orb_match.hip).
+ *
+ * A "frame set" is a list of images processed by one launch per stage.  Each
+ * frame owns, in HBM:
+ *   raw   flat pyramid of unblurred levels: levels contiguous, row pitch
+ *         alignUp(w_l,128), with GUARD zero bytes before level 0 and after the
+ *         last level, so every stencil / patch read stays inside the allocation
+ *         and reads 0 outside the pyramid (the reference's flat addressing,
+ *         orb.cpp:151-175);
+ *   blur  the same layout holding the 7x7 Gaussian of every level;
+ *   keys  the level-0 score map, one uint64 per level-0 pixel (pitch p0) with
+ *         guard rows for the NMS window: key = epoch<<34 | score_bits<<3 |
+ *         (7 - level).  A max over keys is the reference's "strictly greater
+ *         score wins, lowest level wins ties" (orbd.cu:750-754), and a key of
+ *         an older epoch reads as empty, so the map is never cleared.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "orb_structures.h"
+
+namespace orbk
+{
+
+constexpr int kMaxLevels = 8;
+constexpr int kRefMaxLevels = 5;     /* MAX_OCTAVE, orbd.cu:18 */
+constexpr int kEpochShift = 34;
+constexpr uint32_t kEpochLimit = 1u << 30;
+
+/* pyramid tile: 64 x 16 outputs, 256 threads */
+constexpr int kPyrTW = 64, kPyrTH = 16;
+/* detection tile: 64 x 4 pixels, 256 threads */
+constexpr int kDetTW = 64, kDetTH = 4;
+/* NMS block: one level-0 row segment of 256 pixels */
+constexpr int kNmsSeg = 256;
+/* describe: one wavefront per keypoint, 4 per workgroup */
+constexpr int kDescPerBlock = 4;
+
+struct FrameDev {
+    /* per call */
+    const uint8_t* img;
+    int ip;                    /* input row pitch */
+    orb::OrbPoint* pts;
+    uint8_t* desc;
+    int* count;
+    /* geometry */
+    int w0, h0, p0;
+    int nlevels, ref_mode;
+    int border, cap;
+    int lw[kMaxLevels], lh[kMaxLevels], lp[kMaxLevels], radius[kMaxLevels];
+    long long loff[kMaxLevels];
+    float inv_sc[kMaxLevels];  /* 1/s^l, level-0 -> level coordinate (scale != 2) */
+    int tab_off[kMaxLevels];   /* into the int table: xsrc[w], ysrc[h], x2l0[w], y2l0[h] */
+    long long total;
+    uint8_t* raw;              /* level-0 start of the raw flat pyramid */
+    uint8_t* blur;             /* level-0 start of the blurred flat pyramid */
+    unsigned long long* keys;  /* row 0 of the key map */
+    /* NMS blocks */
+    int nms_first, nms_rows, nms_segs;
+};
+
+/* A contiguous range of workgroups covering one (frame, level) region. */
+struct Job {
+    int frame, level, tiles_x, first_block;
+};
+
+/* Per-frame selection state written by k_select, read by k_emit. */
+struct SelectState {
+    uint32_t thr;   /* score bits of the max_pts-th best (0: keep all) */
+    int need;       /* how many keys equal to thr are kept, in raster order */
+    uint32_t epoch; /* epoch of the current detection */
+    int pad;
+};
+
+struct Candidate {
+    uint32_t xy;    /* y << 16 | x, level-0 */
+    uint32_t score; /* float bits */
+    uint32_t level;
+    uint32_t pad;
+};
+
+struct Tables {
+    const int* tabs;          /* resize / coordinate tables */
+    const int2* pattern;      /* 512 sampling points */
+    float k0, k1, k2, k3;     /* Gaussian taps */
+    float harris_scale;
+    int threshold, score_type, wta_k, half_patch;
+    int umax[17];
+};
+
+struct DetectLaunch {
+    const FrameDev* frames;
+    int nframes;
+    const Job* pyr_jobs; int pyr_njobs; int pyr_blocks;
+    const Job* det_jobs; int det_njobs; int det_blocks;
+    int nms_blocks;
+    const int* nms_block_frame; /* frame of each NMS block */
+    Candidate* cand;            /* kNmsSeg per NMS block */
+    int* blk_count;             /* survivors per NMS block */
+    int* blk_out;               /* output offset per NMS block */
+    int* blk_eq;                /* keys equal to thr before the block */
+    SelectState* sel;           /* per frame */
+    uint32_t* epochs;           /* per frame */
+    int max_blocks_per_frame;
+    int desc_blocks_per_frame;
+    Tables t;
+};
+
+/* Stage ids for event timing (orb_profile_read). */
+enum Stage { kStPyramid = 0, kStDetect, kStNms, kStSelect, kStEmit, kStDescribe, kStMatch, kStMatchMerge,
+             kNumStages };
+
+/* Optional start/stop events per stage, recorded around each launch. */
+struct StageEvents {
+    hipEvent_t start[kNumStages];
+    hipEvent_t stop[kNumStages];
+    bool on[kNumStages];
+};
+
+/* orb_kernels.hip */
+hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s, const StageEvents* ev);
+hipError_t launch_debug_sincos(const float* in, int n, float* s, float* c, hipStream_t st);
+hipError_t launch_debug_atan2(const float* yx, int n, float* out, hipStream_t st);
+
+/* orb_match.hip */
+struct MatchLaunch {
+    const uint8_t* q; int nq; const int* d_nq;
+    const uint8_t* db; int ndb; const int* d_ndb;
+    uint32_t* partial; int nchunks; int chunk;
+    int* idx1; int* d1; int* idx2; int* d2;   /* top-2 outputs (may be null) */
+    orb::OrbPoint* pts;                       /* reference-rule output (may be null) */
+};
+hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
+hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
+
+inline void ev_start(const StageEvents* ev, int st, hipStream_t s)
+{
+    if (ev && ev->on[st]) (void)hipEventRecord(ev->start[st], s);
+}
+inline void ev_stop(const StageEvents* ev, int st, hipStream_t s)
+{
+    if (ev && ev->on[st]) (void)hipEventRecord(ev->stop[st], s);
+}
+size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk);
+
+} // namespace orbk

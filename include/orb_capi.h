@@ -1,0 +1,161 @@
+/*
+ * C ABI of the MI355X ORB library (liborb_amd.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory on the
+ * context's device; every call is asynchronous on the context's stream unless
+ * it says otherwise.  Every entry point returns ORB_OK or a negative ORB_E*
+ * code; orb_last_error() gives the message.  A context is not thread-safe:
+ * calls on one context must be serialised by the caller.
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * Accustomer/CUDA-ORB).  The reference keeps its tables in module-global
+ * __constant__ memory (orbd.cu:32-41); here they live in the context.
+ */
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "orb_structures.h"
+
+#ifdef __cplusplus
+extern "C" {
+#define ORB_POINT orb::OrbPoint
+#else
+#define ORB_POINT struct OrbPoint
+#endif
+
+#define ORB_OK 0
+#define ORB_E_ARG -1      /* invalid argument (null pointer, size, parameter) */
+#define ORB_E_STATE -2    /* orb_init not called, or wrong call order */
+#define ORB_E_HIP -3      /* HIP runtime error */
+#define ORB_E_NOMEM -4    /* device allocation failed */
+#define ORB_E_CAPACITY -5 /* a size exceeds a compiled-in or configured limit */
+
+#define ORB_MAX_LEVELS 8
+
+/* Detector parameters: the arguments of Orbor::init (orb.h:18-19) plus the
+   pyramid scale factor (2.0 = the reference's nearest-neighbour octaves). */
+typedef struct {
+    int noctaves;         /* pyramid levels requested (orb.h:36) */
+    int edge_threshold;   /* border without keypoints (orb.h:40) */
+    int wta_k;            /* 2, 3 or 4 (orb.h:42) */
+    int score_type;       /* HARRIS_SCORE or FAST_SCORE (orb.h:47) */
+    int patch_size;       /* 31 (orb.h:49) */
+    int fast_threshold;   /* FAST intensity threshold (orb.h:51) */
+    int retain_topn;      /* stored, unused, as in the reference (orb.cpp:41) */
+    int max_pts;          /* keypoints kept per image (orb.h:55) */
+    double scale_factor;  /* 2.0: reference octaves; (1,2): bilinear levels */
+} orb_params;
+
+typedef struct orb_ctx orb_ctx;
+
+/* Create a context on HIP device `device`, using the device's null stream.
+   Replaces the implicit global state of orbd.cu:32-41 and Orbor() (orb.cpp:14). */
+int orb_ctx_create(orb_ctx** ctx, int device);
+
+/* Free a context and its device memory (Orbor::~Orbor, orb.cpp:19-29). */
+int orb_ctx_destroy(orb_ctx* ctx);
+
+/* Stream for all later asynchronous calls (hipStream_t as void*; NULL = null
+   stream).  No counterpart in the reference, which only uses stream 0. */
+int orb_set_stream(orb_ctx* ctx, void* stream);
+
+/* Upload parameters and tables: replaces setMaxNumPoints, setFastThresholdLUT,
+   setUmax, setPattern, setGaussianKernel, setScaleSqSq (orbd.h:10-30,
+   orbd.cu:45-441) as called by Orbor::init (orb.cpp:32-54). */
+int orb_init(orb_ctx* ctx, const orb_params* params);
+
+/* Number of pyramid levels the geometry rule gives for a w x h image
+   (orb.cpp:148 for scale 2).  Host only. */
+int orb_num_levels(const orb_ctx* ctx, int width, int height);
+
+/* Detect, orient and describe one image (orb.cpp:57-101, orbd.cu:1564-1783).
+   d_img: width x height uint8, row pitch `pitch` (the reference requires
+   alignUp(width,128), orb.cpp:162).  Writes up to max_pts points to d_pts,
+   32-byte descriptors to d_desc (may be NULL when compute_desc is 0) and the
+   point count to *d_count (a device int).  Points are in raster order; if more
+   survive NMS, the max_pts with the highest score are kept.  Asynchronous. */
+int orb_detect_and_compute(orb_ctx* ctx, const uint8_t* d_img, int width, int height, int pitch,
+                           ORB_POINT* d_pts, uint8_t* d_desc, int* d_count, int compute_desc);
+
+/* The same for `nframes` images of any sizes in one launch per stage.
+   Host arrays of length nframes hold the per-frame device pointers and sizes;
+   d_counts is a device array of nframes ints; frame f writes up to max_pts
+   points at d_pts[f] and descriptors at d_desc[f].  No counterpart in the
+   reference (one image per call there). */
+int orb_detect_and_compute_batch(orb_ctx* ctx, int nframes, const uint8_t* const* d_imgs,
+                                 const int* widths, const int* heights, const int* pitches,
+                                 ORB_POINT* const* d_pts, uint8_t* const* d_desc, int* d_counts,
+                                 int compute_desc);
+
+/* Reference match rule (orbd.cu:1471-1559, orb.cpp:104-113): for each of the
+   n1 points of d_pts1 the best and second-best Hamming distance over d_desc2;
+   match/distance = (best index, best distance) iff best < second && best < 64,
+   else -1/-1.  Counts are host ints (n1 == 0 is a no-op).  Asynchronous. */
+int orb_match(orb_ctx* ctx, ORB_POINT* d_pts1, const uint8_t* d_desc1, int n1,
+              const uint8_t* d_desc2, int n2);
+
+/* orb_match with the two counts read from device ints (as written by
+   orb_detect_and_compute), so detection and matching need no host round trip;
+   cap1/cap2 bound the counts. */
+int orb_match_dcount(orb_ctx* ctx, ORB_POINT* d_pts1, const uint8_t* d_desc1, const int* d_n1,
+                     int cap1, const uint8_t* d_desc2, const int* d_n2, int cap2);
+
+#define ORB_MATCH_POPCOUNT 0 /* VALU xor + popcount */
+#define ORB_MATCH_MFMA 1     /* +/-1 INT8 contraction on the matrix cores */
+
+/* Exact brute-force nearest and second-nearest neighbours of nq query
+   descriptors in a database of ndb (both 32-byte rows).  Ties go to the
+   lowest index: (d1, idx1) is the minimum of (distance, index), (d2, idx2)
+   the minimum over the other entries; with no entry idx = -1, d = 257.
+   Any output pointer may be NULL.  Generalises gHammingMatch (orbd.cu:1471)
+   to raw top-2 output. */
+int orb_match_top2(orb_ctx* ctx, const uint8_t* d_query, int nq, const uint8_t* d_db, int ndb,
+                   int method, int* d_idx1, int* d_d1, int* d_idx2, int* d_d2);
+
+/* Wait for all work on the context's stream (the reference synchronises after
+   every launch, orbd.cu:1598-1791). */
+int orb_sync(orb_ctx* ctx);
+
+/* Last error message of the context (or of context creation when ctx is NULL). */
+const char* orb_last_error(const orb_ctx* ctx);
+
+/* Device scratch bytes the context currently holds (pyramids, maps, lists). */
+size_t orb_workspace_bytes(const orb_ctx* ctx);
+
+/* Seeded synthetic test images (host memory, no GPU).  Image A: `nshapes`
+   filled rectangles/ellipses of intensity U[0,255] plus 3-octave value noise
+   of amplitude 30.  orb_synth_warp: A warped by a similarity homography
+   (rotation U[-15,15] deg, scale U[0.9,1.1], shift U[-20,20] px) plus N(0,3^2)
+   noise; H (row-major 3x3, maps A coordinates to B) is written if non-NULL.
+   Both write width x height bytes with row pitch `pitch`. */
+int orb_synth_image(uint64_t seed, int width, int height, int pitch, int nshapes, uint8_t* out);
+int orb_synth_warp(uint64_t seed, const uint8_t* src, int width, int height, int pitch,
+                   uint8_t* out, double* H);
+
+/* Stage timing for measurement (no counterpart in the reference, whose
+   GpuTimer wraps whole API calls, cuda_utils.h:81-108).  While enabled, each
+   launch of a stage in stage_mask records a pair of HIP events on the
+   context's stream; orb_profile_read synchronises, returns the summed
+   milliseconds and launch counts per stage, and resets them. */
+#define ORB_NUM_STAGES 8
+#define ORB_STAGE_PYRAMID 0    /* raw + blurred levels */
+#define ORB_STAGE_DETECT 1     /* FAST-9 + score into the key map */
+#define ORB_STAGE_NMS 2        /* cross-level NMS + segment compaction */
+#define ORB_STAGE_SELECT 3     /* per-frame scan / top-max_pts select */
+#define ORB_STAGE_EMIT 4       /* keypoints in raster order */
+#define ORB_STAGE_DESCRIBE 5   /* orientation + rBRIEF */
+#define ORB_STAGE_MATCH 6      /* brute-force matcher main kernel */
+#define ORB_STAGE_MATCH_MERGE 7
+int orb_profile_enable(orb_ctx* ctx, unsigned stage_mask);
+int orb_profile_read(orb_ctx* ctx, double* ms_sum, int* counts);
+
+/* Device self-checks of the math helpers, for tests: n angles in d_in give
+   sin, cos (own polynomials) and n (y, x) pairs give the centroid atan2. */
+int orb_debug_sincos(orb_ctx* ctx, const float* d_in, int n, float* d_sin, float* d_cos);
+int orb_debug_atan2(orb_ctx* ctx, const float* d_yx, int n, float* d_out);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,197 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on
+the same seeded inputs.  Integer, index and byte outputs must be identical;
+the float outputs (Harris score, angle) are defined by the same IEEE
+operation sequence on both sides and must be bit-identical too."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs an MI355X")]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    return t
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import orb_amd
+    c = orb_amd.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_detect(torch, ctx, img_p, w, h, max_pts, compute_desc=True):
+    import orb_amd
+    dimg = torch.from_numpy(np.ascontiguousarray(img_p)).cuda()
+    pts = ctx.alloc_points(max_pts)
+    desc = ctx.alloc_desc(max_pts)
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.detect_and_compute(dimg, w, h, pts, desc, cnt, compute_desc)
+    torch.cuda.synchronize()
+    n = int(cnt.item())
+    return orb_amd.points_to_numpy(pts, n), desc[:n].cpu().numpy()
+
+
+def assert_same_points(got, want):
+    assert len(got) == len(want)
+    for f in ("x", "y", "octave"):
+        assert np.array_equal(got[f], want[f]), f
+    assert np.array_equal(got["score"].view(np.uint32), want["score"].view(np.uint32)), "score bits"
+    assert np.array_equal(got["angle"].view(np.uint32), want["angle"].view(np.uint32)), "angle bits"
+
+
+CASES = [
+    # w, h, nshapes, seed, noctaves, scale, wta_k, score_type, max_pts
+    (640, 480, 40, 0, 8, 1.2, 2, 0, 1000),
+    (640, 480, 40, 0, 5, 2.0, 2, 0, 1000),
+    (1920, 1080, 80, 1, 8, 1.2, 2, 0, 10000),
+    (1920, 1080, 80, 1, 5, 2.0, 4, 0, 10000),
+    (333, 257, 8, 7, 8, 1.2, 3, 0, 500),
+    (640, 480, 40, 2, 8, 1.2, 4, 1, 1000),
+    (640, 480, 40, 3, 8, 1.2, 2, 0, 150),      # cap well below the survivors: exact selection
+    (1280, 720, 120, 3, 8, 1.25, 2, 0, 5000),
+]
+
+
+@pytest.mark.parametrize("w,h,nshapes,seed,nl,scale,wta,st,cap", CASES)
+def test_detect_and_compute_matches_oracle(torch, ctx, oracle, w, h, nshapes, seed, nl, scale, wta, st, cap):
+    import orb_amd
+    print(f"case {w}x{h} seed={seed} levels={nl} scale={scale} wta={wta} score={st} cap={cap}", flush=True)
+    img = orb_amd.synth_image(seed, w, h, nshapes)
+    ctx.init(noctaves=nl, wta_k=wta, score_type=st, max_pts=cap, scale_factor=scale)
+    got_p, got_d = gpu_detect(torch, ctx, img, w, h, cap)
+    want_p, want_d, nsurv = oracle.detect_and_compute(img[:, :w].copy(), noctaves=nl, scale=scale, wta_k=wta,
+                                                      score_type=st, max_pts=cap)
+    assert len(want_p) > 0
+    assert_same_points(got_p, want_p)
+    assert np.array_equal(got_d, want_d)
+    # the same call again (next epoch of the key map) gives the same bytes
+    got_p2, got_d2 = gpu_detect(torch, ctx, img, w, h, cap)
+    assert_same_points(got_p2, want_p)
+    assert np.array_equal(got_d2, want_d)
+
+
+def test_input_padding_bytes_follow_the_flat_layout(torch, ctx, oracle):
+    """Level-0 row padding is copied from the input (orbd.cu:1584), and reads that cross level
+    boundaries see it; fill the padding with noise and compare."""
+    import orb_amd
+    w, h = 600, 400
+    img = orb_amd.synth_image(5, w, h, 30)
+    img[:, w:] = np.random.default_rng(1).integers(0, 256, (h, img.shape[1] - w), dtype=np.uint8)
+    ctx.init(noctaves=5, max_pts=2000, scale_factor=2.0)
+    got_p, got_d = gpu_detect(torch, ctx, img, w, h, 2000)
+    # the oracle takes the same pitched bytes
+    import ctypes
+    prm = oracle.Params(5, 31, 2, 0, 31, 20, 2000, 2.0)
+    pts = np.zeros(2000, oracle.POINT_DTYPE)
+    desc = np.zeros((2000, 32), np.uint8)
+    n = ctypes.c_int()
+    ns = ctypes.c_int()
+    oracle.lib().orc_detect_and_compute(ctypes.byref(prm), oracle.ptr(img), w, h, oracle.ptr(pts),
+                                        oracle.ptr(desc), ctypes.byref(n), ctypes.byref(ns))
+    assert_same_points(got_p, pts[: n.value])
+    assert np.array_equal(got_d, desc[: n.value])
+
+
+def test_constant_image_gives_no_points(torch, ctx):
+    ctx.init(noctaves=8, max_pts=100, scale_factor=1.2)
+    img = np.full((480, 640), 77, np.uint8)
+    p, d = gpu_detect(torch, ctx, img, 640, 480, 100)
+    assert len(p) == 0
+
+
+def test_batch_matches_per_frame_oracle(torch, ctx, oracle):
+    import orb_amd
+    sizes = [(640, 480), (1280, 720), (333, 257), (1920, 1080), (640, 480)]
+    cap = 3000
+    ctx.init(noctaves=8, max_pts=cap, scale_factor=1.2)
+    imgs, hosts = [], []
+    for i, (w, h) in enumerate(sizes):
+        a = orb_amd.synth_image(20 + i, w, h, max(8, w * h * 40 // 307200))
+        hosts.append(a)
+        imgs.append(torch.from_numpy(a).cuda())
+    pts = ctx.alloc_points(cap, frames=len(sizes))
+    desc = ctx.alloc_desc(cap, frames=len(sizes))
+    cnt = torch.zeros(len(sizes), dtype=torch.int32, device="cuda")
+    for rep in range(2):
+        ctx.detect_and_compute_batch(imgs, [s[0] for s in sizes], [s[1] for s in sizes],
+                                     [pts[i] for i in range(len(sizes))], [desc[i] for i in range(len(sizes))], cnt)
+        torch.cuda.synchronize()
+        for i, (w, h) in enumerate(sizes):
+            print(f"batch frame {i} {w}x{h} rep {rep}", flush=True)
+            n = int(cnt[i].item())
+            want_p, want_d, _ = oracle.detect_and_compute(hosts[i][:, :w].copy(), noctaves=8, scale=1.2,
+                                                          max_pts=cap)
+            assert_same_points(orb_amd.points_to_numpy(pts[i], n), want_p)
+            assert np.array_equal(desc[i][:n].cpu().numpy(), want_d)
+
+
+def test_match_reference_rule_and_device_counts(torch, ctx, oracle):
+    import orb_amd
+    w, h = 640, 480
+    a = orb_amd.synth_image(0, w, h, 40)
+    b, _ = orb_amd.synth_warp(0, a, w, h)
+    ctx.init(noctaves=8, max_pts=1000, scale_factor=1.2)
+    pa, da = gpu_detect(torch, ctx, a, w, h, 1000)
+    pb, db = gpu_detect(torch, ctx, b, w, h, 1000)
+    want_m, want_dist = oracle.match(da, db)
+    pts1 = torch.from_numpy(pa.view(np.int32).reshape(-1, 7).copy()).cuda()
+    ctx.match(pts1, torch.from_numpy(da).cuda(), len(pa), torch.from_numpy(db).cuda(), len(pb))
+    got = orb_amd.points_to_numpy(pts1, len(pa))
+    assert np.array_equal(got["match"], want_m) and np.array_equal(got["distance"], want_dist)
+    assert (want_m >= 0).sum() > 100
+    # device-count variant, counts larger buffers than used
+    cap = 1500
+    pts1b = torch.zeros((cap, 7), dtype=torch.int32, device="cuda")
+    pts1b[: len(pa)] = pts1
+    d1 = torch.zeros((cap, 32), dtype=torch.uint8, device="cuda"); d1[: len(pa)] = torch.from_numpy(da).cuda()
+    d2 = torch.zeros((cap, 32), dtype=torch.uint8, device="cuda"); d2[: len(pb)] = torch.from_numpy(db).cuda()
+    n1 = torch.tensor([len(pa)], dtype=torch.int32, device="cuda")
+    n2 = torch.tensor([len(pb)], dtype=torch.int32, device="cuda")
+    ctx.match_dcount(pts1b, d1, n1, cap, d2, n2, cap)
+    got = orb_amd.points_to_numpy(pts1b, len(pa))
+    assert np.array_equal(got["match"], want_m) and np.array_equal(got["distance"], want_dist)
+
+
+@pytest.mark.parametrize("nq,ndb", [(1, 1), (7, 300), (1000, 4097), (300, 70000), (5000, 0)])
+def test_match_top2_popcount_matches_oracle(torch, ctx, oracle, nq, ndb):
+    print(f"top2 nq={nq} ndb={ndb}", flush=True)
+    rng = np.random.default_rng(nq * 7 + ndb)
+    db = rng.integers(0, 256, (ndb, 32), dtype=np.uint8)
+    if ndb > 10:
+        db[ndb // 2] = db[3]        # duplicates: ties go to the lowest index
+        db[ndb - 1] = db[3]
+    q = rng.integers(0, 256, (nq, 32), dtype=np.uint8)
+    if ndb > 10 and nq > 3:
+        q[0] = db[3]
+        q[1] = db[ndb - 1]
+        q[2] = db[5] ^ 1
+    out = ctx.match_top2(torch.from_numpy(q).cuda(), torch.from_numpy(db).cuda())
+    torch.cuda.synchronize()
+    got = [o.cpu().numpy() for o in out]
+    want = oracle.match_top2(q, db, nthreads=8)
+    for g, w_ in zip(got, want):
+        assert np.array_equal(g, w_)
+
+
+def test_device_sincos_and_atan2_bit_exact(torch, ctx, oracle):
+    a = np.concatenate([np.linspace(-math.pi, math.pi, 40001, dtype=np.float32),
+                        np.array([0.0, -0.0, math.pi / 2, -math.pi / 2, math.pi / 4], np.float32)])
+    s, c = ctx.debug_sincos(torch.from_numpy(a).cuda())
+    torch.cuda.synchronize()
+    want = np.array([oracle.sincosf(float(v)) for v in a], np.float32)
+    assert np.array_equal(s.cpu().numpy().view(np.uint32), want[:, 0].view(np.uint32))
+    assert np.array_equal(c.cpu().numpy().view(np.uint32), want[:, 1].view(np.uint32))
+    rng = np.random.default_rng(3)
+    yx = rng.integers(-3_000_000, 3_000_000, (20000, 2)).astype(np.float32)
+    yx[:8] = [[0, 5], [5, 0], [0, -5], [-5, 0], [3, 3], [-3, -3], [7, -7], [1, 2]]
+    got = ctx.debug_atan2(torch.from_numpy(yx).cuda()).cpu().numpy()
+    want = np.array([oracle.fast_atan2(float(y), float(x)) for y, x in yx], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
