@@ -63,6 +63,87 @@ def stage_bytes(stage, frames, npts):
     return tot
 
 
+def dense_match(ctx, torch, dev, reps=3):
+    """65536 queries against 262144 random 256-bit descriptors (queries = database rows with
+    Binomial(256, 0.05) bit flips): INT8-MFMA and popcount matchers, timed with events on the
+    library's stream, outputs compared element for element."""
+    import orb_amd
+    rng = np.random.default_rng(2)
+    ndb, nq = 262144, 65536
+    db = rng.integers(0, 256, (ndb, 32), dtype=np.uint8)
+    src = rng.integers(0, ndb, nq)
+    flips = np.packbits(rng.random((nq, 256)) < 0.05, axis=1, bitorder="little")
+    q = db[src] ^ flips
+    dq, ddb = torch.from_numpy(q).to(dev), torch.from_numpy(db).to(dev)
+    res = {}
+    outs = {}
+    for name, method in (("popcount", orb_amd.ORB_MATCH_POPCOUNT), ("mfma", orb_amd.ORB_MATCH_MFMA)):
+        out = ctx.match_top2(dq, ddb, method=method)
+        torch.cuda.synchronize()
+        ctx.profile_enable((1 << orb_amd.STAGES.index("match")) | (1 << orb_amd.STAGES.index("match_merge")))
+        for _ in range(reps):
+            ctx.match_top2(dq, ddb, method=method, out=out)
+        prof = ctx.profile_read()
+        ctx.profile_enable(0)
+        ms_main = prof["match"][0] / prof["match"][1]
+        ms = ms_main + prof["match_merge"][0] / prof["match_merge"][1]
+        outs[name] = [o.cpu().numpy() for o in out]
+        res[name] = {"ms": round(ms, 4), "gpairs_per_s": round(nq * ndb / ms / 1e6, 1),
+                     "kernel_ms": round(ms_main, 4),
+                     "int_tops_equiv": round(512.0 * nq * ndb / (ms_main * 1e-3) / 1e12, 1)}
+    identical = all(np.array_equal(a, b) for a, b in zip(outs["popcount"], outs["mfma"]))
+    recall_src = float((outs["mfma"][0] == src).mean())
+    return {"queries": nq, "database": ndb, "popcount": res["popcount"], "mfma": res["mfma"],
+            "mfma_speedup": round(res["popcount"]["ms"] / res["mfma"]["ms"], 3),
+            "identical": identical, "idx1_is_source": round(recall_src, 5),
+            "mfma_frac_of_int8_peak": round(res["mfma"]["int_tops_equiv"] / 5000.0, 4)}
+
+
+def batch_frames(ctx, torch, dev, reps=5):
+    """64 frames of sizes drawn from {640x480, 1280x720, 1920x1080, 3840x2160} (shape count scaled to
+    area), detected and described in one launch per stage, each matched to a warped partner (the
+    partners are the second half of a 128-frame batch); <= 5000 keypoints per frame."""
+    import orb_amd
+    rng = np.random.default_rng(3)
+    sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    pick = [sizes[i] for i in rng.integers(0, 4, 64)]
+    imgs = []
+    for i, (w, h) in enumerate(pick):
+        a = orb_amd.synth_image(300 + i, w, h, max(1, int(40 * w * h / 307200)))
+        b, _ = orb_amd.synth_warp(300 + i, a, w, h)
+        imgs.append((a, b))
+    frames = [torch.from_numpy(a).to(dev) for a, _ in imgs] + [torch.from_numpy(b).to(dev) for _, b in imgs]
+    ws = [w for w, _ in pick] * 2
+    hs = [h for _, h in pick] * 2
+    cap = 5000
+    ctx.init(noctaves=LEVELS, edge_threshold=31, wta_k=2, score_type=0, patch_size=31, fast_threshold=20,
+             max_pts=cap, scale_factor=SCALE)
+    pts = ctx.alloc_points(cap, frames=128)
+    desc = ctx.alloc_desc(cap, frames=128)
+    counts = torch.zeros(128, dtype=torch.int32, device=dev)
+    pl = [pts[i] for i in range(128)]
+    dl = [desc[i] for i in range(128)]
+
+    def step():
+        ctx.detect_and_compute_batch(frames, ws, hs, pl, dl, counts)
+        for i in range(64):
+            ctx.match_dcount(pts[i], desc[i], counts[i:i + 1], cap, desc[64 + i], counts[64 + i:65 + i], cap)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    kp = counts.cpu().numpy()
+    ctx.init(noctaves=LEVELS, edge_threshold=31, wta_k=2, score_type=0, patch_size=31, fast_threshold=20,
+             max_pts=CAP, scale_factor=SCALE)
+    return {"frames": 128, "pairs": 64, "ms_per_batch": round(dt * 1e3, 3),
+            "frames_per_s": round(128 / dt, 1), "mean_keypoints": round(float(kp.mean()), 1),
+            "megapixels": round(sum(w * h for w, h in zip(ws, hs)) / 1e6, 2)}
+
+
 def cpu_baseline(a, b, min_seconds=10.0):
     """The scalar C oracle on the host: the same step (detect+describe both images, match them),
     repeated until about min_seconds of CPU work."""
@@ -88,6 +169,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the dense-matching and batched-frames measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 64 MiB elementwise copy kernel first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -179,6 +262,9 @@ def main():
         if stage_ms["match"] > 0 else None,
         "roofline": roof,
     }
+    if not args.no_extras:
+        out["dense_match"] = dense_match(ctx, torch, dev)
+        out["batch64"] = batch_frames(ctx, torch, dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a, b)
     ctx.close()

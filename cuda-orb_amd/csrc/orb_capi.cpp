@@ -125,9 +125,6 @@ struct FrameSet {
     int nms_blocks = 0;
     Candidate* d_cand = nullptr;
     int* d_blk_count = nullptr;
-    int* d_blk_out = nullptr;
-    int* d_blk_eq = nullptr;
-    SelectState* d_sel = nullptr;
     uint32_t* d_epochs = nullptr;
     int* d_tabs = nullptr;
 };
@@ -260,8 +257,8 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     const int nf = (int)ws.size();
     std::vector<Geom> geo(nf);
     for (int f = 0; f < nf; f++) {
-        if (ws[f] > 16384 || hs[f] > 16384)
-            return fail(c, ORB_E_CAPACITY, "frame %d: %dx%d exceeds 16384", f, ws[f], hs[f]);
+        if (ws[f] > 16383 || hs[f] > 16383)
+            return fail(c, ORB_E_CAPACITY, "frame %d: %dx%d exceeds 16383", f, ws[f], hs[f]);
         if (!compute_geom(ws[f], hs[f], P.noctaves, P.scale_factor, geo[f]))
             return fail(c, ORB_E_ARG, "frame %d: %dx%d too small for the pyramid (scale %g)", f, ws[f], hs[f],
                         P.scale_factor);
@@ -303,14 +300,16 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
             pyr_jobs.push_back(Job{f, l, ptx, pyr_blocks});
             pyr_blocks += ptx * pty;
             if (g.w[l] > 6 && g.h[l] > 6) {
-                const int dtx = (g.w[l] - 6 + kDetTW - 1) / kDetTW, dty = (g.h[l] - 6 + kDetTH - 1) / kDetTH;
+                /* tiles cover [0, w-3) x [0, h-3); pixels outside [3, w-3) x [3, h-3) are masked */
+                const int dtx = (g.w[l] - 3 + kDetTW - 1) / kDetTW, dty = (g.h[l] - 3 + kDetTH - 1) / kDetTH;
                 det_jobs.push_back(Job{f, l, dtx, det_blocks});
                 det_blocks += dtx * dty;
             }
         }
         /* flat-pyramid guard: covers the deepest stencil / patch overhang
-           (Harris +-4, describe +-18 rows/cols) beyond either end */
-        guard[f] = 32 * g.p[0];
+           beyond either end (Harris +-4, describe +-18 rows/cols, and the
+           detection tile staged up to kDetTH + 4 rows past a level's end) */
+        guard[f] = (kDetTH + 20) * g.p[0];
         pyr_off[f] = pyr_bytes;
         pyr_bytes += round256((size_t)g.total + 2 * (size_t)guard[f]);
         key_guard_rows[f] = rmax + 1;
@@ -318,8 +317,9 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
         key_bytes += round256(sizeof(unsigned long long) * (size_t)(g.h[0] + 2 * key_guard_rows[f]) * g.p[0]);
         const int rows = g.h[0] - 2 * border, span = g.w[0] - 2 * border;
         F.nms_first = nms_blocks;
+        F.nms_x0 = border & ~3; /* segments start dword-aligned; pixels left of the border are masked */
         F.nms_rows = rows > 0 && span > 0 ? rows : 0;
-        F.nms_segs = rows > 0 && span > 0 ? (span + kNmsSeg - 1) / kNmsSeg : 0;
+        F.nms_segs = rows > 0 && span > 0 ? (g.w[0] - border - F.nms_x0 + kNmsSeg - 1) / kNmsSeg : 0;
         for (int b = 0; b < F.nms_rows * F.nms_segs; b++) nms_frame.push_back(f);
         nms_blocks += F.nms_rows * F.nms_segs;
     }
@@ -330,11 +330,8 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     const size_t o_pyr_jobs = carve(sizeof(Job) * pyr_jobs.size());
     const size_t o_det_jobs = carve(sizeof(Job) * det_jobs.size());
     const size_t o_nms_frame = carve(sizeof(int) * nms_frame.size());
-    const size_t o_cand = carve(sizeof(Candidate) * (size_t)nms_blocks * (kNmsSeg / 2));
+    const size_t o_cand = carve(sizeof(Candidate) * (size_t)nms_blocks * kNmsCap);
     const size_t o_cnt = carve(sizeof(int) * nms_blocks);
-    const size_t o_out = carve(sizeof(int) * nms_blocks);
-    const size_t o_eq = carve(sizeof(int) * nms_blocks);
-    const size_t o_sel = carve(sizeof(SelectState) * nf);
     const size_t o_epochs = carve(sizeof(uint32_t) * nf);
     const size_t o_tabs = carve(sizeof(int) * (size_t)tab_ints);
     const size_t o_pyr = carve(2 * pyr_bytes);
@@ -353,9 +350,6 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     fs.d_nms_frame = (int*)(base + o_nms_frame);
     fs.d_cand = (Candidate*)(base + o_cand);
     fs.d_blk_count = (int*)(base + o_cnt);
-    fs.d_blk_out = (int*)(base + o_out);
-    fs.d_blk_eq = (int*)(base + o_eq);
-    fs.d_sel = (SelectState*)(base + o_sel);
     fs.d_epochs = (uint32_t*)(base + o_epochs);
     fs.d_tabs = (int*)(base + o_tabs);
     uint8_t* pyr = (uint8_t*)(base + o_pyr);
@@ -493,15 +487,14 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     d.nms_block_frame = fs.d_nms_frame;
     d.cand = fs.d_cand;
     d.blk_count = fs.d_blk_count;
-    d.blk_out = fs.d_blk_out;
-    d.blk_eq = fs.d_blk_eq;
-    d.sel = fs.d_sel;
     d.epochs = fs.d_epochs;
-    d.desc_blocks_per_frame = compute_desc ? (c->prm.max_pts + kDescPerBlock - 1) / kDescPerBlock : 0;
+    /* describe: a fixed number of workgroups per frame loop over the points */
+    const int need_blocks = (c->prm.max_pts + kDescPerBlock - 1) / kDescPerBlock;
+    d.desc_blocks_per_frame = compute_desc ? (need_blocks < kDescMaxBlocks ? need_blocks : kDescMaxBlocks) : 0;
     d.t = make_tables(c);
     StageEvents ev;
     memset(&ev, 0, sizeof(ev));
-    hipError_t e = launch_detect(d, compute_desc, c->stream, take_events(c, ev, 0x3fu));
+    hipError_t e = launch_detect(d, compute_desc, c->stream, take_events(c, ev, 0x2fu));
     if (e != hipSuccess) return hip_fail(c, e, "detect launch");
     return ORB_OK;
 }
@@ -553,6 +546,25 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
 }
 
 } // namespace
+
+/* helpers for the learned-index host code (orb_lindex.cpp) */
+int orb__ctx_stream(orb_ctx* c, void** stream)
+{
+    if (!c) return ORB_E_ARG;
+    *stream = (void*)c->stream;
+    return ORB_OK;
+}
+
+int orb__ctx_fail(orb_ctx* c, int code, const char* msg)
+{
+    return fail(c, code, "%s", msg);
+}
+
+int orb__match_dcount_top2(orb_ctx* c, const uint8_t* q, int nq_cap, const int* d_nq, const uint8_t* db, int ndb,
+                           int* idx1, int* d1, int* idx2, int* d2)
+{
+    return run_match(c, q, nq_cap, d_nq, db, ndb, nullptr, ORB_MATCH_MFMA, idx1, d1, idx2, d2, nullptr);
+}
 
 extern "C" {
 
@@ -698,7 +710,7 @@ size_t orb_workspace_bytes(const orb_ctx* c)
 int orb_profile_enable(orb_ctx* c, unsigned stage_mask)
 {
     if (!c) return fail(nullptr, ORB_E_ARG, "null context");
-    c->prof_mask = stage_mask & ((1u << ORB_NUM_STAGES) - 1u);
+    c->prof_mask = stage_mask & ((1u << ORB_NUM_STAGES) - 1u) & ~(1u << 4); /* stage 4 is unused */
     return ORB_OK;
 }
 
@@ -720,6 +732,7 @@ int orb_profile_read(orb_ctx* c, double* ms_sum, int* counts)
         c->ev_pool.push_back(r.b);
     }
     c->recs.clear();
+    (void)hipGetLastError(); /* an unrecorded pair leaves an error that is not the caller's */
     return ORB_OK;
 }
 

@@ -33,12 +33,16 @@ constexpr uint32_t kEpochLimit = 1u << 30;
 
 /* pyramid tile: 64 x 16 outputs, 256 threads */
 constexpr int kPyrTW = 64, kPyrTH = 16;
-/* detection tile: 64 x 4 pixels, 256 threads */
-constexpr int kDetTW = 64, kDetTH = 4;
-/* NMS block: one level-0 row segment of 256 pixels */
-constexpr int kNmsSeg = 256;
-/* describe: one wavefront per keypoint, 4 per workgroup */
+/* detection tile: 64 x 16 pixels, 256 threads */
+constexpr int kDetTW = 64, kDetTH = 16;
+/* NMS block: one level-0 row segment of 1024 pixels, 4 per thread; at most
+   every other pixel of a row survives, so 512 candidates fit a segment */
+constexpr int kNmsSeg = 1024;
+constexpr int kNmsCap = kNmsSeg / 2;
+/* describe: one wavefront per keypoint, 4 per workgroup, at most
+   kDescMaxBlocks workgroups per frame looping over the points */
 constexpr int kDescPerBlock = 4;
+constexpr int kDescMaxBlocks = 512;
 
 struct FrameDev {
     /* per call */
@@ -59,8 +63,8 @@ struct FrameDev {
     uint8_t* raw;              /* level-0 start of the raw flat pyramid */
     uint8_t* blur;             /* level-0 start of the blurred flat pyramid */
     unsigned long long* keys;  /* row 0 of the key map */
-    /* NMS blocks */
-    int nms_first, nms_rows, nms_segs;
+    /* NMS blocks: rows [border, h0-border), segments of kNmsSeg pixels from nms_x0 */
+    int nms_first, nms_rows, nms_segs, nms_x0;
 };
 
 /* A contiguous range of workgroups covering one (frame, level) region. */
@@ -68,19 +72,10 @@ struct Job {
     int frame, level, tiles_x, first_block;
 };
 
-/* Per-frame selection state written by k_select, read by k_emit. */
-struct SelectState {
-    uint32_t thr;   /* score bits of the max_pts-th best (0: keep all) */
-    int need;       /* how many keys equal to thr are kept, in raster order */
-    uint32_t epoch; /* epoch of the current detection */
-    int pad;
-};
-
+/* NMS survivor: pos = level << 28 | y << 14 | x (level-0 coordinates), score = float bits */
 struct Candidate {
-    uint32_t xy;    /* y << 16 | x, level-0 */
-    uint32_t score; /* float bits */
-    uint32_t level;
-    uint32_t pad;
+    uint32_t pos;
+    uint32_t score;
 };
 
 struct Tables {
@@ -101,9 +96,6 @@ struct DetectLaunch {
     const int* nms_block_frame; /* frame of each NMS block */
     Candidate* cand;            /* kNmsSeg per NMS block */
     int* blk_count;             /* survivors per NMS block */
-    int* blk_out;               /* output offset per NMS block */
-    int* blk_eq;                /* keys equal to thr before the block */
-    SelectState* sel;           /* per frame */
     uint32_t* epochs;           /* per frame */
     int max_blocks_per_frame;
     int desc_blocks_per_frame;
@@ -111,7 +103,7 @@ struct DetectLaunch {
 };
 
 /* Stage ids for event timing (orb_profile_read). */
-enum Stage { kStPyramid = 0, kStDetect, kStNms, kStSelect, kStEmit, kStDescribe, kStMatch, kStMatchMerge,
+enum Stage { kStPyramid = 0, kStDetect, kStNms, kStSelect, kStUnused4, kStDescribe, kStMatch, kStMatchMerge,
              kNumStages };
 
 /* Optional start/stop events per stage, recorded around each launch. */

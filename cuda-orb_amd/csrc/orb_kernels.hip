@@ -6,11 +6,11 @@
  *               (orbd.cu:514-535 pyramid, orbd.cu:1273-1338 blur)
  *   k_detect    FAST-9 segment test + Harris / FAST score on every level,
  *               64-bit atomic max into the level-0 key map (orbd.cu:672-756)
- *   k_nms       square cross-level NMS, wavefront-ballot compaction of the
- *               survivors of each 256-pixel row segment (orbd.cu:919-983)
- *   k_select    per-frame scan of the segment counts and, when more than
- *               max_pts survive, an exact radix select of the score cut
- *   k_emit      survivors -> OrbPoint in raster order
+ *   k_nms       square cross-level NMS, scan compaction of the survivors of
+ *               each 1024-pixel row segment (orbd.cu:919-983)
+ *   k_select    per-frame scan of the segment counts, keypoints out in
+ *               raster order; with more than max_pts survivors, an exact
+ *               radix select of the score cut
  *   k_describe  intensity-centroid angle + rBRIEF, one wavefront per keypoint
  *               (orbd.cu:1001-1036, 1341-1468)
  * Every floating-point step is the explicit IEEE operation (__f*_rn) the CPU
@@ -136,15 +136,41 @@ __device__ __forceinline__ int wave_sum(int v)
 
 /* ---------------------------------------------------------------- pyramid */
 
+/* Workgroup b of n runs logical tile xcd_remap(b): the hardware deals
+   workgroups round-robin to the 8 XCDs, so this gives every XCD (and its L2)
+   one contiguous range of tiles, and vertically adjacent tiles, which share
+   halo rows, land on the same L2. */
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}
+
+/* staged level tile: rows y0-3 .. y0+TH+3, columns x0-4 .. x0+TW+4 */
+constexpr int kPyrSH = kPyrTH + 6, kPyrSW = kPyrTW + 8;
+/* LDS budget for the level-0 source rectangle of a bilinear tile */
+constexpr int kPyrSrcBytes = 20480;
+
+__device__ __forceinline__ int bilerp(int a, int b, int c, int d, int fx, int fy)
+{
+    const int top = (256 - fx) * a + fx * b;
+    const int bot = (256 - fx) * c + fx * d;
+    return ((256 - fy) * top + fy * bot + 32768) >> 16;
+}
+
 __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
                                                  int njobs, Tables T)
 {
-    __shared__ uint8_t s_raw[kPyrTH + 6][kPyrTW + 8];
-    __shared__ float s_h[kPyrTH + 6][kPyrTW + 1];
-    const Job jb = find_job(jobs, njobs, blockIdx.x);
+    __shared__ __align__(16) uint8_t s_raw[kPyrSH][kPyrSW];
+    __shared__ __align__(16) float s_h[kPyrSH][kPyrTW];
+    __shared__ __align__(16) uint8_t s_src[kPyrSrcBytes];
+    __shared__ int s_xt[kPyrSW], s_yt[kPyrSH];
+    __shared__ int s_rect[4];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const Job jb = find_job(jobs, njobs, b);
     const FrameDev& F = frames[jb.frame];
     const int l = jb.level;
-    const int t = blockIdx.x - jb.first_block;
+    const int t = b - jb.first_block;
     const int x0 = (t % jb.tiles_x) * kPyrTW, y0 = (t / jb.tiles_x) * kPyrTH;
     const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
     const int tid = threadIdx.x;
@@ -153,56 +179,147 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
     const bool inside = x0 < w;
 
     if (inside) {
-        for (int i = tid; i < (kPyrTH + 6) * (kPyrTW + 6); i += 256) {
-            const int r = i / (kPyrTW + 6), c = i - r * (kPyrTW + 6);
-            const int yy = reflect101(y0 - 3 + r, h), xx = reflect101(x0 - 3 + c, w);
-            s_raw[r][c] = (uint8_t)src_value(F, T.tabs, l, xx, yy);
+        if (l == 0 || F.ref_mode) {
+            /* level 0, or scale-2 nearest decimation: direct reads of the input */
+            const int sh = l;
+            const bool vec = l == 0 && (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0 && x0 >= 4 &&
+                             x0 + kPyrSW - 4 <= w;
+            for (int i = tid; i < kPyrSH * (kPyrSW / 4); i += 256) {
+                const int r = i / (kPyrSW / 4), c4 = 4 * (i - r * (kPyrSW / 4));
+                const int yy = reflect101(y0 - 3 + r, h);
+                const uint8_t* row = F.img + (long long)(yy << sh) * F.ip;
+                if (vec) {
+                    *(uint32_t*)&s_raw[r][c4] = *(const uint32_t*)(row + x0 - 4 + c4);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) s_raw[r][c4 + k] = row[reflect101(x0 - 4 + c4 + k, w) << sh];
+                }
+            }
+        } else {
+            /* bilinear level: per-tile source tables, then the level-0 source
+               rectangle staged in LDS with dword loads */
+            const int* tab = level_tab(F, T.tabs, l);
+            if (tid < 4) s_rect[tid] = (tid & 1) ? -1 : 0x7fffffff;
+            __syncthreads();
+            if (tid < kPyrSW) {
+                const int v = tab[reflect101(x0 - 4 + tid, w)];
+                s_xt[tid] = v;
+                atomicMin(&s_rect[0], v >> 8);
+                atomicMax(&s_rect[1], min((v >> 8) + 1, F.w0 - 1));
+            } else if (tid >= 128 && tid < 128 + kPyrSH) {
+                const int r = tid - 128;
+                const int v = tab[w + reflect101(y0 - 3 + r, h)];
+                s_yt[r] = v;
+                atomicMin(&s_rect[2], v >> 8);
+                atomicMax(&s_rect[3], min((v >> 8) + 1, F.h0 - 1));
+            }
+            __syncthreads();
+            const int sx0 = s_rect[0] & ~3, sx1 = s_rect[1], sy0 = s_rect[2], sy1 = s_rect[3];
+            const int sw = ((sx1 - sx0 + 4) & ~3), shh = sy1 - sy0 + 1;
+            const bool staged = sw * shh <= kPyrSrcBytes && (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
+            if (staged) {
+                const int dw = sw >> 2;
+                for (int i = tid; i < dw * shh; i += 256) {
+                    const int r = i / dw, c = i - r * dw;
+                    *(uint32_t*)&s_src[r * sw + 4 * c] =
+                        *(const uint32_t*)(F.img + (long long)(sy0 + r) * F.ip + sx0 + 4 * c);
+                }
+                __syncthreads();
+            }
+            for (int i = tid; i < kPyrSH * kPyrSW; i += 256) {
+                const int r = i / kPyrSW, c = i - r * kPyrSW;
+                const int tx = s_xt[c], ty = s_yt[r];
+                const int xa = tx >> 8, fx = tx & 255, ya = ty >> 8, fy = ty & 255;
+                const int xb = min(xa + 1, F.w0 - 1), yb = min(ya + 1, F.h0 - 1);
+                int a, bb, cc, d;
+                if (staged) {
+                    const uint8_t* r0 = s_src + (ya - sy0) * sw - sx0;
+                    const uint8_t* r1 = s_src + (yb - sy0) * sw - sx0;
+                    a = r0[xa]; bb = r0[xb]; cc = r1[xa]; d = r1[xb];
+                } else {
+                    const uint8_t* r0 = F.img + (long long)ya * F.ip;
+                    const uint8_t* r1 = F.img + (long long)yb * F.ip;
+                    a = r0[xa]; bb = r0[xb]; cc = r1[xa]; d = r1[xb];
+                }
+                s_raw[r][c] = (uint8_t)bilerp(a, bb, cc, d, fx, fy);
+            }
         }
     }
     __syncthreads();
-    for (int i = tid; i < kPyrTH * kPyrTW; i += 256) {
-        const int r = i / kPyrTW, c = i - r * kPyrTW;
-        const int y = y0 + r, x = x0 + c;
-        if (y < h && x < p) {
-            uint8_t v;
-            if (x < w) {
-                v = s_raw[r + 3][c + 3];
+    /* raw level (and padding) as dwords: thread -> row tid/16, columns 4*(tid%16) */
+    {
+        const int r = tid >> 4, c4 = 4 * (tid & 15);
+        const int y = y0 + r, x = x0 + c4;
+        if (y < h) {
+            uint32_t v;
+            if (x + 4 <= w) {
+                v = *(const uint32_t*)&s_raw[r + 3][c4 + 4];
             } else {
                 /* row padding: level 0 keeps the input's bytes, other levels 0 */
-                v = (l == 0 && x < F.ip) ? F.img[(long long)y * F.ip + x] : (uint8_t)0;
-                blur[(long long)y * p + x] = v;
+                v = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int xx = x + k;
+                    uint32_t byte;
+                    if (xx < w) byte = s_raw[r + 3][c4 + 4 + k];
+                    else byte = (l == 0 && xx < F.ip) ? F.img[(long long)y * F.ip + xx] : 0u;
+                    v |= byte << (8 * k);
+                }
             }
-            raw[(long long)y * p + x] = v;
+            *(uint32_t*)(raw + (long long)y * p + x) = v;
+            /* a tile wholly in the row padding: the blurred level keeps the same bytes */
+            if (!inside) *(uint32_t*)(blur + (long long)y * p + x) = v;
         }
     }
     if (!inside) return;
-    for (int i = tid; i < (kPyrTH + 6) * kPyrTW; i += 256) {
-        const int r = i / kPyrTW, c = i - r * kPyrTW;
-        const uint8_t* s = &s_raw[r][c + 3];
-        float v = __fmul_rn(T.k0, (float)s[0]);
-        v = __fmaf_rn(T.k1, (float)((int)s[-1] + (int)s[1]), v);
-        v = __fmaf_rn(T.k2, (float)((int)s[-2] + (int)s[2]), v);
-        v = __fmaf_rn(T.k3, (float)((int)s[-3] + (int)s[3]), v);
-        s_h[r][c] = v;
+    /* horizontal 7-tap pass on integer pair sums: 4 outputs per work item */
+    for (int i = tid; i < kPyrSH * (kPyrTW / 4); i += 256) {
+        const int r = i >> 4, c4 = 4 * (i & 15);
+        const uint8_t* sr = &s_raw[r][c4 + 1];   /* columns x0 + c4 - 3 .. x0 + c4 + 6 */
+        int q[10];
+#pragma unroll
+        for (int k = 0; k < 10; k++) q[k] = sr[k];
+        float4 o;
+        float* of = (float*)&o;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float v = __fmul_rn(T.k0, (float)q[k + 3]);
+            v = __fmaf_rn(T.k1, (float)(q[k + 2] + q[k + 4]), v);
+            v = __fmaf_rn(T.k2, (float)(q[k + 1] + q[k + 5]), v);
+            v = __fmaf_rn(T.k3, (float)(q[k] + q[k + 6]), v);
+            of[k] = v;
+        }
+        *(float4*)&s_h[r][c4] = o;
     }
     __syncthreads();
-    for (int i = tid; i < kPyrTH * kPyrTW; i += 256) {
-        const int r = i / kPyrTW, c = i - r * kPyrTW;
-        const int y = y0 + r, x = x0 + c;
-        if (y < h && x < w) {
-            float v = __fmul_rn(T.k0, s_h[r + 3][c]);
-            v = __fmaf_rn(T.k1, __fadd_rn(s_h[r + 2][c], s_h[r + 4][c]), v);
-            v = __fmaf_rn(T.k2, __fadd_rn(s_h[r + 1][c], s_h[r + 5][c]), v);
-            v = __fmaf_rn(T.k3, __fadd_rn(s_h[r][c], s_h[r + 6][c]), v);
-            blur[(long long)y * p + x] = (uint8_t)__float2int_rn(v);
+    /* vertical pass, round half to even, 4 pixels per thread written as a dword */
+    {
+        const int r = tid >> 4, c4 = 4 * (tid & 15);
+        const int y = y0 + r, x = x0 + c4;
+        if (y < h) {
+            float4 col[7];
+#pragma unroll
+            for (int k = 0; k < 7; k++) col[k] = *(const float4*)&s_h[r + k][c4];
+            const float* cf = (const float*)col;
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float a = __fmul_rn(T.k0, cf[4 * 3 + k]);
+                a = __fmaf_rn(T.k1, __fadd_rn(cf[4 * 2 + k], cf[4 * 4 + k]), a);
+                a = __fmaf_rn(T.k2, __fadd_rn(cf[4 * 1 + k], cf[4 * 5 + k]), a);
+                a = __fmaf_rn(T.k3, __fadd_rn(cf[4 * 0 + k], cf[4 * 6 + k]), a);
+                uint32_t byte = (uint32_t)(uint8_t)__float2int_rn(a);
+                /* row padding keeps the raw level's bytes */
+                if (x + k >= w) byte = (l == 0 && x + k < F.ip) ? F.img[(long long)y * F.ip + x + k] : 0u;
+                v |= byte << (8 * k);
+            }
+            *(uint32_t*)(blur + (long long)y * p + x) = v;
         }
     }
 }
 
 /* -------------------------------------------------------------- detection */
 
-__constant__ int c_circle_dx[16] = {0, 1, 2, 3, 3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1};
-__constant__ int c_circle_dy[16] = {3, 3, 2, 1, 0, -1, -2, -3, -3, -3, -2, -1, 0, 1, 2, 3};
 
 /* OpenCV cornerScore<16> as in orbd.cu:538-578 */
 __device__ float fast_score(const int* ring, int v, int threshold)
@@ -236,27 +353,46 @@ __device__ float fast_score(const int* ring, int v, int threshold)
     return (float)(-b0 - 1);
 }
 
-/* Harris response, orbd.cu:651-669: 32-bit wrapping products, then
-   fma(-0.04, B, A) * (1/(4*7*255))^4 */
-__device__ float harris_score(const uint8_t* __restrict__ c, int p, float scale)
+/* detection tile in LDS: rows y0-4 .. y0+TH+4, columns x0-4 .. x0+TW+4 of the
+   flat pyramid (so reads past a level's edge see exactly the bytes the flat
+   layout puts there) */
+constexpr int kDetSW = kDetTW + 8, kDetSH = kDetTH + 8;
+
+/* circle offsets (orbd.cu:466-469) as LDS offsets in the staged tile */
+#define RING_OFF(dx, dy) ((dy) * kDetSW + (dx))
+__device__ __forceinline__ int ring_off(int k)
 {
-    int win[9][9];
-#pragma unroll
-    for (int r = 0; r < 9; r++)
-#pragma unroll
-        for (int q = 0; q < 9; q++) win[r][q] = c[(long long)(r - 4) * p + (q - 4)];
+    switch (k) {
+    case 0: return RING_OFF(0, 3);   case 1: return RING_OFF(1, 3);   case 2: return RING_OFF(2, 2);
+    case 3: return RING_OFF(3, 1);   case 4: return RING_OFF(3, 0);   case 5: return RING_OFF(3, -1);
+    case 6: return RING_OFF(2, -2);  case 7: return RING_OFF(1, -3);  case 8: return RING_OFF(0, -3);
+    case 9: return RING_OFF(-1, -3); case 10: return RING_OFF(-2, -2); case 11: return RING_OFF(-3, -1);
+    case 12: return RING_OFF(-3, 0); case 13: return RING_OFF(-3, 1); case 14: return RING_OFF(-2, 2);
+    default: return RING_OFF(-1, 3);
+    }
+}
+
+/* Harris response from the staged tile, orbd.cu:651-669: Sobel over the 7x7
+   window, 32-bit wrapping products, then fma(-0.04, B, A) * (1/(4*7*255))^4 */
+__device__ float harris_lds(const uint8_t* c, float scale)
+{
     int dxx = 0, dyy = 0, dxy = 0;
 #pragma unroll
-    for (int i = 1; i < 8; i++) {
+    for (int i = -3; i <= 3; i++) {
+        const uint8_t* up = c + (i - 1) * kDetSW;
+        const uint8_t* md = c + i * kDetSW;
+        const uint8_t* dn = c + (i + 1) * kDetSW;
+        int u0 = up[-4], u1 = up[-3], m0 = md[-4], m1 = md[-3], d0 = dn[-4], d1 = dn[-3];
 #pragma unroll
-        for (int j = 1; j < 8; j++) {
-            const int dx = (win[i][j + 1] - win[i][j - 1]) * 2 + (win[i - 1][j + 1] - win[i - 1][j - 1]) +
-                           (win[i + 1][j + 1] - win[i + 1][j - 1]);
-            const int dy = (win[i + 1][j] - win[i - 1][j]) * 2 + (win[i + 1][j - 1] - win[i - 1][j - 1]) +
-                           (win[i + 1][j + 1] - win[i - 1][j + 1]);
+        for (int j = -3; j <= 3; j++) {
+            const int u2 = up[j + 1], m2 = md[j + 1], d2 = dn[j + 1];
+            /* centre column j: left = (u0,m0,d0), mid = (u1,m1,d1), right = (u2,m2,d2) */
+            const int dx = (m2 - m0) * 2 + (u2 - u0) + (d2 - d0);
+            const int dy = (d1 - u1) * 2 + (d0 - u0) + (d2 - u2);
             dxx += dx * dx;
             dyy += dy * dy;
             dxy += dx * dy;
+            u0 = u1; u1 = u2; m0 = m1; m1 = m2; d0 = d1; d1 = d2;
         }
     }
     const int A = (int)((unsigned)dxx * (unsigned)dyy - (unsigned)dxy * (unsigned)dxy);
@@ -268,108 +404,96 @@ __device__ float harris_score(const uint8_t* __restrict__ c, int p, float scale)
 __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
                                                 int njobs, Tables T, const uint32_t* __restrict__ epochs)
 {
-    const Job jb = find_job(jobs, njobs, blockIdx.x);
+    __shared__ __align__(16) uint8_t s_tile[kDetSH * kDetSW];
+    __shared__ uint16_t s_list[kDetTW * kDetTH];
+    __shared__ int s_n;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const Job jb = find_job(jobs, njobs, b);
     const FrameDev& F = frames[jb.frame];
     const int l = jb.level;
-    const int t = blockIdx.x - jb.first_block;
-    const int ix = 3 + (t % jb.tiles_x) * kDetTW + (threadIdx.x & (kDetTW - 1));
-    const int iy = 3 + (t / jb.tiles_x) * kDetTH + (threadIdx.x / kDetTW);
+    const int t = b - jb.first_block;
+    const int x0 = (t % jb.tiles_x) * kDetTW, y0 = (t / jb.tiles_x) * kDetTH;
     const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
-    if (ix >= w - 3 || iy >= h - 3) return;
-    const uint8_t* c = F.raw + F.loff[l] + (long long)iy * p + ix;
-    const int v = c[0];
-    const int thr = T.threshold;
-    int ring[16];
-    unsigned dark = 0, bright = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        ring[k] = c[c_circle_dx[k] + c_circle_dy[k] * p];
-        dark |= (unsigned)(ring[k] - v < -thr) << k;
-        bright |= (unsigned)(ring[k] - v > thr) << k;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_n = 0;
+    /* stage kDetSH rows x kDetSW bytes as dwords (every row start is dword aligned) */
+    {
+        const uint8_t* base = F.raw + F.loff[l] + (long long)(y0 - 4) * p + (x0 - 4);
+        constexpr int dw_per_row = kDetSW / 4;
+        for (int i = tid; i < kDetSH * dw_per_row; i += 256) {
+            const int r = i / dw_per_row, c = i - r * dw_per_row;
+            *(uint32_t*)&s_tile[r * kDetSW + 4 * c] = *(const uint32_t*)(base + (long long)r * p + 4 * c);
+        }
     }
-    /* quick test of orbd.cu:687-700: every opposite pair has a darker
-       (bit 1) / brighter (bit 2) member; the darker arc is tested first and
-       exclusively (orbd.cu:703-740) */
-    const bool d1 = ((dark | (dark >> 8)) & 0xffu) == 0xffu;
-    const bool d2 = ((bright | (bright >> 8)) & 0xffu) == 0xffu;
-    unsigned m = d1 ? dark : (d2 ? bright : 0u);
-    if (m == 0) return;
-    /* > 8 consecutive of the 25-entry wrapped ring == a circular run of 9 */
-    unsigned r = m | (m << 16);
-    unsigned run = r;
+    __syncthreads();
+    /* FAST-9 segment test; corners go to an LDS list */
+    const int thr = T.threshold;
+    const int lx = tid & (kDetTW - 1);
 #pragma unroll
-    for (int k = 1; k < 9; k++) run &= r >> k;
-    if (run == 0) return;
-    const float score = T.score_type == 0 ? harris_score(c, p, T.harris_scale) : fast_score(ring, v, thr);
-    if (!(score > 0.f)) return;
-    const int x0 = to_l0(F, T.tabs, l, ix, false), y0 = to_l0(F, T.tabs, l, iy, true);
-    const unsigned long long key = ((unsigned long long)epochs[jb.frame] << kEpochShift) |
-                                   ((unsigned long long)__float_as_uint(score) << 3) |
-                                   (unsigned long long)(7 - l);
-    atomicMax(F.keys + (long long)y0 * F.p0 + x0, key);
+    for (int rr = 0; rr < kDetTH / 4; rr++) {
+        const int ly = (tid >> 6) + 4 * rr;
+        const int ix = x0 + lx, iy = y0 + ly;
+        const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx + 4];
+        bool corner = false;
+        if (ix >= 3 && ix < w - 3 && iy >= 3 && iy < h - 3) {
+            const int v = c[0];
+            unsigned dark = 0, bright = 0;
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int q = c[ring_off(k)];
+                dark |= (unsigned)(q - v < -thr) << k;
+                bright |= (unsigned)(q - v > thr) << k;
+            }
+            /* quick test of orbd.cu:687-700, then >8 consecutive of the wrapped
+               ring (a circular run of 9) on the darker arc first, exclusively
+               (orbd.cu:703-740) */
+            const bool d1 = ((dark | (dark >> 8)) & 0xffu) == 0xffu;
+            const bool d2 = ((bright | (bright >> 8)) & 0xffu) == 0xffu;
+            const unsigned m = d1 ? dark : (d2 ? bright : 0u);
+            const unsigned r2 = m | (m << 16);
+            unsigned run = r2;
+#pragma unroll
+            for (int k = 1; k < 9; k++) run &= r2 >> k;
+            corner = run != 0;
+        }
+        if (corner) {
+            const int slot = atomicAdd(&s_n, 1);
+            s_list[slot] = (uint16_t)((ly << 8) | lx);
+        }
+    }
+    __syncthreads();
+    const int n = s_n;
+    if (n == 0) return;
+    const unsigned long long ep = (unsigned long long)epochs[jb.frame] << kEpochShift;
+    for (int i = tid; i < n; i += 256) {
+        const int e = s_list[i];
+        const int ly = e >> 8, lx2 = e & 255;
+        const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx2 + 4];
+        float score;
+        if (T.score_type == 0) {
+            score = harris_lds(c, T.harris_scale);
+        } else {
+            int ring[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) ring[k] = c[ring_off(k)];
+            score = fast_score(ring, c[0], thr);
+        }
+        if (!(score > 0.f)) continue;
+        const int ix = x0 + lx2, iy = y0 + ly;
+        const int X0 = to_l0(F, T.tabs, l, ix, false), Y0 = to_l0(F, T.tabs, l, iy, true);
+        const unsigned long long key = ep | ((unsigned long long)__float_as_uint(score) << 3) |
+                                       (unsigned long long)(7 - l);
+        atomicMax(F.keys + (long long)Y0 * F.p0 + X0, key);
+    }
 }
 
 /* -------------------------------------------------------------------- NMS */
 
-__global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames, const int* __restrict__ block_frame,
-                                             const uint32_t* __restrict__ epochs, Candidate* __restrict__ cand,
-                                             int* __restrict__ blk_count)
+/* exclusive scan of one int per thread over a workgroup of NT threads */
+template <int NT>
+__device__ __forceinline__ int block_scan(int v, int* s_tmp, int& total)
 {
-    __shared__ int s_w[4];
-    const int f = block_frame[blockIdx.x];
-    const FrameDev& F = frames[f];
-    const int b = blockIdx.x - F.nms_first;
-    const int row = b / F.nms_segs, seg = b - row * F.nms_segs;
-    const int y = F.border + row;
-    const int x = F.border + seg * kNmsSeg + threadIdx.x;
-    const uint32_t epoch = epochs[f];
-    bool surv = false;
-    uint32_t sbits = 0, lev = 0;
-    if (x < F.w0 - F.border) {
-        const unsigned long long* kp = F.keys + (long long)y * F.p0 + x;
-        const unsigned long long k = *kp;
-        if ((uint32_t)(k >> kEpochShift) == epoch) {
-            sbits = (uint32_t)(k >> 3) & 0x7fffffffu;
-            lev = 7u - (uint32_t)(k & 7u);
-            const int r = F.radius[lev];
-            surv = true;
-            for (int dy = -r; dy <= r && surv; dy++) {
-                const unsigned long long* rp = kp + (long long)dy * F.p0;
-                for (int dx = -r; dx <= r; dx++) {
-                    const unsigned long long nk = rp[dx];
-                    if ((uint32_t)(nk >> kEpochShift) != epoch || (dx == 0 && dy == 0)) continue;
-                    const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
-                    if (ns > sbits || (ns == sbits && dx <= 0 && dy <= 0)) {
-                        surv = false;
-                        break;
-                    }
-                }
-            }
-        }
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(surv);
-    const int pre = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) s_w[wid] = __popcll(m);
-    __syncthreads();
-    int base = 0;
-    for (int i = 0; i < wid; i++) base += s_w[i];
-    if (surv) {
-        Candidate c;
-        c.xy = ((uint32_t)y << 16) | (uint32_t)x;
-        c.score = sbits;
-        c.level = lev;
-        c.pad = 0;
-        cand[(long long)blockIdx.x * (kNmsSeg / 2) + base + pre] = c;
-    }
-    if (threadIdx.x == 0) blk_count[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-}
-
-/* ---------------------------------------------------------------- select */
-
-/* exclusive scan over the 1024 threads of a workgroup */
-__device__ int block_scan_1024(int v, int* s_tmp, int& total)
-{
+    constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     int x = v;
 #pragma unroll
@@ -380,25 +504,107 @@ __device__ int block_scan_1024(int v, int* s_tmp, int& total)
     if (lane == 63) s_tmp[wid] = x;
     __syncthreads();
     if (wid == 0) {
-        int s = lane < 16 ? s_tmp[lane] : 0;
+        int q = lane < NW ? s_tmp[lane] : 0;
 #pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const int y = __shfl_up(s, o, 64);
-            if (lane >= o) s += y;
+        for (int o = 1; o < NW; o <<= 1) {
+            const int y = __shfl_up(q, o, 64);
+            if (lane >= o) q += y;
         }
-        if (lane < 16) s_tmp[16 + lane] = s;
+        if (lane < NW) s_tmp[NW + lane] = q;
     }
     __syncthreads();
-    total = s_tmp[16 + 15];
-    const int res = x - v + (wid > 0 ? s_tmp[16 + wid - 1] : 0);
+    total = s_tmp[2 * NW - 1];
+    const int res = x - v + (wid > 0 ? s_tmp[NW + wid - 1] : 0);
     __syncthreads();
     return res;
 }
 
+/* orbd.cu:945-966: a key survives unless a positive neighbour in its
+   (2r+1)^2 square is greater, or equal and at dx <= 0 && dy <= 0 (flat
+   indexing with pitch p0); r is the radius of the key's level */
+__device__ __noinline__ bool nms_keep(const unsigned long long* __restrict__ kp, long long p0, unsigned long long key,
+                                      uint32_t epoch, int r)
+{
+    const uint32_t sbits = (uint32_t)(key >> 3) & 0x7fffffffu;
+    for (int dy = -r; dy <= r; dy++) {
+        const unsigned long long* rp = kp + (long long)dy * p0;
+        for (int dx = -r; dx <= r; dx++) {
+            const unsigned long long nk = rp[dx];
+            if ((uint32_t)(nk >> kEpochShift) != epoch || (dx == 0 && dy == 0)) continue;
+            const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
+            if (ns > sbits || (ns == sbits && dx <= 0 && dy <= 0)) return false;
+        }
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames, const int* __restrict__ block_frame,
+                                             const uint32_t* __restrict__ epochs, Candidate* __restrict__ cand,
+                                             int* __restrict__ blk_count)
+{
+    __shared__ int s_tmp[8];
+    const int f = block_frame[blockIdx.x];
+    const FrameDev& F = frames[f];
+    const int b = blockIdx.x - F.nms_first;
+    const int row = b / F.nms_segs, seg = b - row * F.nms_segs;
+    const int y = F.border + row;
+    const int xs = F.nms_x0 + seg * kNmsSeg + 4 * threadIdx.x;
+    const int xlo = F.border, xhi = F.w0 - F.border;
+    const uint32_t epoch = epochs[f];
+    const unsigned long long* rowp = F.keys + (long long)y * F.p0;
+    unsigned long long k4[4] = {0ull, 0ull, 0ull, 0ull};
+    if (xs < xhi) {
+        /* xs is a multiple of 4 below w0 <= p0 (a multiple of 128): 32-byte aligned, inside the row */
+        const ulonglong2 a = *(const ulonglong2*)(rowp + xs);
+        const ulonglong2 c = *(const ulonglong2*)(rowp + xs + 2);
+        k4[0] = a.x;
+        k4[1] = a.y;
+        k4[2] = c.x;
+        k4[3] = c.y;
+    }
+    uint32_t surv = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int x = xs + k;
+        const unsigned long long key = k4[k];
+        const bool cand_px = x >= xlo && x < xhi && (uint32_t)(key >> kEpochShift) == epoch;
+        if (cand_px && nms_keep(rowp + x, F.p0, key, epoch, F.radius[7u - (uint32_t)(key & 7u)])) surv |= 1u << k;
+    }
+    int total;
+    int pos = block_scan<256>(__popc(surv), s_tmp, total);
+    Candidate* out = cand + (long long)blockIdx.x * kNmsCap;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (surv & (1u << k)) {
+            const unsigned long long key = k4[k];
+            Candidate c;
+            c.pos = ((7u - (uint32_t)(key & 7u)) << 28) | ((uint32_t)y << 14) | (uint32_t)(xs + k);
+            c.score = (uint32_t)(key >> 3) & 0x7fffffffu;
+            out[pos++] = c;
+        }
+    }
+    if (threadIdx.x == 0) blk_count[blockIdx.x] = total;
+}
+
+/* ---------------------------------------------------------- select + emit */
+
+__device__ __forceinline__ void emit_point(orb::OrbPoint* o, Candidate c)
+{
+    o->x = (int)(c.pos & 0x3fffu);
+    o->y = (int)((c.pos >> 14) & 0x3fffu);
+    o->octave = (int)(c.pos >> 28);
+    o->score = __uint_as_float(c.score);
+    o->angle = 0.f;
+    o->match = -1;
+    o->distance = -1;
+}
+
+/* One workgroup per frame: prefix sum of the NMS segment counts and the
+   keypoints written in raster order.  With more than max_pts survivors, an
+   exact radix select finds the score of the max_pts-th best; all higher
+   scores and the first `need` equal ones in raster order are kept. */
 __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ frames, const Candidate* __restrict__ cand,
-                                                 const int* __restrict__ blk_count, int* __restrict__ blk_out,
-                                                 int* __restrict__ blk_eq, SelectState* __restrict__ sel,
-                                                 uint32_t* __restrict__ epochs)
+                                                 const int* __restrict__ blk_count, uint32_t* __restrict__ epochs)
 {
     __shared__ int s_tmp[32];
     __shared__ int s_hist[256];
@@ -415,18 +621,15 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
     int sum = 0;
     for (int b = b0; b < b1; b++) sum += blk_count[first + b];
     int total;
-    int run = block_scan_1024(sum, s_tmp, total);
+    int out = block_scan<1024>(sum, s_tmp, total);
 
-    uint32_t thr = 0;
-    int need = 0;
     if (total <= cap) {
         for (int b = b0; b < b1; b++) {
-            blk_out[first + b] = run;
-            blk_eq[first + b] = 0;
-            run += blk_count[first + b];
+            const int n = blk_count[first + b];
+            const Candidate* cb = cand + (long long)(first + b) * kNmsCap;
+            for (int i = 0; i < n; i++) emit_point(F.pts + out++, cb[i]);
         }
     } else {
-        /* exact radix select of the cap-th largest score (8 bits per pass) */
         uint32_t prefix = 0, pmask = 0;
         int k = cap;
         for (int shift = 24; shift >= 0; shift -= 8) {
@@ -434,10 +637,10 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
             __syncthreads();
             for (int b = b0; b < b1; b++) {
                 const int n = blk_count[first + b];
-                const Candidate* cb = cand + (long long)(first + b) * (kNmsSeg / 2);
+                const Candidate* cb = cand + (long long)(first + b) * kNmsCap;
                 for (int i = 0; i < n; i++) {
-                    const uint32_t s = cb[i].score;
-                    if ((s & pmask) == prefix) atomicAdd(&s_hist[(s >> shift) & 255u], 1);
+                    const uint32_t sv = cb[i].score;
+                    if ((sv & pmask) == prefix) atomicAdd(&s_hist[(sv >> shift) & 255u], 1);
                 }
             }
             __syncthreads();
@@ -456,45 +659,33 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
             k = s_pick[1];
             __syncthreads();
         }
-        thr = prefix;
-        need = k;
-        /* per-thread counts of > thr and == thr over its blocks */
-        int gt_sum = 0, eq_sum = 0;
+        const uint32_t thr = prefix;
+        const int need = k;
+        int gt = 0, eq = 0;
         for (int b = b0; b < b1; b++) {
             const int n = blk_count[first + b];
-            const Candidate* cb = cand + (long long)(first + b) * (kNmsSeg / 2);
+            const Candidate* cb = cand + (long long)(first + b) * kNmsCap;
             for (int i = 0; i < n; i++) {
-                gt_sum += cb[i].score > thr;
-                eq_sum += cb[i].score == thr;
+                gt += cb[i].score > thr;
+                eq += cb[i].score == thr;
             }
         }
-        int eq_tot;
-        int eq_before = block_scan_1024(eq_sum, s_tmp, eq_tot);
-        /* kept per thread: gt + min(eq, max(0, need - eq_before)) */
-        int kept_thr = gt_sum + min(eq_sum, max(0, need - eq_before));
-        int kept_tot;
-        int out = block_scan_1024(kept_thr, s_tmp, kept_tot);
+        int tot;
+        int eq_before = block_scan<1024>(eq, s_tmp, tot);
+        out = block_scan<1024>(gt + min(eq, max(0, need - eq_before)), s_tmp, tot);
         for (int b = b0; b < b1; b++) {
             const int n = blk_count[first + b];
-            const Candidate* cb = cand + (long long)(first + b) * (kNmsSeg / 2);
-            int g = 0, e = 0;
+            const Candidate* cb = cand + (long long)(first + b) * kNmsCap;
             for (int i = 0; i < n; i++) {
-                g += cb[i].score > thr;
-                e += cb[i].score == thr;
+                const Candidate c = cb[i];
+                bool kept = c.score > thr;
+                if (c.score == thr) kept = eq_before++ < need;
+                if (kept) emit_point(F.pts + out++, c);
             }
-            blk_out[first + b] = out;
-            blk_eq[first + b] = eq_before;
-            out += g + min(e, max(0, need - eq_before));
-            eq_before += e;
         }
         total = cap;
     }
-    if (tid == 0) {
-        sel[f].thr = thr;
-        sel[f].need = need;
-        sel[f].epoch = epoch;
-        *F.count = total;
-    }
+    if (tid == 0) *F.count = total;
     /* advance the epoch; on wrap-around clear this frame's key map */
     if (epoch + 1 >= kEpochLimit) {
         const long long n = (long long)F.h0 * F.p0;
@@ -505,60 +696,26 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
     }
 }
 
-/* ------------------------------------------------------------------ emit */
-
-__device__ __forceinline__ int block_scan_256(int v, int* s_tmp)
-{
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const unsigned long long m = __ballot(v != 0);
-    const int pre = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) s_tmp[wid] = __popcll(m);
-    __syncthreads();
-    int base = 0;
-    for (int i = 0; i < wid; i++) base += s_tmp[i];
-    __syncthreads();
-    return base + pre;
-}
-
-__global__ __launch_bounds__(256) void k_emit(const FrameDev* __restrict__ frames, const int* __restrict__ block_frame,
-                                              const Candidate* __restrict__ cand, const int* __restrict__ blk_count,
-                                              const int* __restrict__ blk_out, const int* __restrict__ blk_eq,
-                                              const SelectState* __restrict__ sel)
-{
-    __shared__ int s_tmp[4];
-    const int n = blk_count[blockIdx.x];
-    if (n == 0) return;
-    const int f = block_frame[blockIdx.x];
-    const FrameDev& F = frames[f];
-    const SelectState S = sel[f];
-    const int i = threadIdx.x;
-    const bool has = i < n;
-    Candidate c = {0, 0, 0, 0};
-    if (has) c = cand[(long long)blockIdx.x * (kNmsSeg / 2) + i];
-    const bool is_eq = has && c.score == S.thr;
-    const int eq_rank = block_scan_256(is_eq, s_tmp);
-    const bool kept = has && (c.score > S.thr || (is_eq && blk_eq[blockIdx.x] + eq_rank < S.need));
-    const int rank = block_scan_256(kept, s_tmp);
-    if (kept) {
-        orb::OrbPoint* o = F.pts + blk_out[blockIdx.x] + rank;
-        o->x = (int)(c.xy & 0xffffu);
-        o->y = (int)(c.xy >> 16);
-        o->octave = (int)c.level;
-        o->score = __uint_as_float(c.score);
-        o->angle = 0.f;
-        o->match = -1;
-        o->distance = -1;
-    }
-}
-
 /* --------------------------------------------------------- angle + describe */
 
-__device__ __forceinline__ int sample(const uint8_t* __restrict__ c, int p, int2 pt, float sn, float cs)
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* staged patches per wavefront: the raw 31-row disc region (36-byte rows) and
+   the blurred 37x37 region of the rotated pattern (40-byte rows), each row the
+   flat-layout bytes around the keypoint */
+constexpr int kRawRow = 36, kBlurRow = 40;
+
+__device__ __forceinline__ int sample_lds(const uint8_t* bp, int2 pt, float sn, float cs)
 {
     const float x = (float)pt.x, y = (float)pt.y;
     const int ix = __float2int_rn(__fmaf_rn(x, cs, -__fmul_rn(y, sn)));
     const int iy = __float2int_rn(__fmaf_rn(x, sn, __fmul_rn(y, cs)));
-    return c[(long long)iy * p + ix];
+    return bp[iy * kBlurRow + ix];
 }
 
 __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ frames, int blocks_per_frame, Tables T,
@@ -567,81 +724,104 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
     __shared__ int2 s_pat[512];
     __shared__ int s_umax[32];
     __shared__ uint8_t s_code[kDescPerBlock][128];
+    __shared__ __align__(16) uint8_t s_raw[kDescPerBlock][31 * kRawRow];
+    __shared__ __align__(16) uint8_t s_blur[kDescPerBlock][37 * kBlurRow];
     const int f = blockIdx.x / blocks_per_frame;
+    const int lb = blockIdx.x - f * blocks_per_frame;
     const FrameDev& F = frames[f];
     for (int i = threadIdx.x; i < 512; i += 256) s_pat[i] = T.pattern[i];
     if (threadIdx.x < 17) s_umax[threadIdx.x] = T.umax[threadIdx.x];
     __syncthreads();
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int pi = (blockIdx.x - f * blocks_per_frame) * kDescPerBlock + wid;
     const int n = *F.count;
-    if (pi >= n) return;
-    orb::OrbPoint* P = F.pts + pi;
-    const int l = P->octave;
-    const int cx = from_l0(F, T.tabs, l, P->x, false), cy = from_l0(F, T.tabs, l, P->y, true);
-    const int p = F.lp[l];
-    const long long co = F.loff[l] + (long long)cy * p + cx;
-
-    /* intensity centroid over the umax disc of the raw level (orbd.cu:1011-1032) */
-    const uint8_t* craw = F.raw + co;
     const int hk = T.half_patch, side = 2 * hk + 1;
-    int m10 = 0, m01 = 0;
-    for (int i = lane; i < side * side; i += 64) {
-        const int v = i / side - hk, u = i % side - hk;
-        if (abs(u) <= s_umax[abs(v)]) {
-            const int I = craw[(long long)v * p + u];
-            m10 += u * I;
-            m01 += v * I;
-        }
-    }
-    m10 = wave_sum(m10);
-    m01 = wave_sum(m01);
-    const float ang = (m01 == 0 && m10 == 0) ? 0.f : dev_atan2(__int2float_rn(m01), __int2float_rn(m10));
-    if (lane == 0) P->angle = ang;
-    if (!compute_desc) return;
+    for (int pi = lb * kDescPerBlock + wid; pi < n; pi += blocks_per_frame * kDescPerBlock) {
+        orb::OrbPoint* P = F.pts + pi;
+        const int l = P->octave;
+        const int cx = from_l0(F, T.tabs, l, P->x, false), cy = from_l0(F, T.tabs, l, P->y, true);
+        const int p = F.lp[l];
+        const long long co = F.loff[l] + (long long)cy * p + cx;
 
-    float sn, cs;
-    dev_sincos(ang, sn, cs);
-    const uint8_t* cb = F.blur + co;
-    unsigned long long* out = (unsigned long long*)(F.desc + (long long)pi * 32);
-    if (T.wta_k == 2) {
-        /* bit k of the descriptor compares points 2k and 2k+1 (orbd.cu:1351-1361) */
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = lane + 64 * j;
-            const bool bit = sample(cb, p, s_pat[2 * k], sn, cs) < sample(cb, p, s_pat[2 * k + 1], sn, cs);
-            const unsigned long long m = __ballot(bit);
-            if (lane == 0) out[j] = m;
+        /* stage both patches with dword loads (every row shares the keypoint's
+           alignment because the pitch is a multiple of 128) */
+        const uint8_t* ra = F.raw + co - 15 - 15LL * p;
+        const int ro = (int)((uintptr_t)ra & 3);
+        const uint8_t* rb = ra - ro;
+        for (int i = lane; i < 31 * (kRawRow / 4); i += 64) {
+            const int r = i / (kRawRow / 4), d = i - r * (kRawRow / 4);
+            *(uint32_t*)&s_raw[wid][r * kRawRow + 4 * d] = *(const uint32_t*)(rb + (long long)r * p + 4 * d);
         }
-    } else {
-        /* 2-bit argmax codes, code q = 4*byte + slot (orbd.cu:1364-1434) */
-        for (int j = 0; j < 2; j++) {
-            const int q = lane + 64 * j, byte = q >> 2, slot = q & 3;
-            int code;
-            if (T.wta_k == 3) {
-                const int2* pp = s_pat + 12 * byte;
-                const int i0 = slot < 3 ? 3 * slot : 8, i1 = slot < 3 ? 3 * slot + 1 : 9,
-                          i2 = slot < 3 ? 3 * slot + 2 : 11;
-                const int t0 = sample(cb, p, pp[i0], sn, cs), t1 = sample(cb, p, pp[i1], sn, cs),
-                          t2 = sample(cb, p, pp[i2], sn, cs);
-                code = t2 > t1 ? (t2 > t0 ? 2 : 0) : (t1 > t0);
-            } else {
-                const int2* pp = s_pat + 16 * byte + 4 * slot;
-                int t0 = sample(cb, p, pp[0], sn, cs), t1 = sample(cb, p, pp[1], sn, cs);
-                int t2 = sample(cb, p, pp[2], sn, cs), t3 = sample(cb, p, pp[3], sn, cs);
-                int u = 0, v = 2;
-                if (t1 > t0) { t0 = t1; u = 1; }
-                if (t3 > t2) { t2 = t3; v = 3; }
-                code = t0 > t2 ? u : v;
+        const uint8_t* ba = F.blur + co - 18 - 18LL * p;
+        const int bo = (int)((uintptr_t)ba & 3);
+        const uint8_t* bb = ba - bo;
+        if (compute_desc) {
+            for (int i = lane; i < 37 * (kBlurRow / 4); i += 64) {
+                const int r = i / (kBlurRow / 4), d = i - r * (kBlurRow / 4);
+                *(uint32_t*)&s_blur[wid][r * kBlurRow + 4 * d] = *(const uint32_t*)(bb + (long long)r * p + 4 * d);
             }
-            s_code[wid][q] = (uint8_t)code;
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        if (lane < 32) {
-            const uint8_t* sc = s_code[wid] + 4 * lane;
-            F.desc[(long long)pi * 32 + lane] = (uint8_t)(sc[0] | (sc[1] << 2) | (sc[2] << 4) | (sc[3] << 6));
+        wave_sync_lds();
+
+        /* intensity centroid over the umax disc of the raw level (orbd.cu:1011-1032) */
+        const uint8_t* rc = &s_raw[wid][15 * kRawRow + ro + 15];
+        int m10 = 0, m01 = 0;
+        for (int i = lane; i < side * side; i += 64) {
+            const int v = i / side - hk, u = i % side - hk;
+            if (abs(u) <= s_umax[abs(v)]) {
+                const int I = rc[v * kRawRow + u];
+                m10 += u * I;
+                m01 += v * I;
+            }
         }
+        m10 = wave_sum(m10);
+        m01 = wave_sum(m01);
+        const float ang = (m01 == 0 && m10 == 0) ? 0.f : dev_atan2(__int2float_rn(m01), __int2float_rn(m10));
+        if (lane == 0) P->angle = ang;
+        if (compute_desc) {
+            float sn, cs;
+            dev_sincos(ang, sn, cs);
+            const uint8_t* bp = &s_blur[wid][18 * kBlurRow + bo + 18];
+            if (T.wta_k == 2) {
+                /* bit k of the descriptor compares points 2k and 2k+1 (orbd.cu:1351-1361) */
+                unsigned long long* outd = (unsigned long long*)(F.desc + (long long)pi * 32);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int k = lane + 64 * j;
+                    const bool bit = sample_lds(bp, s_pat[2 * k], sn, cs) < sample_lds(bp, s_pat[2 * k + 1], sn, cs);
+                    const unsigned long long m = __ballot(bit);
+                    if (lane == 0) outd[j] = m;
+                }
+            } else {
+                /* 2-bit argmax codes, code q = 4*byte + slot (orbd.cu:1364-1434) */
+                for (int j = 0; j < 2; j++) {
+                    const int q = lane + 64 * j, byte = q >> 2, slot = q & 3;
+                    int code;
+                    if (T.wta_k == 3) {
+                        const int2* pp = s_pat + 12 * byte;
+                        const int i0 = slot < 3 ? 3 * slot : 8, i1 = slot < 3 ? 3 * slot + 1 : 9,
+                                  i2 = slot < 3 ? 3 * slot + 2 : 11;
+                        const int t0 = sample_lds(bp, pp[i0], sn, cs), t1 = sample_lds(bp, pp[i1], sn, cs),
+                                  t2 = sample_lds(bp, pp[i2], sn, cs);
+                        code = t2 > t1 ? (t2 > t0 ? 2 : 0) : (t1 > t0);
+                    } else {
+                        const int2* pp = s_pat + 16 * byte + 4 * slot;
+                        int t0 = sample_lds(bp, pp[0], sn, cs), t1 = sample_lds(bp, pp[1], sn, cs);
+                        int t2 = sample_lds(bp, pp[2], sn, cs), t3 = sample_lds(bp, pp[3], sn, cs);
+                        int u = 0, v = 2;
+                        if (t1 > t0) { t0 = t1; u = 1; }
+                        if (t3 > t2) { t2 = t3; v = 3; }
+                        code = t0 > t2 ? u : v;
+                    }
+                    s_code[wid][q] = (uint8_t)code;
+                }
+                wave_sync_lds();
+                if (lane < 32) {
+                    const uint8_t* sc = s_code[wid] + 4 * lane;
+                    F.desc[(long long)pi * 32 + lane] = (uint8_t)(sc[0] | (sc[1] << 2) | (sc[2] << 4) | (sc[3] << 6));
+                }
+            }
+        }
+        wave_sync_lds();
     }
 }
 
@@ -668,6 +848,7 @@ __global__ void k_debug_atan2(const float* yx, int n, float* out)
 
 hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s, const StageEvents* ev)
 {
+    (void)hipGetLastError(); /* report only errors of these launches */
     if (d.pyr_blocks > 0) {
         ev_start(ev, kStPyramid, s);
         hipLaunchKernelGGL(k_pyramid, dim3(d.pyr_blocks), dim3(256), 0, s, d.frames, d.pyr_jobs, d.pyr_njobs, d.t);
@@ -686,15 +867,8 @@ hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s,
         ev_stop(ev, kStNms, s);
     }
     ev_start(ev, kStSelect, s);
-    hipLaunchKernelGGL(k_select, dim3(d.nframes), dim3(1024), 0, s, d.frames, d.cand, d.blk_count, d.blk_out,
-                       d.blk_eq, d.sel, d.epochs);
+    hipLaunchKernelGGL(k_select, dim3(d.nframes), dim3(1024), 0, s, d.frames, d.cand, d.blk_count, d.epochs);
     ev_stop(ev, kStSelect, s);
-    if (d.nms_blocks > 0) {
-        ev_start(ev, kStEmit, s);
-        hipLaunchKernelGGL(k_emit, dim3(d.nms_blocks), dim3(256), 0, s, d.frames, d.nms_block_frame, d.cand,
-                           d.blk_count, d.blk_out, d.blk_eq, d.sel);
-        ev_stop(ev, kStEmit, s);
-    }
     if (compute_desc && d.desc_blocks_per_frame > 0) {
         ev_start(ev, kStDescribe, s);
         hipLaunchKernelGGL(k_describe, dim3(d.desc_blocks_per_frame * d.nframes), dim3(256), 0, s, d.frames,

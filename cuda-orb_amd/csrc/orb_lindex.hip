@@ -1,0 +1,390 @@
+/*
+ * Bucketed database of the learned matcher (gfx950): bucket histograms and
+ * scatter (a counting sort of the database by bucket), the k-modes centroid
+ * update, per-bucket probe lists, and the bucketed exact search -- the INT8
+ * MFMA contraction of orb_match.hip applied to each (bucket, probing queries)
+ * block, with keys carrying the original database index so ties resolve to
+ * the lowest index whatever the bucketed order.
+ */
+#include "orb_internal.h"
+#include "orb_match_common.h"
+#include "orb_mlp.h"
+
+namespace orbk
+{
+
+constexpr int kLxRow = 256 + 16; /* LDS row of one unpacked entry (padded) */
+constexpr int kLxTile = 64;      /* entries per LDS tile */
+constexpr int kLxQ = 256;        /* queries per unit */
+
+__global__ void k_zero_int(int* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_bucket_hist(const int* __restrict__ assign, int n, int* __restrict__ hist)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&hist[assign[i]], 1);
+}
+
+/* single workgroup: exclusive scan of nb counts (nb <= 4096) into offsets[0..nb] */
+__global__ __launch_bounds__(1024) void k_scan_offsets(const int* __restrict__ count, int nb, int* __restrict__ offsets,
+                                                       int* __restrict__ cursor, int unit, int* __restrict__ units)
+{
+    __shared__ int s[4096 + 1];
+    __shared__ int su[4096 + 1];
+    for (int i = threadIdx.x; i < nb; i += 1024) {
+        s[i] = count[i];
+        su[i] = unit > 0 ? (count[i] + unit - 1) / unit : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int i = 0; i < nb; i++) {
+            const int c = s[i], u = su[i];
+            s[i] = a;
+            su[i] = b;
+            a += c;
+            b += u;
+        }
+        s[nb] = a;
+        su[nb] = b;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= nb; i += 1024) {
+        offsets[i] = s[i];
+        if (cursor && i < nb) cursor[i] = s[i];
+        if (units) units[i] = su[i];
+    }
+}
+
+__global__ void k_bucket_scatter(const int* __restrict__ assign, int n, const uint8_t* __restrict__ db,
+                                 int* __restrict__ cursor, int* __restrict__ perm, uint8_t* __restrict__ db_sorted)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int pos = atomicAdd(&cursor[assign[i]], 1);
+    perm[pos] = i;
+    const uint4* s = (const uint4*)(db + (long long)i * 32);
+    uint4* d = (uint4*)(db_sorted + (long long)pos * 32);
+    d[0] = s[0];
+    d[1] = s[1];
+}
+
+/* one workgroup per bucket, one thread per bit: the majority bit of the
+   bucket's entries (a tie or an empty bucket keeps the previous bit) */
+__global__ __launch_bounds__(256) void k_kmodes_update(const uint8_t* __restrict__ db_sorted,
+                                                       const int* __restrict__ offsets, uint8_t* __restrict__ centroids)
+{
+    const int b = blockIdx.x, k = threadIdx.x;
+    const int o0 = offsets[b], o1 = offsets[b + 1];
+    int cnt = 0;
+    for (int j = o0; j < o1; j++) cnt += (db_sorted[(long long)j * 32 + (k >> 3)] >> (k & 7)) & 1;
+    const int size = o1 - o0;
+    const int old = (centroids[(long long)b * 32 + (k >> 3)] >> (k & 7)) & 1;
+    const bool bit = 2 * cnt > size || (2 * cnt == size && old);
+    const unsigned long long m = __ballot(bit);
+    if ((k & 63) == 0) ((unsigned long long*)(centroids + (long long)b * 32))[k >> 6] = m;
+}
+
+__global__ void k_probe_count(const int* __restrict__ probes, int n, int* __restrict__ count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&count[probes[i]], 1);
+}
+
+__global__ void k_probe_fill(const int* __restrict__ probes, int n, int* __restrict__ cursor, int* __restrict__ lists)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) lists[atomicAdd(&cursor[probes[i]], 1)] = i; /* i = query * nprobe + slot */
+}
+
+/* Each unit: up to 256 list entries (query, slot) of one bucket x all of the
+   bucket's entries.  A wave holds 64 queries as +/-1 B operands; each 64-entry
+   tile of the bucket is unpacked into LDS once per workgroup. */
+__global__ __launch_bounds__(256) void k_bucket_search(const uint8_t* __restrict__ q, int nprobe,
+                                                       const int* __restrict__ lists,
+                                                       const int* __restrict__ list_offsets,
+                                                       const int* __restrict__ unit_offsets, int nb,
+                                                       const uint8_t* __restrict__ db_sorted,
+                                                       const int* __restrict__ perm, const int* __restrict__ offsets,
+                                                       uint2* __restrict__ partial)
+{
+    __shared__ __align__(16) int8_t s_db[kLxTile * kLxRow];
+    __shared__ int s_idx[kLxTile];
+    const int total_units = unit_offsets[nb];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int g = lane >> 4, col = lane & 15;
+    for (int u = blockIdx.x; u < total_units; u += gridDim.x) {
+        /* bucket of unit u: the last b with unit_offsets[b] <= u */
+        int lo = 0, hi = nb - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (unit_offsets[mid] <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        const int b = lo;
+        const int e0 = list_offsets[b] + (u - unit_offsets[b]) * kLxQ, e1 = min(list_offsets[b + 1], e0 + kLxQ);
+        v4i bq[4][4];
+        int ent[4];
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+            const int e = e0 + wid * 64 + qb * 16 + col;
+            ent[qb] = e < e1 ? lists[e] : -1;
+            uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+            if (ent[qb] >= 0) {
+                const uint4* qp = (const uint4*)(q + (long long)(ent[qb] / nprobe) * 32);
+                d0 = qp[0];
+                d1 = qp[1];
+            }
+            query_operands(d0, d1, g, bq[qb]);
+        }
+        uint32_t k1[4], k2[4];
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) k1[qb] = k2[qb] = kNoKey;
+        const int j0 = offsets[b], j1 = offsets[b + 1];
+        for (int jt = j0; jt < j1; jt += kLxTile) {
+            __syncthreads();
+            {
+                const int e = tid >> 2, qq = tid & 3, j = jt + e;
+                uint32_t lo2 = 0, hi2 = 0;
+                if (j < j1) {
+                    const uint2 v = *(const uint2*)(db_sorted + (long long)j * 32 + qq * 8);
+                    lo2 = v.x;
+                    hi2 = v.y;
+                    if (qq == 0) s_idx[e] = perm[j];
+                }
+                int8_t* dst = s_db + e * kLxRow + qq * 64;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t hw = k < 2 ? (lo2 >> (16 * k)) & 0xffffu : (hi2 >> (16 * (k - 2))) & 0xffffu;
+                    *(v4i*)(dst + 16 * k) = pm1_16(hw);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ab = 0; ab < 4; ab++) {
+                v4i a[4];
+                const int8_t* src = s_db + (ab * 16 + col) * kLxRow + 16 * g;
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(src + 64 * ks);
+                const int rb = ab * 16 + 4 * g;
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) {
+                    v4i acc = {0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++)
+                        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (jt + rb + r < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)s_idx[rb + r]));
+                }
+            }
+        }
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+#pragma unroll
+            for (int m = 16; m <= 32; m <<= 1) {
+                const uint32_t o1 = __shfl_xor(k1[qb], m, 64), o2 = __shfl_xor(k2[qb], m, 64);
+                top2_push(k1[qb], k2[qb], o1);
+                top2_push(k1[qb], k2[qb], o2);
+            }
+            if (g == 0 && ent[qb] >= 0) partial[ent[qb]] = make_uint2(k1[qb], k2[qb]);
+        }
+    }
+}
+
+/* low-confidence queries (max softmax < thresh) gathered for the exact search */
+__global__ void k_fallback_compact(const uint8_t* __restrict__ q, int nq, const float* __restrict__ pmax, float thresh,
+                                   int* __restrict__ fb_count, int* __restrict__ fb_index, int* __restrict__ fb_qidx,
+                                   uint8_t* __restrict__ fb_q)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    if (!(pmax[i] < thresh)) {
+        fb_index[i] = -1;
+        return;
+    }
+    const int pos = atomicAdd(fb_count, 1);
+    fb_index[i] = pos;
+    fb_qidx[pos] = i;
+    const uint4* s = (const uint4*)(q + (long long)i * 32);
+    uint4* d = (uint4*)(fb_q + (long long)pos * 32);
+    d[0] = s[0];
+    d[1] = s[1];
+}
+
+__global__ void k_probe_merge(const uint2* __restrict__ partial, int nq, int nprobe, const int* __restrict__ fb_index,
+                              const int* __restrict__ ex_idx1, const int* __restrict__ ex_d1,
+                              const int* __restrict__ ex_idx2, const int* __restrict__ ex_d2, int* __restrict__ idx1,
+                              int* __restrict__ d1, int* __restrict__ idx2, int* __restrict__ d2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int f = fb_index ? fb_index[i] : -1;
+    if (f >= 0) {
+        if (idx1) idx1[i] = ex_idx1[f];
+        if (d1) d1[i] = ex_d1[f];
+        if (idx2) idx2[i] = ex_idx2[f];
+        if (d2) d2[i] = ex_d2[f];
+        return;
+    }
+    uint32_t k1 = kNoKey, k2 = kNoKey;
+    for (int s = 0; s < nprobe; s++) {
+        const uint2 v = partial[(long long)i * nprobe + s];
+        top2_push(k1, k2, v.x);
+        top2_push(k1, k2, v.y);
+    }
+    if (idx1) idx1[i] = k1 == kNoKey ? -1 : (int)(k1 & kIdxMask);
+    if (d1) d1[i] = k1 == kNoKey ? 257 : (int)(k1 >> kIdxBits);
+    if (idx2) idx2[i] = k2 == kNoKey ? -1 : (int)(k2 & kIdxMask);
+    if (d2) d2[i] = k2 == kNoKey ? 257 : (int)(k2 >> kIdxBits);
+}
+
+/* gather rows idx[i] (or the first n) of q and, if given, their labels */
+__global__ void k_gather(const uint8_t* __restrict__ q, const int* __restrict__ idx, const int* __restrict__ lab_src,
+                         int n, uint8_t* __restrict__ qd, int* __restrict__ lab)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = idx ? idx[i] : i;
+    const uint4* s4 = (const uint4*)(q + (long long)r * 32);
+    uint4* d4 = (uint4*)(qd + (long long)i * 32);
+    d4[0] = s4[0];
+    d4[1] = s4[1];
+    if (lab && lab_src) lab[i] = lab_src[r];
+}
+
+/* one wavefront per query: the nprobe nearest centroids in Hamming distance
+   (ties to the lower bucket id) -- the probe without the network */
+__global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restrict__ q, int nq,
+                                                        const uint8_t* __restrict__ cen, int nb, int k,
+                                                        int* __restrict__ probes, float* __restrict__ pmax)
+{
+    __shared__ uint4 s_c[1024 * 2];
+    for (int i = threadIdx.x; i < nb * 2; i += 256) s_c[i] = ((const uint4*)cen)[i];
+    __syncthreads();
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= nq) return;
+    const uint4 qa = ((const uint4*)(q + (long long)r * 32))[0], qb = ((const uint4*)(q + (long long)r * 32))[1];
+    constexpr int kPer = 16;
+    int d[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; t++) {
+        const int c = lane + 64 * t;
+        if (c < nb) {
+            const uint4 a = s_c[2 * c], b = s_c[2 * c + 1];
+            d[t] = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
+                   __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
+        } else {
+            d[t] = 1 << 20;
+        }
+    }
+    for (int j = 0; j < k; j++) {
+        int bv = 1 << 21, bi = 0x7fffffff;
+#pragma unroll
+        for (int t = 0; t < kPer; t++) {
+            const int c = lane + 64 * t;
+            if (d[t] < bv || (d[t] == bv && c < bi)) { bv = d[t]; bi = c; }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const int ov = __shfl_xor(bv, o, 64), oi = __shfl_xor(bi, o, 64);
+            if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (lane == 0) probes[(long long)r * k + j] = bi;
+        if ((bi & 63) == lane) d[bi >> 6] = 1 << 20;
+    }
+    if (lane == 0) pmax[r] = 1.f;
+}
+
+/* ---------------------------------------------------------------- launchers */
+
+hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, int n, uint8_t* qd, int* lab,
+                         hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, q, idx, lab_src, n, qd, lab);
+    return hipGetLastError();
+}
+
+hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
+                                 float* pmax, hipStream_t s)
+{
+    if (nq > 0)
+        hipLaunchKernelGGL(k_centroid_probe, dim3((nq + 3) / 4), dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+    return hipGetLastError();
+}
+
+
+static inline dim3 grid1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t launch_bucket_hist(const int* assign, int n, int* hist, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_bucket_hist, grid1(n), dim3(256), 0, s, assign, n, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_offsets(const int* hist, int nb, int* offsets, int* cursor, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, hist, nb, offsets, cursor, 0, (int*)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_scatter(const int* assign, int n, const uint8_t* db, int* cursor, int* perm,
+                                 uint8_t* db_sorted, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_bucket_scatter, grid1(n), dim3(256), 0, s, assign, n, db, cursor, perm, db_sorted);
+    return hipGetLastError();
+}
+
+hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, int nb, uint8_t* centroids,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_kmodes_update, dim3(nb), dim3(256), 0, s, db_sorted, offsets, centroids);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, int* list_count, int* list_offsets,
+                              int* list_cursor, int* lists, int* unit_offsets, hipStream_t s)
+{
+    const long long n = (long long)nq * nprobe;
+    hipLaunchKernelGGL(k_zero_int, grid1(nb), dim3(256), 0, s, list_count, nb);
+    if (n > 0) hipLaunchKernelGGL(k_probe_count, grid1(n), dim3(256), 0, s, probes, (int)n, list_count);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, list_count, nb, list_offsets, list_cursor, kLxQ,
+                       unit_offsets);
+    if (n > 0) hipLaunchKernelGGL(k_probe_fill, grid1(n), dim3(256), 0, s, probes, (int)n, list_cursor, lists);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
+                                const int* unit_offsets, int nb, const uint8_t* db_sorted, const int* perm,
+                                const int* offsets, uint2* partial, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bucket_search, dim3(blocks), dim3(256), 0, s, q, nprobe, lists, list_offsets, unit_offsets,
+                       nb, db_sorted, perm, offsets, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_fallback_compact(const uint8_t* q, int nq, const float* pmax, float thresh, int* fb_count,
+                                   int* fb_index, int* fb_queries_idx, uint8_t* fb_q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(256), 0, s, fb_count, 1);
+    if (nq > 0)
+        hipLaunchKernelGGL(k_fallback_compact, grid1(nq), dim3(256), 0, s, q, nq, pmax, thresh, fb_count, fb_index,
+                           fb_queries_idx, fb_q);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,
+                              const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1, int* d1, int* idx2,
+                              int* d2, hipStream_t s)
+{
+    if (nq > 0)
+        hipLaunchKernelGGL(k_probe_merge, grid1(nq), dim3(256), 0, s, partial, nq, nprobe, fb_index, ex_idx1, ex_d1,
+                           ex_idx2, ex_d2, idx1, d1, idx2, d2);
+    return hipGetLastError();
+}
+
+} // namespace orbk

@@ -1,0 +1,569 @@
+/*
+ * Host side of the learned bucket matcher (include/orb_capi.h, orb_lindex_*):
+ * k-modes partition of a descriptor database into buckets, training of the
+ * BF16 MLP that maps a query descriptor to the bucket of its exact nearest
+ * neighbour, and the probe-then-search query path with the exact matcher as
+ * fallback.  No counterpart in the reference (it has only the brute-force
+ * matcher, orbd.cu:1471-1559).
+ */
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "orb_capi.h"
+#include "orb_internal.h"
+#include "orb_mlp.h"
+
+using namespace orbk;
+
+/* internal helpers of orb_capi.cpp */
+int orb__ctx_stream(orb_ctx* ctx, void** stream);
+int orb__ctx_fail(orb_ctx* ctx, int code, const char* msg);
+int orb__match_dcount_top2(orb_ctx* ctx, const uint8_t* q, int nq_cap, const int* d_nq, const uint8_t* db, int ndb,
+                           int* idx1, int* d1, int* idx2, int* d2);
+
+namespace
+{
+
+constexpr int kIn = 256;
+
+struct Rng {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    bool ensure(size_t count)
+    {
+        if (count <= n) return true;
+        reset();
+        if (hipMalloc((void**)&p, sizeof(T) * (count ? count : 1)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        n = count;
+        return true;
+    }
+};
+
+} // namespace
+
+struct orb_lindex {
+    orb_ctx* ctx = nullptr;
+    orb_lindex_params prm{};
+    int H = 0, C = 0;
+    /* parameters: offsets of W1, b1, W2, b2, W3, b3 in one fp32 vector */
+    size_t off[7] = {};
+    DBuf<float> w, g, m, v;
+    DBuf<uint16_t> w16; /* bf16 copies of W1, W2, W3 (same offsets minus biases is awkward: full copy) */
+    int step = 0;
+    /* batch workspace */
+    int cap_batch = 0;
+    DBuf<uint16_t> X, A1, A2, dZ1, dZ2, dZ3;
+    DBuf<float> Z1, Z2, Z3, loss;
+    DBuf<int> labels, order;
+    /* database */
+    int ndb = 0;
+    DBuf<uint8_t> db, db_sorted, centroids;
+    DBuf<int> assign, hist, offsets, cursor, perm;
+    /* query workspace */
+    DBuf<int> probes, list_count, list_offsets, list_cursor, lists, unit_offsets, fb_count, fb_index, fb_qidx;
+    DBuf<int> ex_idx1, ex_d1, ex_idx2, ex_d2, tmp_idx;
+    DBuf<float> pmax;
+    DBuf<uint8_t> fb_q;
+    DBuf<uint2> partial;
+};
+
+namespace
+{
+
+hipStream_t stream_of(orb_lindex* L)
+{
+    void* s = nullptr;
+    orb__ctx_stream(L->ctx, &s);
+    return (hipStream_t)s;
+}
+
+int lfail(orb_lindex* L, int code, const char* msg) { return orb__ctx_fail(L ? L->ctx : nullptr, code, msg); }
+
+#define LTRY(L, call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            std::string m_ = std::string(#call) + ": " + hipGetErrorString(e_);         \
+            return lfail(L, e_ == hipErrorOutOfMemory ? ORB_E_NOMEM : ORB_E_HIP, m_.c_str()); \
+        }                                                                               \
+    } while (0)
+
+#define LALLOC(L, buf, n) \
+    do { if (!(buf).ensure(n)) return lfail(L, ORB_E_NOMEM, "device allocation failed"); } while (0)
+
+size_t nparams(const orb_lindex* L) { return L->off[6]; }
+
+int ensure_batch(orb_lindex* L, int B)
+{
+    if (B <= L->cap_batch) return ORB_OK;
+    const size_t b = (size_t)B;
+    LALLOC(L, L->X, b * kIn);
+    LALLOC(L, L->A1, b * L->H);
+    LALLOC(L, L->A2, b * L->H);
+    LALLOC(L, L->dZ1, b * L->H);
+    LALLOC(L, L->dZ2, b * L->H);
+    LALLOC(L, L->dZ3, b * L->C);
+    LALLOC(L, L->Z1, b * L->H);
+    LALLOC(L, L->Z2, b * L->H);
+    LALLOC(L, L->Z3, b * L->C);
+    LALLOC(L, L->loss, b);
+    LALLOC(L, L->labels, b);
+    L->cap_batch = B;
+    return ORB_OK;
+}
+
+GemmArgs gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B, long long sbk, long long sbn, int M,
+              int N, int K)
+{
+    GemmArgs g;
+    memset(&g, 0, sizeof(g));
+    g.A = A;
+    g.sam = sam;
+    g.sak = sak;
+    g.B = B;
+    g.sbk = sbk;
+    g.sbn = sbn;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.ldc = N;
+    return g;
+}
+
+/* forward of B rows already expanded in L->X: Z1, A1, Z2, A2, Z3 */
+int forward(orb_lindex* L, int B, hipStream_t s)
+{
+    const int H = L->H, C = L->C;
+    const float* w = L->w.p;
+    const uint16_t* w16 = L->w16.p;
+    GemmArgs g = gemm(L->X.p, kIn, 1, w16 + L->off[0], H, 1, B, H, kIn);
+    g.bias = w + L->off[1];
+    g.gelu = 1;
+    g.out32 = L->Z1.p;
+    g.out16 = L->A1.p;
+    LTRY(L, launch_gemm(g, s));
+    g = gemm(L->A1.p, H, 1, w16 + L->off[2], H, 1, B, H, H);
+    g.bias = w + L->off[3];
+    g.gelu = 1;
+    g.out32 = L->Z2.p;
+    g.out16 = L->A2.p;
+    LTRY(L, launch_gemm(g, s));
+    g = gemm(L->A2.p, H, 1, w16 + L->off[4], C, 1, B, C, H);
+    g.bias = w + L->off[5];
+    g.out32 = L->Z3.p;
+    LTRY(L, launch_gemm(g, s));
+    return ORB_OK;
+}
+
+/* backward into L->g after forward and labels; loss per row in L->loss */
+int backward(orb_lindex* L, int B, hipStream_t s)
+{
+    const int H = L->H, C = L->C;
+    const uint16_t* w16 = L->w16.p;
+    float* gr = L->g.p;
+    LTRY(L, launch_softmax_ce(L->Z3.p, L->labels.p, B, C, L->dZ3.p, L->loss.p, s));
+    GemmArgs g = gemm(L->A2.p, 1, H, L->dZ3.p, C, 1, H, C, B); /* dW3 = A2^T dZ3 */
+    g.out32 = gr + L->off[4];
+    LTRY(L, launch_gemm(g, s));
+    LTRY(L, launch_colsum(L->dZ3.p, B, C, gr + L->off[5], s));
+    g = gemm(L->dZ3.p, C, 1, w16 + L->off[4], 1, C, B, H, C); /* dZ2 = (dZ3 W3^T) gelu'(Z2) */
+    g.zmul = L->Z2.p;
+    g.out16 = L->dZ2.p;
+    LTRY(L, launch_gemm(g, s));
+    g = gemm(L->A1.p, 1, H, L->dZ2.p, H, 1, H, H, B); /* dW2 = A1^T dZ2 */
+    g.out32 = gr + L->off[2];
+    LTRY(L, launch_gemm(g, s));
+    LTRY(L, launch_colsum(L->dZ2.p, B, H, gr + L->off[3], s));
+    g = gemm(L->dZ2.p, H, 1, w16 + L->off[2], 1, H, B, H, H); /* dZ1 = (dZ2 W2^T) gelu'(Z1) */
+    g.zmul = L->Z1.p;
+    g.out16 = L->dZ1.p;
+    LTRY(L, launch_gemm(g, s));
+    g = gemm(L->X.p, 1, kIn, L->dZ1.p, H, 1, kIn, H, B); /* dW1 = X^T dZ1 */
+    g.out32 = gr + L->off[0];
+    LTRY(L, launch_gemm(g, s));
+    LTRY(L, launch_colsum(L->dZ1.p, B, H, gr + L->off[1], s));
+    return ORB_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int orb_lindex_create(orb_ctx* ctx, const orb_lindex_params* p, orb_lindex** out)
+{
+    if (!ctx || !p || !out) return orb__ctx_fail(ctx, ORB_E_ARG, "null argument");
+    *out = nullptr;
+    if (p->nbuckets < 64 || p->nbuckets > 1024 || p->nbuckets % 64)
+        return orb__ctx_fail(ctx, ORB_E_ARG, "nbuckets must be a multiple of 64 in [64, 1024]");
+    if (p->hidden < 64 || p->hidden > 4096 || p->hidden % 64)
+        return orb__ctx_fail(ctx, ORB_E_ARG, "hidden must be a multiple of 64 in [64, 4096]");
+    if (p->nprobe < 1 || p->nprobe > p->nbuckets) return orb__ctx_fail(ctx, ORB_E_ARG, "nprobe out of range");
+    orb_lindex* L = new orb_lindex();
+    L->ctx = ctx;
+    L->prm = *p;
+    L->H = p->hidden;
+    L->C = p->nbuckets;
+    const size_t sizes[6] = {(size_t)kIn * L->H, (size_t)L->H, (size_t)L->H * L->H, (size_t)L->H,
+                             (size_t)L->H * L->C, (size_t)L->C};
+    L->off[0] = 0;
+    for (int i = 0; i < 6; i++) L->off[i + 1] = L->off[i] + sizes[i];
+    const size_t np = nparams(L);
+    if (!L->w.ensure(np) || !L->g.ensure(np) || !L->m.ensure(np) || !L->v.ensure(np) || !L->w16.ensure(np)) {
+        delete L;
+        return orb__ctx_fail(ctx, ORB_E_NOMEM, "device allocation failed");
+    }
+    /* uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases */
+    std::vector<float> h(np, 0.f);
+    Rng rng{p->seed * 0x2545F4914F6CDD1Dull + 11};
+    const int fan_in[3] = {kIn, L->H, L->H};
+    for (int t = 0; t < 3; t++) {
+        const double a = 1.0 / std::sqrt((double)fan_in[t]);
+        for (size_t i = L->off[2 * t]; i < L->off[2 * t + 1]; i++) h[i] = (float)((2.0 * rng.uniform() - 1.0) * a);
+    }
+    int rc = orb_lindex_set_weights(L, h.data());
+    if (rc != ORB_OK) {
+        delete L;
+        return rc;
+    }
+    *out = L;
+    return ORB_OK;
+}
+
+int orb_lindex_destroy(orb_lindex* L)
+{
+    if (!L) return ORB_OK;
+    (void)hipStreamSynchronize(stream_of(L));
+    delete L;
+    return ORB_OK;
+}
+
+int orb_lindex_num_params(const orb_lindex* L)
+{
+    return L ? (int)nparams(L) : 0;
+}
+
+int orb_lindex_set_weights(orb_lindex* L, const float* h_w)
+{
+    if (!L || !h_w) return lfail(L, ORB_E_ARG, "null argument");
+    hipStream_t s = stream_of(L);
+    const size_t np = nparams(L);
+    LTRY(L, hipMemcpyAsync(L->w.p, h_w, sizeof(float) * np, hipMemcpyHostToDevice, s));
+    LTRY(L, hipMemsetAsync(L->m.p, 0, sizeof(float) * np, s));
+    LTRY(L, hipMemsetAsync(L->v.p, 0, sizeof(float) * np, s));
+    LTRY(L, launch_to_bf16(L->w.p, L->w16.p, (long long)np, s));
+    LTRY(L, hipStreamSynchronize(s));
+    L->step = 0;
+    return ORB_OK;
+}
+
+int orb_lindex_get_weights(orb_lindex* L, float* h_w)
+{
+    if (!L || !h_w) return lfail(L, ORB_E_ARG, "null argument");
+    hipStream_t s = stream_of(L);
+    LTRY(L, hipMemcpyAsync(h_w, L->w.p, sizeof(float) * nparams(L), hipMemcpyDeviceToHost, s));
+    LTRY(L, hipStreamSynchronize(s));
+    return ORB_OK;
+}
+
+int orb_lindex_build(orb_lindex* L, const uint8_t* d_db, int ndb, int iters)
+{
+    if (!L || !d_db || ndb <= 0) return lfail(L, ORB_E_ARG, "bad database");
+    if (ndb >= (1 << 23)) return lfail(L, ORB_E_CAPACITY, "database above 2^23-1 entries");
+    if (iters < 0) return lfail(L, ORB_E_ARG, "iters must be >= 0");
+    hipStream_t s = stream_of(L);
+    const int nb = L->C;
+    LALLOC(L, L->db, (size_t)ndb * 32);
+    LALLOC(L, L->db_sorted, (size_t)ndb * 32);
+    LALLOC(L, L->assign, ndb);
+    LALLOC(L, L->perm, ndb);
+    LALLOC(L, L->hist, nb);
+    LALLOC(L, L->offsets, nb + 1);
+    LALLOC(L, L->cursor, nb);
+    const bool have_centroids = L->centroids.p != nullptr && L->ndb == -1;
+    LALLOC(L, L->centroids, (size_t)nb * 32);
+    LTRY(L, hipMemcpyAsync(L->db.p, d_db, (size_t)ndb * 32, hipMemcpyDeviceToDevice, s));
+    if (!have_centroids) {
+        /* initial centroids: nb distinct database rows, evenly spaced with a seeded offset */
+        Rng rng{L->prm.seed * 0x9E3779B97F4A7C15ull + 5};
+        std::vector<int> pick(nb);
+        const double step = (double)ndb / nb;
+        for (int b = 0; b < nb; b++) pick[b] = (int)((b + rng.uniform()) * step) % ndb;
+        for (int b = 0; b < nb; b++)
+            LTRY(L, hipMemcpyAsync(L->centroids.p + (size_t)b * 32, d_db + (size_t)pick[b] * 32, 32,
+                                   hipMemcpyDeviceToDevice, s));
+    }
+    L->ndb = ndb;
+    /* Lloyd iterations in Hamming space: assign with the exact matcher, then
+       bucket-wise bit majority; a final assignment builds the bucketed copy */
+    for (int it = 0; it <= iters; it++) {
+        int rc = orb_match_top2(L->ctx, L->db.p, ndb, L->centroids.p, nb, ORB_MATCH_MFMA, L->assign.p, nullptr,
+                                nullptr, nullptr);
+        if (rc != ORB_OK) return rc;
+        LTRY(L, hipMemsetAsync(L->hist.p, 0, sizeof(int) * nb, s));
+        LTRY(L, launch_bucket_hist(L->assign.p, ndb, L->hist.p, s));
+        LTRY(L, launch_bucket_offsets(L->hist.p, nb, L->offsets.p, L->cursor.p, s));
+        LTRY(L, launch_bucket_scatter(L->assign.p, ndb, L->db.p, L->cursor.p, L->perm.p, L->db_sorted.p, s));
+        if (it < iters) LTRY(L, launch_kmodes_update(L->db_sorted.p, L->offsets.p, nb, L->centroids.p, s));
+    }
+    LTRY(L, hipStreamSynchronize(s));
+    return ORB_OK;
+}
+
+int orb_lindex_bucket_info(orb_lindex* L, int* h_offsets, uint8_t* h_centroids, int* d_bucket_of)
+{
+    if (!L || L->ndb <= 0) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
+    hipStream_t s = stream_of(L);
+    if (h_offsets)
+        LTRY(L, hipMemcpyAsync(h_offsets, L->offsets.p, sizeof(int) * (L->C + 1), hipMemcpyDeviceToHost, s));
+    if (h_centroids)
+        LTRY(L, hipMemcpyAsync(h_centroids, L->centroids.p, (size_t)L->C * 32, hipMemcpyDeviceToHost, s));
+    if (d_bucket_of)
+        LTRY(L, hipMemcpyAsync(d_bucket_of, L->assign.p, sizeof(int) * L->ndb, hipMemcpyDeviceToDevice, s));
+    LTRY(L, hipStreamSynchronize(s));
+    return ORB_OK;
+}
+
+int orb_lindex_grads(orb_lindex* L, const uint8_t* d_q, const int* d_labels, int n, float* h_grads, float* h_loss)
+{
+    if (!L || !d_q || !d_labels || n <= 0 || n % 64) return lfail(L, ORB_E_ARG, "n must be a positive multiple of 64");
+    hipStream_t s = stream_of(L);
+    int rc = ensure_batch(L, n);
+    if (rc != ORB_OK) return rc;
+    LALLOC(L, L->fb_q, (size_t)n * 32);
+    LTRY(L, launch_gather(d_q, nullptr, d_labels, n, L->fb_q.p, L->labels.p, s));
+    LTRY(L, launch_bits_pm1(L->fb_q.p, n, L->X.p, s));
+    rc = forward(L, n, s);
+    if (rc != ORB_OK) return rc;
+    rc = backward(L, n, s);
+    if (rc != ORB_OK) return rc;
+    if (h_grads) LTRY(L, hipMemcpyAsync(h_grads, L->g.p, sizeof(float) * nparams(L), hipMemcpyDeviceToHost, s));
+    std::vector<float> lv(n);
+    LTRY(L, hipMemcpyAsync(lv.data(), L->loss.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    LTRY(L, hipStreamSynchronize(s));
+    if (h_loss) {
+        double t = 0;
+        for (float x : lv) t += x;
+        *h_loss = (float)(t / n);
+    }
+    return ORB_OK;
+}
+
+int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int n, int epochs, int batch, float lr,
+                     float* h_epoch_loss)
+{
+    if (!L || !d_q || !d_target || n < batch || batch <= 0 || batch % 64 || epochs < 1)
+        return lfail(L, ORB_E_ARG, "bad training arguments (batch a multiple of 64, n >= batch)");
+    if (L->ndb <= 0) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
+    hipStream_t s = stream_of(L);
+    int rc = ensure_batch(L, batch);
+    if (rc != ORB_OK) return rc;
+    LALLOC(L, L->order, n);
+    LALLOC(L, L->fb_q, (size_t)batch * 32);
+    LALLOC(L, L->tmp_idx, n);
+    /* labels: bucket of each target database entry */
+    {
+        std::vector<int> tgt(n), asg(L->ndb);
+        LTRY(L, hipMemcpyAsync(tgt.data(), d_target, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+        LTRY(L, hipMemcpyAsync(asg.data(), L->assign.p, sizeof(int) * L->ndb, hipMemcpyDeviceToHost, s));
+        LTRY(L, hipStreamSynchronize(s));
+        for (int i = 0; i < n; i++) {
+            if (tgt[i] < 0 || tgt[i] >= L->ndb) return lfail(L, ORB_E_ARG, "target index outside the database");
+            tgt[i] = asg[tgt[i]];
+        }
+        LTRY(L, hipMemcpyAsync(L->tmp_idx.p, tgt.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        LTRY(L, hipStreamSynchronize(s));
+    }
+    Rng rng{L->prm.seed * 0xD1B54A32D192ED03ull + 3};
+    std::vector<int> ord(n);
+    const int nbatch = n / batch;
+    std::vector<float> lv(batch);
+    for (int e = 0; e < epochs; e++) {
+        for (int i = 0; i < n; i++) ord[i] = i;
+        for (int i = n - 1; i > 0; i--) {
+            const int j = (int)(rng.next() % (uint64_t)(i + 1));
+            const int t = ord[i];
+            ord[i] = ord[j];
+            ord[j] = t;
+        }
+        LTRY(L, hipMemcpyAsync(L->order.p, ord.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        double loss_sum = 0;
+        for (int bi = 0; bi < nbatch; bi++) {
+            LTRY(L, launch_gather(d_q, L->order.p + (size_t)bi * batch, L->tmp_idx.p, batch, L->fb_q.p,
+                                  L->labels.p, s));
+            LTRY(L, launch_bits_pm1(L->fb_q.p, batch, L->X.p, s));
+            rc = forward(L, batch, s);
+            if (rc != ORB_OK) return rc;
+            rc = backward(L, batch, s);
+            if (rc != ORB_OK) return rc;
+            L->step++;
+            LTRY(L, launch_adam(L->w.p, L->w16.p, L->g.p, L->m.p, L->v.p, (long long)nparams(L), lr, 0.9f, 0.999f,
+                                1e-8f, L->step, s));
+            if (h_epoch_loss && (bi == nbatch - 1)) {
+                LTRY(L, hipMemcpyAsync(lv.data(), L->loss.p, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
+                LTRY(L, hipStreamSynchronize(s));
+                for (float x : lv) loss_sum += x;
+            }
+        }
+        if (h_epoch_loss) h_epoch_loss[e] = (float)(loss_sum / batch);
+    }
+    LTRY(L, hipStreamSynchronize(s));
+    return ORB_OK;
+}
+
+int orb_lindex_logits(orb_lindex* L, const uint8_t* d_q, int n, float* d_logits)
+{
+    if (!L || !d_q || !d_logits || n <= 0 || n % 64) return lfail(L, ORB_E_ARG, "n must be a positive multiple of 64");
+    hipStream_t s = stream_of(L);
+    const int chunk = 8192;
+    int rc = ensure_batch(L, n < chunk ? n : chunk);
+    if (rc != ORB_OK) return rc;
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int B = n - c0 < chunk ? n - c0 : chunk;
+        LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
+        rc = forward(L, B, s);
+        if (rc != ORB_OK) return rc;
+        LTRY(L, hipMemcpyAsync(d_logits + (size_t)c0 * L->C, L->Z3.p, sizeof(float) * B * L->C,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    return ORB_OK;
+}
+
+int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d_idx1, int* d_d1, int* d_idx2,
+                     int* d_d2, int* h_nfallback)
+{
+    if (!L || !d_q || nq <= 0 || nq % 64) return lfail(L, ORB_E_ARG, "nq must be a positive multiple of 64");
+    if (L->ndb <= 0) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
+    if (mode != ORB_LINDEX_MLP && mode != ORB_LINDEX_CENTROIDS) return lfail(L, ORB_E_ARG, "unknown mode");
+    hipStream_t s = stream_of(L);
+    const int np = L->prm.nprobe, nb = L->C;
+    LALLOC(L, L->probes, (size_t)nq * np);
+    LALLOC(L, L->pmax, nq);
+    LALLOC(L, L->list_count, nb);
+    LALLOC(L, L->list_offsets, nb + 1);
+    LALLOC(L, L->list_cursor, nb);
+    LALLOC(L, L->unit_offsets, nb + 1);
+    LALLOC(L, L->lists, (size_t)nq * np);
+    LALLOC(L, L->partial, (size_t)nq * np);
+    LALLOC(L, L->fb_count, 1);
+    LALLOC(L, L->fb_index, nq);
+    LALLOC(L, L->fb_qidx, nq);
+    LALLOC(L, L->fb_q, (size_t)nq * 32);
+    LALLOC(L, L->ex_idx1, nq);
+    LALLOC(L, L->ex_d1, nq);
+    LALLOC(L, L->ex_idx2, nq);
+    LALLOC(L, L->ex_d2, nq);
+    if (mode == ORB_LINDEX_MLP) {
+        /* network forward in chunks, top-nprobe buckets and max softmax */
+        const int chunk = 8192;
+        int rc = ensure_batch(L, nq < chunk ? nq : chunk);
+        if (rc != ORB_OK) return rc;
+        for (int c0 = 0; c0 < nq; c0 += chunk) {
+            const int B = nq - c0 < chunk ? nq - c0 : chunk;
+            LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
+            rc = forward(L, B, s);
+            if (rc != ORB_OK) return rc;
+            LTRY(L, launch_probe(L->Z3.p, B, nb, np, L->probes.p + (size_t)c0 * np, L->pmax.p + c0, s));
+        }
+    } else {
+        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, L->probes.p, L->pmax.p, s));
+    }
+    LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->list_count.p, L->list_offsets.p, L->list_cursor.p,
+                               L->lists.p, L->unit_offsets.p, s));
+    LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, nb, L->db_sorted.p,
+                                 L->perm.p, L->offsets.p, L->partial.p, 2048, s));
+    const float thresh = mode == ORB_LINDEX_MLP ? L->prm.fallback : -1.f;
+    LTRY(L, launch_fallback_compact(d_q, nq, L->pmax.p, thresh, L->fb_count.p, L->fb_index.p, L->fb_qidx.p,
+                                    L->fb_q.p, s));
+    /* exact search of the low-confidence queries (count read on the device) */
+    int rc = orb__match_dcount_top2(L->ctx, L->fb_q.p, nq, L->fb_count.p, L->db.p, L->ndb, L->ex_idx1.p, L->ex_d1.p,
+                                    L->ex_idx2.p, L->ex_d2.p);
+    if (rc != ORB_OK) return rc;
+    LTRY(L, launch_probe_merge(L->partial.p, nq, np, L->fb_index.p, L->ex_idx1.p, L->ex_d1.p, L->ex_idx2.p,
+                               L->ex_d2.p, d_idx1, d_d1, d_idx2, d_d2, s));
+    if (h_nfallback) {
+        LTRY(L, hipMemcpyAsync(h_nfallback, L->fb_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        LTRY(L, hipStreamSynchronize(s));
+    }
+    return ORB_OK;
+}
+
+/* File: "ORBLIDX1", int32 version=1, nbuckets, hidden, nprobe, float fallback,
+   then all fp32 parameters (W1 b1 W2 b2 W3 b3, row-major), then the
+   nbuckets x 32 centroid bytes. */
+int orb_lindex_save(orb_lindex* L, const char* path)
+{
+    if (!L || !path) return lfail(L, ORB_E_ARG, "null argument");
+    if (L->ndb <= 0) return lfail(L, ORB_E_STATE, "nothing to save before orb_lindex_build");
+    std::vector<float> w(nparams(L));
+    std::vector<uint8_t> cen((size_t)L->C * 32);
+    int rc = orb_lindex_get_weights(L, w.data());
+    if (rc != ORB_OK) return rc;
+    rc = orb_lindex_bucket_info(L, nullptr, cen.data(), nullptr);
+    if (rc != ORB_OK) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) return lfail(L, ORB_E_ARG, "cannot open the file for writing");
+    const int32_t hdr[4] = {1, L->C, L->H, L->prm.nprobe};
+    const bool ok = fwrite("ORBLIDX1", 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
+                    fwrite(&L->prm.fallback, sizeof(float), 1, f) == 1 &&
+                    fwrite(w.data(), sizeof(float), w.size(), f) == w.size() &&
+                    fwrite(cen.data(), 1, cen.size(), f) == cen.size();
+    fclose(f);
+    return ok ? ORB_OK : lfail(L, ORB_E_ARG, "write failed");
+}
+
+int orb_lindex_load(orb_lindex* L, const char* path)
+{
+    if (!L || !path) return lfail(L, ORB_E_ARG, "null argument");
+    FILE* f = fopen(path, "rb");
+    if (!f) return lfail(L, ORB_E_ARG, "cannot open the file");
+    char magic[8];
+    int32_t hdr[4];
+    float fb;
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ORBLIDX1", 8) == 0 && fread(hdr, sizeof(hdr), 1, f) == 1 &&
+              hdr[0] == 1 && hdr[1] == L->C && hdr[2] == L->H && fread(&fb, sizeof(float), 1, f) == 1;
+    std::vector<float> w(nparams(L));
+    std::vector<uint8_t> cen((size_t)L->C * 32);
+    ok = ok && fread(w.data(), sizeof(float), w.size(), f) == w.size() && fread(cen.data(), 1, cen.size(), f) == cen.size();
+    fclose(f);
+    if (!ok) return lfail(L, ORB_E_ARG, "not a matching learned-index file (magic, version or shape)");
+    int rc = orb_lindex_set_weights(L, w.data());
+    if (rc != ORB_OK) return rc;
+    LALLOC(L, L->centroids, (size_t)L->C * 32);
+    hipStream_t s = stream_of(L);
+    LTRY(L, hipMemcpyAsync(L->centroids.p, cen.data(), cen.size(), hipMemcpyHostToDevice, s));
+    LTRY(L, hipStreamSynchronize(s));
+    L->prm.nprobe = hdr[3];
+    L->prm.fallback = fb;
+    L->ndb = -1; /* centroids loaded: orb_lindex_build(db, n, 0) assigns the database to them */
+    return ORB_OK;
+}
+
+} // extern "C"

@@ -2,63 +2,78 @@
  * Exact brute-force Hamming matchers for 256-bit descriptors (gfx950).
  *
  * The reference's gHammingMatch (orbd.cu:1471-1559) gives each query a
- * 32-thread block that re-reads the whole train set.  Here a workgroup holds
- * 256 queries (one per lane) in registers and streams a chunk of the
- * database through LDS; the database is split into chunks across workgroups
- * so even a few thousand queries fill the 256 CUs, and a merge pass combines
- * the per-chunk results.
+ * 32-thread block that re-reads the whole train set.  Here both matchers cut
+ * the (query, database) plane into units of a query tile x a database chunk,
+ * sized on the device from the real counts so a grid launched for the
+ * capacities still spreads the actual work over every CU; a merge pass
+ * combines the per-chunk results.
  *
  * Each (query, entry) pair becomes a packed key (distance << 23 | index): the
  * minimum key is the nearest entry with ties to the lowest index, and the
  * running top-2 is (min(k1, key), med3(k1, key, k2)).
+ *
+ *   popcount: a wavefront holds 64 queries (one per lane) and streams its
+ *     chunk with wave-uniform (scalar) loads: 8 x (xor, popcount) per pair.
+ *   MFMA: the distance is an exact integer contraction,
+ *     popcount(a ^ b) = (256 - <a', b'>) / 2 with a', b' the +/-1 expansions
+ *     of the bits, computed by v_mfma_i32_16x16x64_i8 (INT8 in, INT32
+ *     accumulate).  A workgroup keeps 256 queries as +/-1 B operands in
+ *     registers and unpacks each 64-entry database tile into LDS once.
  */
 #include "orb_internal.h"
+#include "orb_match_common.h"
 
 namespace orbk
 {
 
-constexpr int kMatchQ = 256;       /* queries per workgroup */
-constexpr int kMatchTile = 256;    /* database entries per LDS tile */
-constexpr uint32_t kIdxBits = 23;
-constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
-constexpr uint32_t kNoKey = 0xffffffffu;
+constexpr int kGranule = 64;          /* database entries per chunk granule */
+constexpr int kPopQ = 64;             /* popcount: queries per wavefront */
+constexpr int kMfmaQ = 256;           /* MFMA: queries per workgroup */
+constexpr int kMfmaRow = 256 + 16;    /* LDS row of one unpacked entry (padded) */
 
-__device__ __forceinline__ void top2_push(uint32_t& k1, uint32_t& k2, uint32_t key)
+/* Split nq queries x ndb entries into qunits x chunks units for `slots`
+   parallel workers (identical on host and device). */
+struct Plan {
+    int qunits, chunks, chunk;
+};
+
+__host__ __device__ inline Plan make_plan(int nq, int ndb, int qtile, int slots)
 {
-    k2 = min(k2, max(k1, key)); /* med3(k1, key, k2) since k1 <= k2 */
-    k1 = min(k1, key);
+    Plan p;
+    p.qunits = (nq + qtile - 1) / qtile;
+    const int granules = (ndb + kGranule - 1) / kGranule;
+    int c = p.qunits > 0 ? slots / p.qunits : 1;
+    if (c > granules) c = granules;
+    if (c < 1) c = 1;
+    p.chunk = granules > 0 ? ((granules + c - 1) / c) * kGranule : kGranule;
+    p.chunks = ndb > 0 ? (ndb + p.chunk - 1) / p.chunk : 0;
+    return p;
 }
 
+/* ----------------------------------------------------------- popcount */
+
 __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q, int nq_cap,
-                                                   const int* __restrict__ d_nq, const uint8_t* __restrict__ db,
-                                                   int ndb_cap, const int* __restrict__ d_ndb, int chunk,
-                                                   uint32_t* __restrict__ partial)
+                                                   const int* __restrict__ d_nq, const uint4* __restrict__ db,
+                                                   int ndb_cap, const int* __restrict__ d_ndb, int slots,
+                                                   uint2* __restrict__ partial)
 {
-    __shared__ uint4 s_db[kMatchTile][2];
     const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
     const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
-    const int qi = blockIdx.x * kMatchQ + threadIdx.x;
-    if ((int)(blockIdx.x * kMatchQ) >= nq) return;
-    const int j0 = blockIdx.y * chunk, j1 = min(ndb, j0 + chunk);
-    uint4 qa = make_uint4(0, 0, 0, 0), qb = qa;
-    if (qi < nq) {
-        const uint4* qp = (const uint4*)(q + (long long)qi * 32);
-        qa = qp[0];
-        qb = qp[1];
-    }
-    uint32_t k1 = kNoKey, k2 = kNoKey;
-    for (int jt = j0; jt < j1; jt += kMatchTile) {
-        __syncthreads();
-        const int j = jt + threadIdx.x;
-        if (j < j1) {
-            const uint4* dp = (const uint4*)(db + (long long)j * 32);
-            s_db[threadIdx.x][0] = dp[0];
-            s_db[threadIdx.x][1] = dp[1];
+    const Plan P = make_plan(nq, ndb, kPopQ, slots);
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    for (int u = wave; u < P.qunits * P.chunks; u += slots) {
+        const int qu = u % P.qunits, c = u / P.qunits;
+        const int qi = qu * kPopQ + lane;
+        uint4 qa = make_uint4(0, 0, 0, 0), qb = qa;
+        if (qi < nq) {
+            const uint4* qp = (const uint4*)(q + (long long)qi * 32);
+            qa = qp[0];
+            qb = qp[1];
         }
-        __syncthreads();
-        const int nt = min(kMatchTile, j1 - jt);
-        for (int t = 0; t < nt; t++) {
-            const uint4 a = s_db[t][0], b = s_db[t][1];
+        const int j0 = c * P.chunk, j1 = min(ndb, j0 + P.chunk);
+        uint32_t k1 = kNoKey, k2 = kNoKey;
+        auto pair = [&](uint4 a, uint4 b, int j) {
             uint32_t d = __popc(qa.x ^ a.x);
             d = __popc(qa.y ^ a.y) + d;
             d = __popc(qa.z ^ a.z) + d;
@@ -67,96 +82,219 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
             d = __popc(qb.y ^ b.y) + d;
             d = __popc(qb.z ^ b.z) + d;
             d = __popc(qb.w ^ b.w) + d;
-            top2_push(k1, k2, (d << kIdxBits) | (uint32_t)(jt + t));
+            top2_push(k1, k2, (d << kIdxBits) | (uint32_t)j);
+        };
+        int j = j0;
+        /* the entry index is wave-uniform: readfirstlane lets the compiler
+           fetch four entries with scalar loads before using them */
+        for (; j + 4 <= j1; j += 4) {
+            const int ju = __builtin_amdgcn_readfirstlane(j);
+            uint4 e[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) e[t] = db[2 * ju + t];
+#pragma unroll
+            for (int t = 0; t < 4; t++) pair(e[2 * t], e[2 * t + 1], ju + t);
         }
-    }
-    if (qi < nq) {
-        uint2* o = (uint2*)partial + (long long)blockIdx.y * nq_cap + qi;
-        *o = make_uint2(k1, k2);
+        for (; j < j1; j++) {
+            const int ju = __builtin_amdgcn_readfirstlane(j);
+            pair(db[2 * ju], db[2 * ju + 1], ju);
+        }
+        if (qi < nq) partial[(long long)c * nq + qi] = make_uint2(k1, k2);
     }
 }
 
+/* --------------------------------------------------------------- MFMA */
+
+__global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ q, int nq_cap,
+                                                    const int* __restrict__ d_nq, const uint8_t* __restrict__ db,
+                                                    int ndb_cap, const int* __restrict__ d_ndb, int slots,
+                                                    uint2* __restrict__ partial)
+{
+    __shared__ __align__(16) int8_t s_db[kGranule * kMfmaRow];
+    const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+    const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
+    const Plan P = make_plan(nq, ndb, kMfmaQ, slots);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int g = lane >> 4, col = lane & 15;
+    for (int u = blockIdx.x; u < P.qunits * P.chunks; u += slots) {
+        const int qu = u % P.qunits, c = u / P.qunits;
+        /* B operands: query (qb*16 + col) of this wave's 64, k-group g */
+        v4i bq[4][4];
+        int qidx[4];
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+            const int qi = qu * kMfmaQ + wid * 64 + qb * 16 + col;
+            qidx[qb] = qi;
+            uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+            if (qi < nq) {
+                const uint4* qp = (const uint4*)(q + (long long)qi * 32);
+                d0 = qp[0];
+                d1 = qp[1];
+            }
+            query_operands(d0, d1, g, bq[qb]);
+        }
+        uint32_t k1[4], k2[4];
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) k1[qb] = k2[qb] = kNoKey;
+        const int j0 = c * P.chunk, j1 = min(ndb, j0 + P.chunk);
+        for (int jt = j0; jt < j1; jt += kGranule) {
+            __syncthreads();
+            {
+                /* unpack 64 entries: thread -> entry tid/4, bits 64*(tid%4) .. +63 */
+                const int e = tid >> 2, qq = tid & 3, j = jt + e;
+                uint32_t lo = 0, hi = 0;
+                if (j < j1) {
+                    const uint2 v = *(const uint2*)(db + (long long)j * 32 + qq * 8);
+                    lo = v.x;
+                    hi = v.y;
+                }
+                int8_t* dst = s_db + e * kMfmaRow + qq * 64;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t hword = k < 2 ? (lo >> (16 * k)) & 0xffffu : (hi >> (16 * (k - 2))) & 0xffffu;
+                    *(v4i*)(dst + 16 * k) = pm1_16(hword);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ab = 0; ab < 4; ab++) {
+                /* A operands: entry (ab*16 + col) of the tile, k-group g */
+                v4i a[4];
+                const int8_t* src = s_db + (ab * 16 + col) * kMfmaRow + 16 * g;
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(src + 64 * ks);
+                /* lane holds C[4g + r][col]: entries ab*16 + 4g + r, query qb*16 + col */
+                const int jb = jt + ab * 16 + 4 * g;
+#pragma unroll
+                for (int qb = 0; qb < 4; qb++) {
+                    v4i acc = {0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++)
+                        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int j = jb + r;
+                        if (j < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)j));
+                    }
+                }
+            }
+        }
+        /* merge the 4 k-groups that hold the same query */
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++) {
+#pragma unroll
+            for (int m = 16; m <= 32; m <<= 1) {
+                const uint32_t o1 = __shfl_xor(k1[qb], m, 64), o2 = __shfl_xor(k2[qb], m, 64);
+                top2_push(k1[qb], k2[qb], o1);
+                top2_push(k1[qb], k2[qb], o2);
+            }
+            if (g == 0 && qidx[qb] < nq) partial[(long long)c * nq + qidx[qb]] = make_uint2(k1[qb], k2[qb]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------- merge */
+
 /* merge the per-chunk top-2 keys of each query and write the outputs */
-__global__ __launch_bounds__(256) void k_match_merge(int nq_cap, const int* __restrict__ d_nq, int nchunks,
-                                                     int ndb_cap, const int* __restrict__ d_ndb, int chunk,
-                                                     const uint32_t* __restrict__ partial, int* __restrict__ idx1,
+__global__ __launch_bounds__(256) void k_match_merge(int nq_cap, const int* __restrict__ d_nq, int ndb_cap,
+                                                     const int* __restrict__ d_ndb, int qtile, int slots,
+                                                     const uint2* __restrict__ partial, int* __restrict__ idx1,
                                                      int* __restrict__ d1, int* __restrict__ idx2,
                                                      int* __restrict__ d2, orb::OrbPoint* __restrict__ pts)
 {
     const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
     const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
-    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= nq) return;
-    uint32_t k1 = kNoKey, k2 = kNoKey;
-    const int used = min(nchunks, (ndb + chunk - 1) / chunk);
-    for (int c = 0; c < used; c++) {
-        const uint2 v = ((const uint2*)partial)[(long long)c * nq_cap + qi];
-        top2_push(k1, k2, v.x);
-        top2_push(k1, k2, v.y);
-    }
-    const int a1 = k1 == kNoKey ? -1 : (int)(k1 & kIdxMask), b1 = k1 == kNoKey ? 257 : (int)(k1 >> kIdxBits);
-    const int a2 = k2 == kNoKey ? -1 : (int)(k2 & kIdxMask), b2 = k2 == kNoKey ? 257 : (int)(k2 >> kIdxBits);
-    if (idx1) idx1[qi] = a1;
-    if (d1) d1[qi] = b1;
-    if (idx2) idx2[qi] = a2;
-    if (d2) d2[qi] = b2;
-    if (pts) {
-        /* orbd.cu:1544-1558: both distances start at 256; accept a unique best below 64 */
-        const int s1 = min(b1, 256), s2 = min(b2, 256);
-        orb::OrbPoint* p = pts + qi;
-        if (s1 < s2 && s1 < 64) {
-            p->match = a1;
-            p->distance = s1;
-        } else {
-            p->match = -1;
-            p->distance = -1;
+    const Plan P = make_plan(nq, ndb, qtile, slots);
+    for (int qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
+        uint32_t k1 = kNoKey, k2 = kNoKey;
+        for (int c = 0; c < P.chunks; c++) {
+            const uint2 v = partial[(long long)c * nq + qi];
+            top2_push(k1, k2, v.x);
+            top2_push(k1, k2, v.y);
+        }
+        const int a1 = k1 == kNoKey ? -1 : (int)(k1 & kIdxMask), b1 = k1 == kNoKey ? 257 : (int)(k1 >> kIdxBits);
+        const int a2 = k2 == kNoKey ? -1 : (int)(k2 & kIdxMask), b2 = k2 == kNoKey ? 257 : (int)(k2 >> kIdxBits);
+        if (idx1) idx1[qi] = a1;
+        if (d1) d1[qi] = b1;
+        if (idx2) idx2[qi] = a2;
+        if (d2) d2[qi] = b2;
+        if (pts) {
+            /* orbd.cu:1544-1558: both distances start at 256; accept a unique best below 64 */
+            const int s1 = min(b1, 256), s2 = min(b2, 256);
+            orb::OrbPoint* p = pts + qi;
+            if (s1 < s2 && s1 < 64) {
+                p->match = a1;
+                p->distance = s1;
+            } else {
+                p->match = -1;
+                p->distance = -1;
+            }
         }
     }
 }
 
+/* ---------------------------------------------------------- launchers */
+
+constexpr int kPopBlocks = 2048;   /* 8192 wavefront slots */
+constexpr int kMfmaBlocks = 1024;  /* workgroup slots */
+
 size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk)
 {
-    (void)method;
-    const int qblocks = (nq + kMatchQ - 1) / kMatchQ;
-    const int tiles = (ndb + kMatchTile - 1) / kMatchTile;
-    int nc = (2048 + qblocks - 1) / (qblocks > 0 ? qblocks : 1);
-    if (nc > tiles) nc = tiles;
-    if (nc < 1) nc = 1;
-    int ch = ((tiles + nc - 1) / nc) * kMatchTile;
-    if (ch < kMatchTile) ch = kMatchTile;
-    nc = (ndb + ch - 1) / ch;
-    if (nc < 1) nc = 1;
-    *nchunks = nc;
-    *chunk = ch;
-    return (size_t)nc * (size_t)(nq > 0 ? nq : 1) * 2;
+    (void)ndb;
+    /* chunks * nq <= (slots / qunits + 1) * qunits * qtile <= (slots + qunits) * qtile */
+    const int qtile = method == 1 ? kMfmaQ : kPopQ;
+    const int slots = method == 1 ? kMfmaBlocks : kPopBlocks * 4;
+    const long long qunits = (nq + qtile - 1) / qtile;
+    *nchunks = 0;
+    *chunk = 0;
+    long long entries = (slots + qunits) * (long long)qtile;
+    if (entries < nq) entries = nq;
+    return (size_t)entries * 2;
 }
 
-static void launch_merge(const MatchLaunch& m, hipStream_t s, const StageEvents* ev)
+static void launch_merge(const MatchLaunch& m, int qtile, int slots, hipStream_t s, const StageEvents* ev)
 {
     ev_start(ev, kStMatchMerge, s);
-    hipLaunchKernelGGL(k_match_merge, dim3((m.nq + 255) / 256), dim3(256), 0, s, m.nq, m.d_nq, m.nchunks, m.ndb,
-                       m.d_ndb, m.chunk, m.partial, m.idx1, m.d1, m.idx2, m.d2, m.pts);
+    const int blocks = (m.nq + 255) / 256 < 1024 ? (m.nq + 255) / 256 : 1024;
+    hipLaunchKernelGGL(k_match_merge, dim3(blocks), dim3(256), 0, s, m.nq, m.d_nq, m.ndb, m.d_ndb, qtile, slots,
+                       (const uint2*)m.partial, m.idx1, m.d1, m.idx2, m.d2, m.pts);
     ev_stop(ev, kStMatchMerge, s);
 }
 
 hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const StageEvents* ev)
 {
+    (void)hipGetLastError(); /* report only errors of these launches */
     if (m.nq <= 0) return hipSuccess;
-    const dim3 grid((m.nq + kMatchQ - 1) / kMatchQ, m.nchunks);
+    const int slots = kPopBlocks * 4;
+    /* enough workgroups for the units the capacities allow, at most kPopBlocks */
+    const Plan P = make_plan(m.nq, m.ndb, kPopQ, slots);
+    const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
+    int blocks = (int)((units + 3) / 4);
+    if (blocks > kPopBlocks) blocks = kPopBlocks;
+    if (blocks < 1) blocks = 1;
     ev_start(ev, kStMatch, s);
-    hipLaunchKernelGGL(k_match_pop, grid, dim3(256), 0, s, m.q, m.nq, m.d_nq, m.db, m.ndb, m.d_ndb, m.chunk,
-                       m.partial);
+    hipLaunchKernelGGL(k_match_pop, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db, m.ndb,
+                       m.d_ndb, slots, (uint2*)m.partial);
     ev_stop(ev, kStMatch, s);
-    launch_merge(m, s, ev);
+    launch_merge(m, kPopQ, slots, s, ev);
     return hipGetLastError();
 }
 
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev)
 {
-    (void)m;
-    (void)s;
-    (void)ev;
-    return hipErrorNotSupported;
+    (void)hipGetLastError(); /* report only errors of these launches */
+    if (m.nq <= 0) return hipSuccess;
+    const int slots = kMfmaBlocks;
+    const Plan P = make_plan(m.nq, m.ndb, kMfmaQ, slots);
+    const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
+    int blocks = units < kMfmaBlocks ? (int)units : kMfmaBlocks;
+    if (blocks < 1) blocks = 1;
+    ev_start(ev, kStMatch, s);
+    hipLaunchKernelGGL(k_match_mfma, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, m.db, m.ndb, m.d_ndb, slots,
+                       (uint2*)m.partial);
+    ev_stop(ev, kStMatch, s);
+    launch_merge(m, kMfmaQ, slots, s, ev);
+    return hipGetLastError();
 }
 
 } // namespace orbk

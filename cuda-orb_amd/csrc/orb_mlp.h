@@ -1,0 +1,72 @@
+/*
+ * Internal interface of the learned bucket matcher kernels (orb_mlp.hip,
+ * orb_lindex.hip) used by the host code in orb_lindex.cpp.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orbk
+{
+
+/* C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak],
+   B(k, n) = B[k*sbk + n*sbn] (bf16 bits); M, N, K multiples of 64.
+   Epilogue, in order: z += bias[n]; z *= gelu'(zmul[m*ldc + n]) if zmul;
+   out32[m*ldc + n] = z; out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
+struct GemmArgs {
+    const uint16_t* A;
+    long long sam, sak;
+    const uint16_t* B;
+    long long sbk, sbn;
+    int M, N, K;
+    const float* bias;
+    int gelu;
+    const float* zmul;
+    float* out32;
+    uint16_t* out16;
+    long long ldc;
+};
+
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
+hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s);
+hipError_t launch_softmax_ce(const float* Z, const int* labels, int n, int C, uint16_t* dZ, float* loss,
+                             hipStream_t s);
+hipError_t launch_colsum(const uint16_t* X, int n, int C, float* out, hipStream_t s);
+hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
+                       float b2, float eps, int step, hipStream_t s);
+hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t s);
+hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s);
+
+/* bucketed database (orb_lindex.hip) */
+hipError_t launch_bucket_hist(const int* assign, int n, int* hist, hipStream_t s);
+hipError_t launch_bucket_offsets(const int* hist, int nb, int* offsets, int* cursor, hipStream_t s);
+hipError_t launch_bucket_scatter(const int* assign, int n, const uint8_t* db, int* cursor, int* perm,
+                                 uint8_t* db_sorted, hipStream_t s);
+hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, int nb, uint8_t* centroids,
+                                hipStream_t s);
+
+/* probe lists: queries x nprobe (bucket ids) -> per-bucket lists of q*nprobe+slot */
+hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, int* list_count, int* list_offsets,
+                              int* list_cursor, int* lists, int* unit_offsets, hipStream_t s);
+/* bucketed exact search: for every (query, probed bucket) the top-2 keys
+   (distance << 23 | original index) over that bucket's entries */
+hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
+                                const int* unit_offsets, int nb, const uint8_t* db_sorted, const int* perm,
+                                const int* offsets, uint2* partial, int blocks, hipStream_t s);
+/* per query: merge of the probed buckets, or the exact result when the query
+   was sent to the fallback (fb_index >= 0) */
+hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,
+                              const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1, int* d1, int* idx2,
+                              int* d2, hipStream_t s);
+/* rows idx[i] (or the first n) of q, with labels lab_src[idx[i]] when given */
+hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, int n, uint8_t* qd, int* lab,
+                         hipStream_t s);
+/* the k nearest centroids of each query (Hamming), ties to the lower id */
+hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
+                                 float* pmax, hipStream_t s);
+/* compaction of the low-confidence queries for the exact fallback */
+hipError_t launch_fallback_compact(const uint8_t* q, int nq, const float* pmax, float thresh, int* fb_count,
+                                   int* fb_index, int* fb_queries_idx, uint8_t* fb_q, hipStream_t s);
+
+} // namespace orbk

@@ -32,7 +32,13 @@ EXPORTS = [
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
     "orb_match_top2", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
+    "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
+    "orb_lindex_logits", "orb_lindex_grads", "orb_lindex_num_params", "orb_lindex_get_weights",
+    "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_save", "orb_lindex_load",
 ]
+
+ORB_LINDEX_MLP = 0
+ORB_LINDEX_CENTROIDS = 1
 
 NUM_STAGES = 8
 STAGES = ["pyramid", "detect", "nms", "select", "emit", "describe", "match", "match_merge"]
@@ -40,6 +46,11 @@ STAGES = ["pyramid", "detect", "nms", "select", "emit", "describe", "match", "ma
 
 class OrbError(RuntimeError):
     pass
+
+
+class LindexParams(ctypes.Structure):
+    _fields_ = [("nbuckets", ctypes.c_int), ("hidden", ctypes.c_int), ("nprobe", ctypes.c_int),
+                ("fallback", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 
 class Params(ctypes.Structure):
@@ -82,6 +93,19 @@ def lib():
         L.orb_debug_atan2.argtypes = [P, V, I, V]
         L.orb_profile_enable.argtypes = [P, ctypes.c_uint]
         L.orb_profile_read.argtypes = [P, V, V]
+        L.orb_lindex_create.argtypes = [P, ctypes.POINTER(LindexParams), ctypes.POINTER(P)]
+        L.orb_lindex_destroy.argtypes = [P]
+        L.orb_lindex_build.argtypes = [P, V, I, I]
+        L.orb_lindex_bucket_info.argtypes = [P, V, V, V]
+        L.orb_lindex_train.argtypes = [P, V, V, I, I, I, ctypes.c_float, V]
+        L.orb_lindex_logits.argtypes = [P, V, I, V]
+        L.orb_lindex_grads.argtypes = [P, V, V, I, V, V]
+        L.orb_lindex_num_params.argtypes = [P]
+        L.orb_lindex_get_weights.argtypes = [P, V]
+        L.orb_lindex_set_weights.argtypes = [P, V]
+        L.orb_lindex_query.argtypes = [P, V, I, I, V, V, V, V, V]
+        L.orb_lindex_save.argtypes = [P, ctypes.c_char_p]
+        L.orb_lindex_load.argtypes = [P, ctypes.c_char_p]
         _lib = L
     return _lib
 
@@ -244,6 +268,109 @@ class Context:
         out = self._torch.empty(yx.shape[0], dtype=self._torch.float32, device=yx.device)
         self._check(lib().orb_debug_atan2(self.h, _dptr(yx), yx.shape[0], _dptr(out)), "orb_debug_atan2")
         return out
+
+
+class LearnedIndex:
+    """Learned bucket matcher (orb_lindex_*): k-modes buckets over a descriptor database, a BF16 MLP
+    that routes queries to buckets, bucketed exact search with exact fallback."""
+
+    def __init__(self, ctx, nbuckets=1024, hidden=512, nprobe=8, fallback=0.3, seed=4):
+        self.ctx = ctx
+        torch = ctx._torch
+        self._torch = torch
+        self.prm = LindexParams(nbuckets, hidden, nprobe, fallback, seed)
+        h = ctypes.c_void_p()
+        ctx._bind_stream()
+        ctx._check(lib().orb_lindex_create(ctx.h, ctypes.byref(self.prm), ctypes.byref(h)), "orb_lindex_create")
+        self.h = h
+        self.nbuckets, self.hidden = nbuckets, hidden
+
+    def close(self):
+        if self.h:
+            lib().orb_lindex_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        self.ctx._check(rc, what)
+
+    def build(self, db, iters=8):
+        self.ctx._bind_stream()
+        self._c(lib().orb_lindex_build(self.h, _dptr(db), db.shape[0], iters), "orb_lindex_build")
+        self.ndb = db.shape[0]
+
+    def bucket_info(self):
+        torch = self._torch
+        off = np.zeros(self.nbuckets + 1, np.int32)
+        cen = np.zeros((self.nbuckets, 32), np.uint8)
+        bof = torch.empty(self.ndb, dtype=torch.int32, device=f"cuda:{self.ctx.device}")
+        self._c(lib().orb_lindex_bucket_info(self.h, _np_ptr(off), _np_ptr(cen), _dptr(bof)), "orb_lindex_bucket_info")
+        return off, cen, bof
+
+    def train(self, q, target, epochs=3, batch=4096, lr=1e-3):
+        self.ctx._bind_stream()
+        losses = np.zeros(epochs, np.float32)
+        self._c(lib().orb_lindex_train(self.h, _dptr(q), _dptr(target), q.shape[0], epochs, batch, lr,
+                                       _np_ptr(losses)), "orb_lindex_train")
+        return losses
+
+    def logits(self, q):
+        self.ctx._bind_stream()
+        out = self._torch.empty((q.shape[0], self.nbuckets), dtype=self._torch.float32, device=q.device)
+        self._c(lib().orb_lindex_logits(self.h, _dptr(q), q.shape[0], _dptr(out)), "orb_lindex_logits")
+        return out
+
+    def grads(self, q, labels):
+        self.ctx._bind_stream()
+        g = np.zeros(self.num_params(), np.float32)
+        loss = ctypes.c_float()
+        self._c(lib().orb_lindex_grads(self.h, _dptr(q), _dptr(labels), q.shape[0], _np_ptr(g), ctypes.byref(loss)),
+                "orb_lindex_grads")
+        return g, loss.value
+
+    def num_params(self):
+        return lib().orb_lindex_num_params(self.h)
+
+    def get_weights(self):
+        w = np.zeros(self.num_params(), np.float32)
+        self._c(lib().orb_lindex_get_weights(self.h, _np_ptr(w)), "orb_lindex_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        self._c(lib().orb_lindex_set_weights(self.h, _np_ptr(w)), "orb_lindex_set_weights")
+
+    def split(self, flat):
+        """Flat parameter vector -> dict W1 (256,H), b1, W2 (H,H), b2, W3 (H,C), b3."""
+        H, C = self.hidden, self.nbuckets
+        shapes = [("W1", (256, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H, C)), ("b3", (C,))]
+        out, o = {}, 0
+        for name, shp in shapes:
+            n = int(np.prod(shp))
+            out[name] = flat[o:o + n].reshape(shp)
+            o += n
+        return out
+
+    def query(self, q, mode=ORB_LINDEX_MLP, out=None):
+        self.ctx._bind_stream()
+        torch = self._torch
+        if out is None:
+            out = [torch.empty(q.shape[0], dtype=torch.int32, device=q.device) for _ in range(4)]
+        nfb = ctypes.c_int(0)
+        self._c(lib().orb_lindex_query(self.h, _dptr(q), q.shape[0], mode, *[_dptr(o) for o in out],
+                                       ctypes.byref(nfb)), "orb_lindex_query")
+        return out, nfb.value
+
+    def save(self, path):
+        self._c(lib().orb_lindex_save(self.h, str(path).encode()), "orb_lindex_save")
+
+    def load(self, path):
+        self._c(lib().orb_lindex_load(self.h, str(path).encode()), "orb_lindex_load")
 
 
 def points_to_numpy(pts_tensor, n):

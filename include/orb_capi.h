@@ -134,6 +134,63 @@ int orb_synth_image(uint64_t seed, int width, int height, int pitch, int nshapes
 int orb_synth_warp(uint64_t seed, const uint8_t* src, int width, int height, int pitch,
                    uint8_t* out, double* H);
 
+/* ------------------------------------------------------------------------
+   Learned bucket matcher (no counterpart in the reference).  A database of
+   256-bit descriptors is partitioned into buckets by k-modes in Hamming
+   space; a BF16 MLP (256 -> hidden -> hidden -> nbuckets, GELU, FP32
+   accumulate) trained on (query, exact nearest neighbour) pairs maps a query
+   to buckets; a query searches its nprobe most likely buckets exactly, and
+   falls back to the exact search of the whole database when the network's
+   largest softmax probability is below `fallback`.  Results follow
+   orb_match_top2's conventions (ties to the lowest database index). */
+typedef struct {
+    int nbuckets;    /* multiple of 64, <= 1024 */
+    int hidden;      /* multiple of 64 */
+    int nprobe;      /* buckets searched per query */
+    float fallback;  /* exact search when max softmax < fallback */
+    uint64_t seed;   /* weight initialisation, centroid choice, shuffling */
+} orb_lindex_params;
+
+typedef struct orb_lindex orb_lindex;
+
+#define ORB_LINDEX_MLP 0        /* probe the network's top buckets (with fallback) */
+#define ORB_LINDEX_CENTROIDS 1  /* probe the nearest centroids (no network, no fallback) */
+
+int orb_lindex_create(orb_ctx* ctx, const orb_lindex_params* params, orb_lindex** out);
+int orb_lindex_destroy(orb_lindex* lx);
+/* Copy a database (ndb device descriptors) and partition it: `iters` k-modes
+   steps from seeded initial centroids (or from loaded ones), then the final
+   assignment and the bucket-sorted copy.  Synchronous. */
+int orb_lindex_build(orb_lindex* lx, const uint8_t* d_db, int ndb, int iters);
+/* Bucket offsets (nbuckets + 1 host ints), centroids (nbuckets x 32 host
+   bytes) and the bucket of every database entry (ndb device ints); any NULL
+   output is skipped. */
+int orb_lindex_bucket_info(orb_lindex* lx, int* h_offsets, uint8_t* h_centroids, int* d_bucket_of);
+/* Train on n queries whose exact nearest database entries are d_target
+   (labels = their buckets): `epochs` shuffled passes of Adam steps with the
+   given batch (multiple of 64) and learning rate; h_epoch_loss (may be NULL)
+   receives the mean loss of each epoch's last batch.  Synchronous. */
+int orb_lindex_train(orb_lindex* lx, const uint8_t* d_q, const int* d_target, int n, int epochs, int batch,
+                     float lr, float* h_epoch_loss);
+/* Bucket logits of n queries (multiple of 64) into d_logits (n x nbuckets floats). */
+int orb_lindex_logits(orb_lindex* lx, const uint8_t* d_q, int n, float* d_logits);
+/* Gradients of the mean cross-entropy for n queries (multiple of 64) with
+   labels d_labels, without updating: all parameters (W1 b1 W2 b2 W3 b3,
+   row-major fp32) to h_grads, mean loss to *h_loss.  For checking. */
+int orb_lindex_grads(orb_lindex* lx, const uint8_t* d_q, const int* d_labels, int n, float* h_grads, float* h_loss);
+int orb_lindex_num_params(const orb_lindex* lx);
+int orb_lindex_get_weights(orb_lindex* lx, float* h_w);
+int orb_lindex_set_weights(orb_lindex* lx, const float* h_w);
+/* Nearest and second-nearest database entries of nq queries (multiple of 64);
+   *h_nfallback (may be NULL) receives how many went to the exact search. */
+int orb_lindex_query(orb_lindex* lx, const uint8_t* d_q, int nq, int mode, int* d_idx1, int* d_d1, int* d_idx2,
+                     int* d_d2, int* h_nfallback);
+/* Versioned file: "ORBLIDX1", version, shape, weights, centroids.  After a
+   load, orb_lindex_build(lx, db, n, 0) assigns a database to the loaded
+   centroids. */
+int orb_lindex_save(orb_lindex* lx, const char* path);
+int orb_lindex_load(orb_lindex* lx, const char* path);
+
 /* Stage timing for measurement (no counterpart in the reference, whose
    GpuTimer wraps whole API calls, cuda_utils.h:81-108).  While enabled, each
    launch of a stage in stage_mask records a pair of HIP events on the

@@ -1,0 +1,105 @@
+"""CPU oracle of the learned bucket matcher -- TEST INFRASTRUCTURE.
+
+Only tests/ and bench.py's cpu_baseline leg use this module, as the checker.
+It restates, in float64 numpy, the network and training step the HIP kernels
+compute (cuda-orb_amd/csrc/orb_mlp.hip), with the same bf16 rounding points,
+so GPU gradients can be compared to it within a stated tolerance, and the
+exact bucketed search used to check the probe results.
+
+Network (no counterpart in the reference; it is the learned part the
+workload adds): X = +/-1 expansion of the 256 descriptor bits,
+Z1 = X W1 + b1, A1 = bf16(gelu(Z1)), Z2 = A1 W2 + b2, A2 = bf16(gelu(Z2)),
+Z3 = A2 W3 + b3 (bucket logits), loss = mean cross-entropy.
+Backward: dZ3 = bf16((softmax(Z3) - onehot) / n), dW3 = A2^T dZ3,
+db3 = colsum(dZ3), dZ2 = bf16((dZ3 W3^T) * gelu'(Z2)), ... down to dW1.
+gelu(x) = x * Phi(x) with Phi the standard normal CDF (erf form).
+"""
+import math
+
+import numpy as np
+from scipy.special import erf
+
+SQRT1_2 = 1.0 / math.sqrt(2.0)
+INV_SQRT_2PI = 1.0 / math.sqrt(2.0 * math.pi)
+
+
+def bf16(x):
+    """Round float values to bfloat16 (round to nearest even), returned as float64."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def bits_pm1(desc):
+    """(n, 32) uint8 descriptors -> (n, 256) float64 of +/-1 (bit k = byte k//8, bit k%8)."""
+    bits = np.unpackbits(np.asarray(desc, np.uint8), axis=1, bitorder="little")
+    return bits.astype(np.float64) * 2.0 - 1.0
+
+
+def gelu(z):
+    return z * 0.5 * (1.0 + erf(z * SQRT1_2))
+
+
+def gelu_grad(z):
+    return 0.5 * (1.0 + erf(z * SQRT1_2)) + z * INV_SQRT_2PI * np.exp(-0.5 * z * z)
+
+
+def forward(params, desc):
+    """params: dict of float64 arrays W1 (256,H), b1, W2 (H,H), b2, W3 (H,C), b3 (bf16-valued weights)."""
+    X = bits_pm1(desc)
+    Z1 = X @ params["W1"] + params["b1"]
+    A1 = bf16(gelu(Z1))
+    Z2 = A1 @ params["W2"] + params["b2"]
+    A2 = bf16(gelu(Z2))
+    Z3 = A2 @ params["W3"] + params["b3"]
+    return {"X": X, "Z1": Z1, "A1": A1, "Z2": Z2, "A2": A2, "Z3": Z3}
+
+
+def loss_and_grads(params, desc, labels):
+    f = forward(params, desc)
+    n = len(labels)
+    Z3 = f["Z3"]
+    m = Z3.max(1, keepdims=True)
+    e = np.exp(Z3 - m)
+    P = e / e.sum(1, keepdims=True)
+    loss = float(-np.log(P[np.arange(n), labels]).mean())
+    G = P.copy()
+    G[np.arange(n), labels] -= 1.0
+    dZ3 = bf16(G / n)
+    g = {}
+    g["W3"] = f["A2"].T @ dZ3
+    g["b3"] = dZ3.sum(0)
+    dZ2 = bf16((dZ3 @ params["W3"].T) * gelu_grad(f["Z2"]))
+    g["W2"] = f["A1"].T @ dZ2
+    g["b2"] = dZ2.sum(0)
+    dZ1 = bf16((dZ2 @ params["W2"].T) * gelu_grad(f["Z1"]))
+    g["W1"] = f["X"].T @ dZ1
+    g["b1"] = dZ1.sum(0)
+    return loss, g, f
+
+
+def probe(logits, nprobe=8):
+    """Top-nprobe buckets by logit (ties to the lower bucket id) and the max softmax probability."""
+    order = np.lexsort((np.broadcast_to(np.arange(logits.shape[1]), logits.shape), -logits), axis=1)
+    m = logits.max(1, keepdims=True)
+    e = np.exp(logits - m)
+    pmax = 1.0 / e.sum(1)
+    return order[:, :nprobe], pmax
+
+
+def bucket_search(q, db, bucket_of, buckets):
+    """Exact top-1 of each query restricted to the database entries whose bucket is in its probe
+    list; ties to the lowest database index.  Returns (idx, dist), -1/257 when the lists are empty."""
+    qb = np.unpackbits(q, axis=1)
+    dbb = np.unpackbits(db, axis=1)
+    idx = np.full(len(q), -1, np.int64)
+    dist = np.full(len(q), 257, np.int64)
+    for i in range(len(q)):
+        cand = np.nonzero(np.isin(bucket_of, buckets[i]))[0]
+        if len(cand) == 0:
+            continue
+        d = (dbb[cand] != qb[i]).sum(1)
+        j = np.lexsort((cand, d))[0]
+        idx[i], dist[i] = cand[j], d[j]
+    return idx, dist

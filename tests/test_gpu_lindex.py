@@ -1,0 +1,152 @@
+"""Learned bucket matcher on the GPU against the CPU oracle (oracle/mlp.py):
+gradients of the hand-written BF16 forward/backward kernels, the bucketed
+exact search, the exact fallback, training, and the weight file."""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs an MI355X")]
+
+# Gradients: the GPU accumulates in FP32 where the oracle uses FP64, and a
+# BF16 rounding of an intermediate can then land one BF16 step apart; the
+# stated tolerance is a relative L2 error of 2e-2 per parameter tensor.
+GRAD_RTOL = 2e-2
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    return t
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import orb_amd
+    c = orb_amd.Context(0)
+    yield c
+    c.close()
+
+
+def flip_copies(rng, db, n, p=0.05):
+    src = rng.integers(0, len(db), n)
+    flips = np.packbits(rng.random((n, 256)) < p, axis=1, bitorder="little")
+    return db[src] ^ flips, src
+
+
+@pytest.mark.parametrize("hidden,nb", [(512, 1024), (128, 256)])
+def test_gradients_match_oracle(torch, ctx, hidden, nb):
+    import orb_amd
+    from oracle import mlp
+    print(f"grad check hidden={hidden} buckets={nb}", flush=True)
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=nb, hidden=hidden, seed=7)
+    rng = np.random.default_rng(1)
+    n = 256
+    q = rng.integers(0, 256, (n, 32), dtype=np.uint8)
+    labels = rng.integers(0, nb, n).astype(np.int32)
+    # non-zero biases so their gradients and the bias epilogue are exercised
+    w = lx.get_weights()
+    P = lx.split(w)
+    for b in ("b1", "b2", "b3"):
+        P[b][:] = rng.normal(0, 0.1, P[b].shape).astype(np.float32)
+    lx.set_weights(w)
+    g, loss = lx.grads(torch.from_numpy(q).cuda(), torch.from_numpy(labels).cuda())
+    params = {k: (mlp.bf16(v) if k.startswith("W") else v.astype(np.float64)) for k, v in lx.split(w).items()}
+    ref_loss, ref_g, f = mlp.loss_and_grads(params, q, labels)
+    assert loss == pytest.approx(ref_loss, rel=2e-3)
+    G = lx.split(g)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        err = np.linalg.norm(G[k] - ref_g[k]) / np.linalg.norm(ref_g[k])
+        print(f"  {k}: relative L2 error {err:.2e}", flush=True)
+        assert err < GRAD_RTOL, k
+    logits = lx.logits(torch.from_numpy(q).cuda()).cpu().numpy()
+    assert np.linalg.norm(logits - f["Z3"]) / np.linalg.norm(f["Z3"]) < 1e-2
+    lx.close()
+
+
+@pytest.fixture(scope="module")
+def small_index(torch, ctx):
+    import orb_amd
+    rng = np.random.default_rng(2)
+    db = rng.integers(0, 256, (20000, 32), dtype=np.uint8)
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=256, hidden=128, nprobe=4, fallback=0.3, seed=3)
+    ddb = torch.from_numpy(db).cuda()
+    lx.build(ddb, iters=4)
+    yield lx, db, ddb
+    lx.close()
+
+
+def test_kmodes_partition_is_consistent(torch, ctx, small_index, oracle):
+    lx, db, ddb = small_index
+    off, cen, bof = lx.bucket_info()
+    bof = bof.cpu().numpy()
+    assert off[0] == 0 and off[-1] == len(db) and (np.diff(off) >= 0).all()
+    assert np.array_equal(np.bincount(bof, minlength=256), np.diff(off))
+    # every entry sits in the bucket of its nearest centroid (ties to the lower id)
+    i1, d1, _, _ = oracle.match_top2(db, cen, nthreads=8)
+    assert np.array_equal(bof, i1)
+
+
+def test_centroid_probe_search_equals_brute_force(torch, ctx, small_index):
+    import orb_amd
+    from oracle import mlp
+    lx, db, ddb = small_index
+    rng = np.random.default_rng(5)
+    q, _ = flip_copies(rng, db, 512)
+    (i1, d1, i2, d2), nfb = lx.query(torch.from_numpy(q).cuda(), mode=orb_amd.ORB_LINDEX_CENTROIDS)
+    assert nfb == 0
+    _, cen, bof = lx.bucket_info()
+    bof = bof.cpu().numpy()
+    qd = np.unpackbits(q, axis=1)
+    cd = np.unpackbits(cen, axis=1)
+    D = (qd[:, None, :] != cd[None, :, :]).sum(-1)
+    probes = np.argsort(D, axis=1, kind="stable")[:, :4]
+    want_i, want_d = mlp.bucket_search(q, db, bof, probes)
+    assert np.array_equal(i1.cpu().numpy(), want_i) and np.array_equal(d1.cpu().numpy(), want_d)
+
+
+def test_network_probe_search_and_fallback(torch, ctx, small_index, oracle):
+    import orb_amd
+    from oracle import mlp
+    lx, db, ddb = small_index
+    rng = np.random.default_rng(6)
+    q, _ = flip_copies(rng, db, 512)
+    dq = torch.from_numpy(q).cuda()
+    logits = lx.logits(dq).cpu().numpy().astype(np.float64)
+    probes, pmax = mlp.probe(logits, 4)
+    _, _, bof = lx.bucket_info()
+    bof = bof.cpu().numpy()
+    (i1, d1, i2, d2), nfb = lx.query(dq)
+    fb = pmax < 0.3
+    assert nfb == int(fb.sum())
+    want_i, want_d = mlp.bucket_search(q, db, bof, probes)
+    ex = oracle.match_top2(q, db, nthreads=8)
+    want_i = np.where(fb, ex[0], want_i)
+    want_d = np.where(fb, ex[1], want_d)
+    assert np.array_equal(i1.cpu().numpy(), want_i) and np.array_equal(d1.cpu().numpy(), want_d)
+
+
+def test_training_lowers_loss_and_save_load_round_trip(torch, ctx, small_index, tmp_path):
+    import orb_amd
+    lx, db, ddb = small_index
+    rng = np.random.default_rng(7)
+    q, _ = flip_copies(rng, db, 16384)
+    dq = torch.from_numpy(q).cuda()
+    tgt = ctx.match_top2(dq, ddb, method=orb_amd.ORB_MATCH_MFMA)[0]
+    losses = lx.train(dq, tgt, epochs=4, batch=1024, lr=2e-3)
+    print("epoch losses", losses, flush=True)
+    assert losses[-1] < 0.6 * losses[0]
+    qt, _ = flip_copies(rng, db, 1024)
+    dqt = torch.from_numpy(qt).cuda()
+    out1, n1 = lx.query(dqt)
+    path = tmp_path / "lindex.bin"
+    lx.save(path)
+    lx2 = orb_amd.LearnedIndex(ctx, nbuckets=256, hidden=128, nprobe=4, fallback=0.3, seed=99)
+    lx2.load(path)
+    lx2.build(ddb, iters=0)
+    out2, n2 = lx2.query(dqt)
+    assert n1 == n2
+    for a, b in zip(out1, out2):
+        assert torch.equal(a, b)
+    lx2.close()

@@ -39,10 +39,18 @@ def gpu_detect(torch, ctx, img_p, w, h, max_pts, compute_desc=True):
     return orb_amd.points_to_numpy(pts, n), desc[:n].cpu().numpy()
 
 
+def describe_mismatch(got, want, field):
+    bad = np.nonzero(got[field] != want[field])[0]
+    rows = [(int(i), int(got["x"][i]), int(got["y"][i]), int(got["octave"][i]), int(want["octave"][i]),
+             hex(int(got["score"].view(np.uint32)[i])), hex(int(want["score"].view(np.uint32)[i])))
+            for i in bad[:8]]
+    return f"{field}: {len(bad)} of {len(got)} differ; (i, x, y, oct, oct_want, score, score_want): {rows}"
+
+
 def assert_same_points(got, want):
     assert len(got) == len(want)
     for f in ("x", "y", "octave"):
-        assert np.array_equal(got[f], want[f]), f
+        assert np.array_equal(got[f], want[f]), describe_mismatch(got, want, f)
     assert np.array_equal(got["score"].view(np.uint32), want["score"].view(np.uint32)), "score bits"
     assert np.array_equal(got["angle"].view(np.uint32), want["angle"].view(np.uint32)), "angle bits"
 
@@ -160,9 +168,11 @@ def test_match_reference_rule_and_device_counts(torch, ctx, oracle):
     assert np.array_equal(got["match"], want_m) and np.array_equal(got["distance"], want_dist)
 
 
-@pytest.mark.parametrize("nq,ndb", [(1, 1), (7, 300), (1000, 4097), (300, 70000), (5000, 0)])
-def test_match_top2_popcount_matches_oracle(torch, ctx, oracle, nq, ndb):
-    print(f"top2 nq={nq} ndb={ndb}", flush=True)
+@pytest.mark.parametrize("method", [0, 1], ids=["popcount", "mfma"])
+@pytest.mark.parametrize("nq,ndb", [(1, 1), (7, 300), (1000, 4097), (300, 70000), (5000, 0), (4100, 9000),
+                                    (257, 63), (65, 129)])
+def test_match_top2_matches_oracle(torch, ctx, oracle, nq, ndb, method):
+    print(f"top2 method={method} nq={nq} ndb={ndb}", flush=True)
     rng = np.random.default_rng(nq * 7 + ndb)
     db = rng.integers(0, 256, (ndb, 32), dtype=np.uint8)
     if ndb > 10:
@@ -173,11 +183,35 @@ def test_match_top2_popcount_matches_oracle(torch, ctx, oracle, nq, ndb):
         q[0] = db[3]
         q[1] = db[ndb - 1]
         q[2] = db[5] ^ 1
-    out = ctx.match_top2(torch.from_numpy(q).cuda(), torch.from_numpy(db).cuda())
+    out = ctx.match_top2(torch.from_numpy(q).cuda(), torch.from_numpy(db).cuda(), method=method)
     torch.cuda.synchronize()
     got = [o.cpu().numpy() for o in out]
     want = oracle.match_top2(q, db, nthreads=8)
     for g, w_ in zip(got, want):
+        assert np.array_equal(g, w_)
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["popcount", "mfma"])
+def test_match_top2_near_duplicates_full_size_property(torch, ctx, oracle, method):
+    """Queries that are copies of database rows with Binomial(256, 0.05) bit flips (the dense-matching
+    workload at 1/4 of its database): the matcher's distance to the source row is the number of flips,
+    so d1 <= flips and idx1 == source unless another row ties or beats it; cross-checked on a subset."""
+    rng = np.random.default_rng(42)
+    ndb, nq = 65536, 16384
+    db = rng.integers(0, 256, (ndb, 32), dtype=np.uint8)
+    src = rng.integers(0, ndb, nq)
+    flips = rng.random((nq, 256)) < 0.05
+    q = db[src] ^ np.packbits(flips, axis=1, bitorder="little")
+    out = ctx.match_top2(torch.from_numpy(q).cuda(), torch.from_numpy(db).cuda(), method=method)
+    torch.cuda.synchronize()
+    i1, d1, i2, d2 = (o.cpu().numpy() for o in out)
+    nflip = flips.sum(1)
+    assert (d1 <= nflip).all() and (d1 <= d2).all()
+    assert ((i1 == src) | (d1 < nflip) | ((d1 == nflip) & (i1 < src))).all()
+    assert (i1 == src).mean() > 0.999
+    sub = slice(0, 512)
+    want = oracle.match_top2(q[sub], db, nthreads=8)
+    for g, w_ in zip((i1[sub], d1[sub], i2[sub], d2[sub]), want):
         assert np.array_equal(g, w_)
 
 
