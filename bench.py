@@ -144,6 +144,146 @@ def batch_frames(ctx, torch, dev, reps=5):
             "megapixels": round(sum(w * h for w, h in zip(ws, hs)) / 1e6, 2)}
 
 
+def synth_frames(seeds, w, h, nshapes, warp_seed=None, threads=16):
+    """Seeded synthetic frames (host threads; the generator releases the GIL)."""
+    import orb_amd
+    from concurrent.futures import ThreadPoolExecutor
+
+    def make(s):
+        a = orb_amd.synth_image(s, w, h, nshapes)
+        if warp_seed is None:
+            return a
+        return orb_amd.synth_warp(warp_seed * 1000003 + s, a, w, h)[0]
+
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(make, seeds))
+
+
+def extract(ctx, torch, dev, seeds, w, h, nshapes, cap, warp_seed=None, limit=None, batch=64):
+    """Detect + describe seeded frames in batches of 64; descriptors of all frames, in order."""
+    out, total = [], 0
+    for i in range(0, len(seeds), batch):
+        imgs = synth_frames(seeds[i:i + batch], w, h, nshapes, warp_seed)
+        n = len(imgs)
+        frames = [torch.from_numpy(a).to(dev) for a in imgs]
+        pts = ctx.alloc_points(cap, frames=n)
+        desc = ctx.alloc_desc(cap, frames=n)
+        counts = torch.zeros(n, dtype=torch.int32, device=dev)
+        ctx.detect_and_compute_batch(frames, [w] * n, [h] * n, [pts[j] for j in range(n)],
+                                     [desc[j] for j in range(n)], counts)
+        cnt = counts.cpu().numpy()
+        for j in range(n):
+            out.append(desc[j, :cnt[j]].clone())
+            total += int(cnt[j])
+        if limit is not None and total >= limit:
+            break
+    d = torch.cat(out)
+    return d[:limit] if limit is not None else d
+
+
+def learned_match(ctx, torch, dev, log=None):
+    """Learned-bucket matching: a 1,048,576-descriptor database extracted by the pipeline from 2000
+    seeded 1280x720 frames, 65536 queries from warped copies; MLP 256->512->512->1024 buckets (BF16,
+    FP32 accumulate, GELU) trained on exact matches of other warped copies; top-8 bucket probe, exact
+    fallback when max softmax < 0.3; recall@1 against exact brute force."""
+    import orb_amd
+    w, h, nshapes, per = 1280, 720, 120, 525
+    ndb, nq = 1 << 20, 65536
+    t_all = time.perf_counter()
+    ctx.init(noctaves=LEVELS, edge_threshold=31, wta_k=2, score_type=0, patch_size=31, fast_threshold=20,
+             max_pts=per, scale_factor=SCALE)
+    seeds = list(range(10000, 12000))
+    note = lambda msg: print(f"[learned] {msg} ({time.perf_counter() - t_all:.1f} s)", file=sys.stderr, flush=True)
+    db = extract(ctx, torch, dev, seeds, w, h, nshapes, per, limit=ndb)
+    note(f"database {db.shape[0]} descriptors")
+    more = 12000
+    while db.shape[0] < ndb:   # frames with fewer than `per` keypoints: extend the seed list
+        extra = extract(ctx, torch, dev, list(range(more, more + 64)), w, h, nshapes, per)
+        db = torch.cat([db, extra])[:ndb]
+        more += 64
+    t_db = time.perf_counter()
+    train_q = extract(ctx, torch, dev, seeds, w, h, nshapes, per, warp_seed=1)
+    train_q = train_q[: (train_q.shape[0] // 4096) * 4096]
+    note(f"training queries {train_q.shape[0]}")
+    test_q = extract(ctx, torch, dev, seeds, w, h, nshapes, per, warp_seed=2, limit=nq)
+    t_q = time.perf_counter()
+    note(f"test queries {test_q.shape[0]}")
+    ctx.init(noctaves=LEVELS, edge_threshold=31, wta_k=2, score_type=0, patch_size=31, fast_threshold=20,
+             max_pts=CAP, scale_factor=SCALE)
+    # exact labels of the training queries
+    labels = ctx.match_top2(train_q, db, method=orb_amd.ORB_MATCH_MFMA)[0]
+    torch.cuda.synchronize()
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=1024, hidden=512, nprobe=8, fallback=0.3, seed=4)
+    t0 = time.perf_counter()
+    lx.build(db, iters=8)
+    t_build = time.perf_counter() - t0
+    off, _, _ = lx.bucket_info()
+    sizes = np.diff(off)
+    note("buckets built")
+    t0 = time.perf_counter()
+    losses = lx.train(train_q, labels, epochs=3, batch=4096, lr=1e-3)
+    note(f"trained, losses {losses}")
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return r, best * 1e3
+
+    exact, ms_exact = timed(lambda: ctx.match_top2(test_q, db, method=orb_amd.ORB_MATCH_MFMA))
+    (lq, nfb), ms_learned = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
+    (cq, _), ms_cent = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS))
+    d_ex = exact[1].cpu().numpy()
+    rec = lambda o: float((o[1].cpu().numpy() == d_ex).mean())
+    rec_idx = lambda o: float((o[0].cpu().numpy() == exact[0].cpu().numpy()).mean())
+    # the network alone (no fallback), and a sweep of probe settings
+    lx.set_probe(8, 0.0)
+    (nq0, _), ms_net = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
+    sweep = []
+    for npb in (8, 16, 32):
+        for fb in (0.3, 0.5, 0.7):
+            lx.set_probe(npb, fb)
+            (o, nf), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
+            sweep.append({"nprobe": npb, "fallback": fb, "ms": round(ms, 3), "recall_at_1": round(rec(o), 5),
+                          "fallback_queries": int(nf), "speedup": round(ms_exact / ms, 3)})
+    ok = [r for r in sweep if r["recall_at_1"] >= 0.99]
+    best = min(ok, key=lambda r: r["ms"]) if ok else None
+    # the same bucket search probed by nearest centroids (no network, no fallback)
+    cent = []
+    for npb in (8, 16, 32, 64, 128):
+        lx.set_probe(npb, 0.3)
+        (o, _), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS), reps=2)
+        cent.append({"nprobe": npb, "ms": round(ms, 3), "recall_at_1": round(rec(o), 5),
+                     "speedup": round(ms_exact / ms, 3)})
+    ok_c = [r for r in cent if r["recall_at_1"] >= 0.99]
+    best_c = min(ok_c, key=lambda r: r["ms"]) if ok_c else None
+    res = {
+        "database": int(db.shape[0]), "queries": int(test_q.shape[0]), "train_queries": int(train_q.shape[0]),
+        "buckets": 1024, "nprobe": 8, "fallback_threshold": 0.3,
+        "bucket_size": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
+        "exact_mfma_ms": round(ms_exact, 3),
+        "learned_ms": round(ms_learned, 3), "learned_recall_at_1": round(rec(lq), 5),
+        "learned_recall_idx": round(rec_idx(lq), 5), "fallback_queries": int(nfb),
+        "speedup_vs_exact": round(ms_exact / ms_learned, 3),
+        "centroid_probe_ms": round(ms_cent, 3), "centroid_probe_recall_at_1": round(rec(cq), 5),
+        "network_only_ms": round(ms_net, 3), "network_only_recall_at_1": round(rec(nq0), 5),
+        "sweep": sweep, "fastest_at_99_recall": best,
+        "centroid_sweep": cent, "centroid_fastest_at_99_recall": best_c,
+        "train_loss_per_epoch": [round(float(x), 4) for x in losses],
+        "build_s": round(t_build, 2), "train_s": round(t_train, 2),
+        "data_s": round(t_q - t_all, 1),
+    }
+    lx.close()
+    return res
+
+
 def cpu_baseline(a, b, min_seconds=10.0):
     """The scalar C oracle on the host: the same step (detect+describe both images, match them),
     repeated until about min_seconds of CPU work."""
@@ -170,7 +310,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the dense-matching and batched-frames measurements")
+                    help="skip the dense-matching, batched-frames and learned-matching measurements")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-matching measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 64 MiB elementwise copy kernel first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -265,6 +406,8 @@ def main():
     if not args.no_extras:
         out["dense_match"] = dense_match(ctx, torch, dev)
         out["batch64"] = batch_frames(ctx, torch, dev)
+        if not args.no_learned:
+            out["learned"] = learned_match(ctx, torch, dev)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a, b)
     ctx.close()

@@ -313,6 +313,7 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
         pyr_off[f] = pyr_bytes;
         pyr_bytes += round256((size_t)g.total + 2 * (size_t)guard[f]);
         key_guard_rows[f] = rmax + 1;
+        F.rmax = rmax;
         key_off[f] = key_bytes;
         key_bytes += round256(sizeof(unsigned long long) * (size_t)(g.h[0] + 2 * key_guard_rows[f]) * g.p[0]);
         const int rows = g.h[0] - 2 * border, span = g.w[0] - 2 * border;

@@ -65,6 +65,7 @@ struct FrameDev {
     unsigned long long* keys;  /* row 0 of the key map */
     /* NMS blocks: rows [border, h0-border), segments of kNmsSeg pixels from nms_x0 */
     int nms_first, nms_rows, nms_segs, nms_x0;
+    int rmax;                  /* largest NMS radius of the frame's levels */
 };
 
 /* A contiguous range of workgroups covering one (frame, level) region. */

@@ -519,23 +519,84 @@ __device__ __forceinline__ int block_scan(int v, int* s_tmp, int& total)
     return res;
 }
 
-/* orbd.cu:945-966: a key survives unless a positive neighbour in its
+/* NMS of orbd.cu:945-966: a key survives unless a positive neighbour in its
    (2r+1)^2 square is greater, or equal and at dx <= 0 && dy <= 0 (flat
-   indexing with pitch p0); r is the radius of the key's level */
-__device__ __noinline__ bool nms_keep(const unsigned long long* __restrict__ kp, long long p0, unsigned long long key,
-                                      uint32_t epoch, int r)
+   indexing with pitch p0); r is the radius of the key's own level.
+   A thread owns 4 adjacent level-0 pixels of one row. */
+
+/* radius <= 4 (every bilinear pyramid up to 8 levels at scale <= 1.25): per
+   neighbour row, the 12 keys x-4 .. x+7 around the 4 pixels come in three
+   aligned 32-byte loads and all comparisons run from registers */
+__device__ __forceinline__ uint32_t nms_rows4(const unsigned long long* __restrict__ rowp, long long p0, int xs,
+                                              const unsigned long long* k4, uint32_t valid, uint32_t epoch,
+                                              const int* rk, int R)
 {
-    const uint32_t sbits = (uint32_t)(key >> 3) & 0x7fffffffu;
-    for (int dy = -r; dy <= r; dy++) {
-        const unsigned long long* rp = kp + (long long)dy * p0;
-        for (int dx = -r; dx <= r; dx++) {
-            const unsigned long long nk = rp[dx];
-            if ((uint32_t)(nk >> kEpochShift) != epoch || (dx == 0 && dy == 0)) continue;
-            const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
-            if (ns > sbits || (ns == sbits && dx <= 0 && dy <= 0)) return false;
+    uint32_t supp = 0;
+    uint32_t sk[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) sk[k] = (uint32_t)(k4[k] >> 3) & 0x7fffffffu;
+    for (int dy = -R; dy <= R; dy++) {
+        const ulonglong4* rp = (const ulonglong4*)(rowp + (long long)dy * p0 + xs - 4);
+        unsigned long long nb[12];
+#pragma unroll
+        for (int t = 0; t < 3; t++) {
+            const ulonglong4 v = rp[t];
+            nb[4 * t] = v.x;
+            nb[4 * t + 1] = v.y;
+            nb[4 * t + 2] = v.z;
+            nb[4 * t + 3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int r = rk[k];
+            if (!((valid >> k) & 1u) || abs(dy) > r) continue;
+#pragma unroll
+            for (int dx = -4; dx <= 4; dx++) {
+                if (abs(dx) > r || (dx == 0 && dy == 0)) continue;
+                const unsigned long long nk = nb[4 + k + dx];
+                const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
+                const bool live = (uint32_t)(nk >> kEpochShift) == epoch;
+                if (live && (ns > sk[k] || (ns == sk[k] && dx <= 0 && dy <= 0))) supp |= 1u << k;
+            }
         }
     }
-    return true;
+    return valid & ~supp;
+}
+
+/* any radius: the wavefront takes its candidates one at a time and spreads
+   each neighbourhood over its 64 lanes */
+__device__ __forceinline__ uint32_t nms_wave(const unsigned long long* __restrict__ rowp, long long p0, int xs,
+                                             const unsigned long long* k4, uint32_t valid, uint32_t epoch,
+                                             const int* __restrict__ radius)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t surv = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        unsigned long long m = __ballot((valid >> k) & 1u);
+        while (m) {
+            const int src = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const uint32_t klo = (uint32_t)__shfl((int)(uint32_t)k4[k], src, 64);
+            const uint32_t khi = (uint32_t)__shfl((int)(uint32_t)(k4[k] >> 32), src, 64);
+            const unsigned long long key = ((unsigned long long)khi << 32) | klo;
+            const int x = xs + 4 * (src - lane) + k;
+            const uint32_t sbits = (uint32_t)(key >> 3) & 0x7fffffffu;
+            const int r = radius[7u - (uint32_t)(key & 7u)];
+            const int side = 2 * r + 1;
+            bool sup = false;
+            for (int i = lane; i < side * side; i += 64) {
+                const int dy = i / side - r, dx = i - (i / side) * side - r;
+                if (dx == 0 && dy == 0) continue;
+                const unsigned long long nk = rowp[(long long)dy * p0 + x + dx];
+                const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
+                if ((uint32_t)(nk >> kEpochShift) == epoch && (ns > sbits || (ns == sbits && dx <= 0 && dy <= 0)))
+                    sup = true;
+            }
+            if (!__any(sup) && lane == src) surv |= 1u << k;
+        }
+    }
+    return surv;
 }
 
 __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames, const int* __restrict__ block_frame,
@@ -555,20 +616,25 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
     unsigned long long k4[4] = {0ull, 0ull, 0ull, 0ull};
     if (xs < xhi) {
         /* xs is a multiple of 4 below w0 <= p0 (a multiple of 128): 32-byte aligned, inside the row */
-        const ulonglong2 a = *(const ulonglong2*)(rowp + xs);
-        const ulonglong2 c = *(const ulonglong2*)(rowp + xs + 2);
+        const ulonglong4 a = *(const ulonglong4*)(rowp + xs);
         k4[0] = a.x;
         k4[1] = a.y;
-        k4[2] = c.x;
-        k4[3] = c.y;
+        k4[2] = a.z;
+        k4[3] = a.w;
     }
-    uint32_t surv = 0;
+    uint32_t valid = 0;
+    int rk[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int x = xs + k;
-        const unsigned long long key = k4[k];
-        const bool cand_px = x >= xlo && x < xhi && (uint32_t)(key >> kEpochShift) == epoch;
-        if (cand_px && nms_keep(rowp + x, F.p0, key, epoch, F.radius[7u - (uint32_t)(key & 7u)])) surv |= 1u << k;
+        const bool c = x >= xlo && x < xhi && (uint32_t)(k4[k] >> kEpochShift) == epoch;
+        valid |= (uint32_t)c << k;
+        rk[k] = c ? F.radius[7u - (uint32_t)(k4[k] & 7u)] : 0;
+    }
+    uint32_t surv = 0;
+    if (__any(valid != 0)) {
+        if (F.rmax <= 4) surv = nms_rows4(rowp, F.p0, xs, k4, valid, epoch, rk, F.rmax);
+        else surv = nms_wave(rowp, F.p0, xs, k4, valid, epoch, F.radius);
     }
     int total;
     int pos = block_scan<256>(__popc(surv), s_tmp, total);

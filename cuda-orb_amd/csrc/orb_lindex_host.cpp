@@ -515,6 +515,15 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     return ORB_OK;
 }
 
+int orb_lindex_set_probe(orb_lindex* L, int nprobe, float fallback)
+{
+    if (!L) return lfail(L, ORB_E_ARG, "null argument");
+    if (nprobe < 1 || nprobe > L->C) return lfail(L, ORB_E_ARG, "nprobe out of range");
+    L->prm.nprobe = nprobe;
+    L->prm.fallback = fallback;
+    return ORB_OK;
+}
+
 /* File: "ORBLIDX1", int32 version=1, nbuckets, hidden, nprobe, float fallback,
    then all fp32 parameters (W1 b1 W2 b2 W3 b3, row-major), then the
    nbuckets x 32 centroid bytes. */

@@ -27,6 +27,7 @@ namespace orbk
 {
 
 constexpr int kGranule = 64;          /* database entries per chunk granule */
+constexpr int kMinChunkGranules = 2;  /* at least 128 entries per chunk: bounds the merge work */
 constexpr int kPopQ = 64;             /* popcount: queries per wavefront */
 constexpr int kMfmaQ = 256;           /* MFMA: queries per workgroup */
 constexpr int kMfmaRow = 256 + 16;    /* LDS row of one unpacked entry (padded) */
@@ -43,7 +44,8 @@ __host__ __device__ inline Plan make_plan(int nq, int ndb, int qtile, int slots)
     p.qunits = (nq + qtile - 1) / qtile;
     const int granules = (ndb + kGranule - 1) / kGranule;
     int c = p.qunits > 0 ? slots / p.qunits : 1;
-    if (c > granules) c = granules;
+    const int cmax = (granules + kMinChunkGranules - 1) / kMinChunkGranules;
+    if (c > cmax) c = cmax;
     if (c < 1) c = 1;
     p.chunk = granules > 0 ? ((granules + c - 1) / c) * kGranule : kGranule;
     p.chunks = ndb > 0 ? (ndb + p.chunk - 1) / p.chunk : 0;
@@ -207,7 +209,18 @@ __global__ __launch_bounds__(256) void k_match_merge(int nq_cap, const int* __re
     const Plan P = make_plan(nq, ndb, qtile, slots);
     for (int qi = blockIdx.x * blockDim.x + threadIdx.x; qi < nq; qi += gridDim.x * blockDim.x) {
         uint32_t k1 = kNoKey, k2 = kNoKey;
-        for (int c = 0; c < P.chunks; c++) {
+        int c = 0;
+        for (; c + 4 <= P.chunks; c += 4) {
+            uint2 v[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) v[t] = partial[(long long)(c + t) * nq + qi];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                top2_push(k1, k2, v[t].x);
+                top2_push(k1, k2, v[t].y);
+            }
+        }
+        for (; c < P.chunks; c++) {
             const uint2 v = partial[(long long)c * nq + qi];
             top2_push(k1, k2, v.x);
             top2_push(k1, k2, v.y);

@@ -34,7 +34,7 @@ EXPORTS = [
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
     "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
     "orb_lindex_logits", "orb_lindex_grads", "orb_lindex_num_params", "orb_lindex_get_weights",
-    "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_save", "orb_lindex_load",
+    "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_save", "orb_lindex_load", "orb_lindex_set_probe",
 ]
 
 ORB_LINDEX_MLP = 0
@@ -106,6 +106,7 @@ def lib():
         L.orb_lindex_query.argtypes = [P, V, I, I, V, V, V, V, V]
         L.orb_lindex_save.argtypes = [P, ctypes.c_char_p]
         L.orb_lindex_load.argtypes = [P, ctypes.c_char_p]
+        L.orb_lindex_set_probe.argtypes = [P, I, ctypes.c_float]
         _lib = L
     return _lib
 
@@ -365,6 +366,9 @@ class LearnedIndex:
         self._c(lib().orb_lindex_query(self.h, _dptr(q), q.shape[0], mode, *[_dptr(o) for o in out],
                                        ctypes.byref(nfb)), "orb_lindex_query")
         return out, nfb.value
+
+    def set_probe(self, nprobe, fallback):
+        self._c(lib().orb_lindex_set_probe(self.h, nprobe, fallback), "orb_lindex_set_probe")
 
     def save(self, path):
         self._c(lib().orb_lindex_save(self.h, str(path).encode()), "orb_lindex_save")

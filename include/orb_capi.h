@@ -181,6 +181,8 @@ int orb_lindex_grads(orb_lindex* lx, const uint8_t* d_q, const int* d_labels, in
 int orb_lindex_num_params(const orb_lindex* lx);
 int orb_lindex_get_weights(orb_lindex* lx, float* h_w);
 int orb_lindex_set_weights(orb_lindex* lx, const float* h_w);
+/* Change the number of probed buckets (1..nbuckets) and the fallback threshold. */
+int orb_lindex_set_probe(orb_lindex* lx, int nprobe, float fallback);
 /* Nearest and second-nearest database entries of nq queries (multiple of 64);
    *h_nfallback (may be NULL) receives how many went to the exact search. */
 int orb_lindex_query(orb_lindex* lx, const uint8_t* d_q, int nq, int mode, int* d_idx1, int* d_d1, int* d_idx2,
