@@ -249,6 +249,61 @@ def test_fast_segment_test_is_textbook_fast9(oracle):
     assert hits > 500
 
 
+def test_fast_score_is_the_largest_threshold_keeping_a_9_arc(oracle):
+    """FAST_SCORE (OpenCV cornerScore<16>, orbd.cu:538-648) in closed form: with d_k = centre -
+    ring_k, the score is max(t, max over 9-arcs of min d, max over 9-arcs of -max d) - 1, i.e. one
+    less than the smallest margin by which some contiguous 9-arc is all brighter or all darker."""
+    rng = np.random.default_rng(14)
+    t = 20
+    for trial in range(6):
+        img = rng.integers(0, 256, (40, 128)).astype(np.uint8)
+        if trial % 2:   # smooth regions with a few strong corners, so large scores occur
+            img = (rng.integers(0, 2, (40, 128)) * 200 + rng.integers(0, 30, (40, 128))).astype(np.uint8)
+        flat = img.ravel()
+        for y in range(3, 37):
+            for x in range(3, 125):
+                c = int(img[y, x])
+                d = np.array([c - int(img[y + dy, x + dx]) for dx, dy in CIRCLE])
+                arcs = np.stack([np.roll(d, -s)[:9] for s in range(16)])
+                want = max(t, int(arcs.min(1).max()), int((-arcs.max(1)).max())) - 1
+                got = oracle.lib().orc_fast_score(oracle.ptr(flat), flat.size, y * 128 + x, 128, t)
+                assert got == want, (trial, x, y)
+
+
+def _wta_codes(B, cx, cy, pat, k):
+    """2-bit argmax codes at angle 0 straight from the pattern table (values distinct)."""
+    codes, distinct = [], []
+    for b in range(32):
+        idx = ([[12 * b + 3 * q + i for i in range(3)] for q in range(3)] + [[12 * b + 8, 12 * b + 9, 12 * b + 11]]
+               if k == 3 else [[16 * b + 4 * q + i for i in range(4)] for q in range(4)])
+        for tup in idx:
+            v = [int(B[cy + pat[j][1], cx + pat[j][0]]) for j in tup]
+            codes.append(int(np.argmax(v)))
+            distinct.append(len(set(v)) == len(v))
+    return np.array(codes), np.array(distinct)
+
+
+@pytest.mark.parametrize("k", [3, 4])
+def test_descriptor_wta_codes_are_argmax_of_pattern_tuples(oracle, k):
+    """WTA_K 3/4 (orbd.cu:1364-1434): each 2-bit code is the index of the brightest sample of its
+    tuple; for WTA_K 3 the fourth tuple of every byte is samples 8, 9, 11 (the reference's
+    indices).  Checked wherever the tuple's values are distinct, so no tie rule is assumed."""
+    rng = np.random.default_rng(20 + k)
+    img = rng.integers(0, 256, (128, 128)).astype(np.uint8)
+    g = oracle.geometry(128, 128, 1, 1.2)
+    _, blur = oracle.pyramid(g, oracle.pitched(img))
+    B = blur[: 128 * 128].reshape(128, 128)
+    pat = oracle.pattern(k)
+    for cx, cy in ((60, 70), (40, 41), (85, 90)):
+        pt = np.zeros(1, oracle.POINT_DTYPE)
+        pt["x"], pt["y"] = cx, cy
+        desc = oracle.describe(g, blur, pt, k)[0]
+        got = np.stack([(desc >> s) & 3 for s in (0, 2, 4, 6)], -1).ravel()
+        want, distinct = _wta_codes(B, cx, cy, pat, k)
+        assert distinct.mean() > 0.5
+        assert np.array_equal(got[distinct], want[distinct])
+
+
 def test_harris_against_float64_with_wrapped_ints(oracle):
     """Harris response vs an independent numpy evaluation (int64 Sobel and box sums wrapped to
     int32, float64 tail); the float32 result must agree to a few ulp, and the sample must include
