@@ -402,6 +402,9 @@ def main():
         "match_gpairs_per_s": round(n1 * n2 / (stage_ms["match"] + stage_ms["match_merge"]) / 1e6, 2)
         if stage_ms["match"] > 0 else None,
         "roofline": roof,
+        "autotune": {"pyramid_strip_height": orb_amd.PYRAMID_STRIP_HEIGHTS[ctx.tune_get(orb_amd.TUNE_PYRAMID)[0]],
+                     "pyramid_ms_per_variant": dict(zip(orb_amd.PYRAMID_STRIP_HEIGHTS,
+                                                        [round(t, 5) for t in ctx.tune_get(orb_amd.TUNE_PYRAMID)[1]]))},
     }
     if not args.no_extras:
         out["dense_match"] = dense_match(ctx, torch, dev)

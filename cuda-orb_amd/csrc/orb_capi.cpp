@@ -118,9 +118,12 @@ struct FrameSet {
     void* arena = nullptr;
     size_t arena_bytes = 0;
     FrameDev* d_frames = nullptr;
-    Job* d_pyr = nullptr;
+    Job* d_pyr[kPyrVariants] = {};
     Job* d_det = nullptr;
-    int pyr_njobs = 0, pyr_blocks = 0, det_njobs = 0, det_blocks = 0;
+    int pyr_njobs = 0, pyr_units[kPyrVariants] = {}, det_njobs = 0, det_blocks = 0;
+    /* autotuner: measured ms per pyramid variant and the choice (-1: not tuned) */
+    float pyr_ms[kPyrVariants] = {};
+    int pyr_choice = -1;
     int* d_nms_frame = nullptr;
     int nms_blocks = 0;
     Candidate* d_cand = nullptr;
@@ -137,6 +140,7 @@ struct orb_ctx {
     bool inited = false;
     orb_params prm{};
     int2* d_pattern = nullptr;
+    int pyr_forced = -1;       /* pyramid variant forced by orb_tune_set, or -1 */
     float gauss[4] = {};
     float harris_scale = 0.f;
     int umax[32] = {};
@@ -268,9 +272,9 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     std::vector<int> guard(nf), key_guard_rows(nf), tab_base(nf);
     size_t pyr_bytes = 0, key_bytes = 0;
     int tab_ints = 0;
-    std::vector<Job> pyr_jobs, det_jobs;
+    std::vector<Job> pyr_jobs[kPyrVariants], det_jobs;
     std::vector<int> nms_frame;
-    int pyr_blocks = 0, det_blocks = 0, nms_blocks = 0;
+    int pyr_units[kPyrVariants] = {}, det_blocks = 0, nms_blocks = 0;
     fs.hframes.assign(nf, FrameDev());
     for (int f = 0; f < nf; f++) {
         const Geom& g = geo[f];
@@ -295,10 +299,12 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
             F.tab_off[l] = tab_ints;
             if (!g.ref && l > 0) tab_ints += 2 * (g.w[l] + g.h[l]);
             if (g.radius[l] > rmax) rmax = g.radius[l];
-            /* pyramid tiles cover the whole pitch (padding written too) */
-            const int ptx = g.p[l] / kPyrTW, pty = (g.h[l] + kPyrTH - 1) / kPyrTH;
-            pyr_jobs.push_back(Job{f, l, ptx, pyr_blocks});
-            pyr_blocks += ptx * pty;
+            /* pyramid strips cover the whole pitch (padding written too) */
+            const int ptx = (g.p[l] + kPyrStripW - 1) / kPyrStripW;
+            for (int v = 0; v < kPyrVariants; v++) {
+                pyr_jobs[v].push_back(Job{f, l, ptx, pyr_units[v]});
+                pyr_units[v] += ptx * ((g.h[l] + kPyrTHs[v] - 1) / kPyrTHs[v]);
+            }
             if (g.w[l] > 6 && g.h[l] > 6) {
                 /* tiles cover [0, w-3) x [0, h-3); pixels outside [3, w-3) x [3, h-3) are masked */
                 const int dtx = (g.w[l] - 3 + kDetTW - 1) / kDetTW, dty = (g.h[l] - 3 + kDetTH - 1) / kDetTH;
@@ -328,7 +334,8 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     size_t off = 0;
     auto carve = [&](size_t bytes) { size_t o = off; off += round256(bytes ? bytes : 1); return o; };
     const size_t o_frames = carve(sizeof(FrameDev) * nf);
-    const size_t o_pyr_jobs = carve(sizeof(Job) * pyr_jobs.size());
+    size_t o_pyr_jobs[kPyrVariants];
+    for (int v = 0; v < kPyrVariants; v++) o_pyr_jobs[v] = carve(sizeof(Job) * pyr_jobs[v].size());
     const size_t o_det_jobs = carve(sizeof(Job) * det_jobs.size());
     const size_t o_nms_frame = carve(sizeof(int) * nms_frame.size());
     const size_t o_cand = carve(sizeof(Candidate) * (size_t)nms_blocks * kNmsCap);
@@ -346,7 +353,7 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     char* base = (char*)arena;
     HIP_TRY(c, hipMemsetAsync(arena, 0, total, c->stream));
     fs.d_frames = (FrameDev*)(base + o_frames);
-    fs.d_pyr = (Job*)(base + o_pyr_jobs);
+    for (int v = 0; v < kPyrVariants; v++) fs.d_pyr[v] = (Job*)(base + o_pyr_jobs[v]);
     fs.d_det = (Job*)(base + o_det_jobs);
     fs.d_nms_frame = (int*)(base + o_nms_frame);
     fs.d_cand = (Candidate*)(base + o_cand);
@@ -373,7 +380,9 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
         }
     }
     std::vector<uint32_t> epochs(nf, 1u);
-    HIP_TRY(c, hipMemcpyAsync(fs.d_pyr, pyr_jobs.data(), sizeof(Job) * pyr_jobs.size(), hipMemcpyHostToDevice, c->stream));
+    for (int v = 0; v < kPyrVariants; v++)
+        HIP_TRY(c, hipMemcpyAsync(fs.d_pyr[v], pyr_jobs[v].data(), sizeof(Job) * pyr_jobs[v].size(),
+                                  hipMemcpyHostToDevice, c->stream));
     if (!det_jobs.empty())
         HIP_TRY(c, hipMemcpyAsync(fs.d_det, det_jobs.data(), sizeof(Job) * det_jobs.size(), hipMemcpyHostToDevice, c->stream));
     if (!nms_frame.empty())
@@ -389,8 +398,8 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     fs.border = border;
     fs.cap = P.max_pts;
     fs.nframes = nf;
-    fs.pyr_njobs = (int)pyr_jobs.size();
-    fs.pyr_blocks = pyr_blocks;
+    fs.pyr_njobs = (int)pyr_jobs[0].size();
+    for (int v = 0; v < kPyrVariants; v++) fs.pyr_units[v] = pyr_units[v];
     fs.det_njobs = (int)det_jobs.size();
     fs.det_blocks = det_blocks;
     fs.nms_blocks = nms_blocks;
@@ -442,6 +451,50 @@ Tables make_tables(const orb_ctx* c)
     return t;
 }
 
+void set_pyramid_variant(const FrameSet& fs, DetectLaunch& d, int v)
+{
+    d.pyr_variant = v;
+    d.pyr_jobs = fs.d_pyr[v];
+    d.pyr_units = fs.pyr_units[v];
+    d.pyr_blocks = (fs.pyr_units[v] + 3) / 4;
+}
+
+/* Autotuner for the pyramid strip height: on the first call with a new list
+   of frame sizes, every variant runs once untimed and kTuneReps times between
+   events on the context's stream (the stage is idempotent: it rewrites the
+   same levels from the same input); the fastest mean is kept for the frame
+   set. */
+int tune_pyramid(orb_ctx* c, DetectLaunch& d)
+{
+    constexpr int kTuneReps = 3;
+    FrameSet& fs = c->fs;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(c, hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_fail(c, e, "hipEventCreate");
+    }
+    int best = 0;
+    for (int v = 0; v < kPyrVariants && e == hipSuccess; v++) {
+        set_pyramid_variant(fs, d, v);
+        e = launch_pyramid(d, v, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(a, c->stream);
+        for (int r = 0; r < kTuneReps && e == hipSuccess; r++) e = launch_pyramid(d, v, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(b, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        fs.pyr_ms[v] = ms / kTuneReps;
+        if (fs.pyr_ms[v] < fs.pyr_ms[best]) best = v;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_fail(c, e, "pyramid autotune");
+    fs.pyr_choice = best;
+    return ORB_OK;
+}
+
 int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, const int* hs, const int* pitches,
                orb::OrbPoint* const* pts, uint8_t* const* desc, int* d_counts, int compute_desc)
 {
@@ -478,9 +531,7 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     memset(&d, 0, sizeof(d));
     d.frames = fs.d_frames;
     d.nframes = nf;
-    d.pyr_jobs = fs.d_pyr;
     d.pyr_njobs = fs.pyr_njobs;
-    d.pyr_blocks = fs.pyr_blocks;
     d.det_jobs = fs.d_det;
     d.det_njobs = fs.det_njobs;
     d.det_blocks = fs.det_blocks;
@@ -493,6 +544,15 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     const int need_blocks = (c->prm.max_pts + kDescPerBlock - 1) / kDescPerBlock;
     d.desc_blocks_per_frame = compute_desc ? (need_blocks < kDescMaxBlocks ? need_blocks : kDescMaxBlocks) : 0;
     d.t = make_tables(c);
+    int variant = c->pyr_forced;
+    if (variant < 0) {
+        if (fs.pyr_choice < 0) {
+            rc = tune_pyramid(c, d);
+            if (rc != ORB_OK) return rc;
+        }
+        variant = fs.pyr_choice;
+    }
+    set_pyramid_variant(fs, d, variant);
     StageEvents ev;
     memset(&ev, 0, sizeof(ev));
     hipError_t e = launch_detect(d, compute_desc, c->stream, take_events(c, ev, 0x2fu));
@@ -737,6 +797,26 @@ int orb_profile_read(orb_ctx* c, double* ms_sum, int* counts)
     return ORB_OK;
 }
 
+int orb_tune_set(orb_ctx* c, int stage, int variant)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (stage != ORB_TUNE_PYRAMID) return fail(c, ORB_E_ARG, "unknown tuning stage %d", stage);
+    if (variant < -1 || variant >= kPyrVariants) return fail(c, ORB_E_ARG, "variant %d out of range", variant);
+    c->pyr_forced = variant;
+    return ORB_OK;
+}
+
+int orb_tune_get(orb_ctx* c, int stage, int* variant, int* nvariants, float* ms)
+{
+    if (!c || !variant || !nvariants) return fail(c, ORB_E_ARG, "null argument");
+    if (stage != ORB_TUNE_PYRAMID) return fail(c, ORB_E_ARG, "unknown tuning stage %d", stage);
+    *nvariants = kPyrVariants;
+    *variant = c->pyr_forced >= 0 ? c->pyr_forced : c->fs.pyr_choice;
+    if (ms)
+        for (int v = 0; v < kPyrVariants; v++) ms[v] = c->fs.pyr_ms[v];
+    return ORB_OK;
+}
+
 int orb_debug_sincos(orb_ctx* c, const float* in, int n, float* s, float* co)
 {
     if (!c) return fail(nullptr, ORB_E_ARG, "null context");
@@ -749,6 +829,19 @@ int orb_debug_atan2(orb_ctx* c, const float* yx, int n, float* out)
     if (!c) return fail(nullptr, ORB_E_ARG, "null context");
     hipError_t e = launch_debug_atan2(yx, n, out, c->stream);
     return e == hipSuccess ? ORB_OK : hip_fail(c, e, "debug atan2");
+}
+
+int orb_debug_pyramid(orb_ctx* c, int frame, int which, uint8_t* d_out, size_t cap, size_t* nbytes)
+{
+    if (!c || !nbytes) return fail(c, ORB_E_ARG, "null argument");
+    if (frame < 0 || frame >= c->fs.nframes) return fail(c, ORB_E_STATE, "no frame %d in the last call", frame);
+    if (which != 0 && which != 1) return fail(c, ORB_E_ARG, "which must be 0 (raw) or 1 (blurred)");
+    const FrameDev& F = c->fs.hframes[frame];
+    *nbytes = (size_t)F.total;
+    if (!d_out) return ORB_OK;
+    if (cap < (size_t)F.total) return fail(c, ORB_E_CAPACITY, "buffer of %zu bytes < %lld", cap, F.total);
+    HIP_TRY(c, hipMemcpyAsync(d_out, which ? F.blur : F.raw, (size_t)F.total, hipMemcpyDeviceToDevice, c->stream));
+    return ORB_OK;
 }
 
 } // extern "C"

@@ -31,8 +31,12 @@ constexpr int kRefMaxLevels = 5;     /* MAX_OCTAVE, orbd.cu:18 */
 constexpr int kEpochShift = 34;
 constexpr uint32_t kEpochLimit = 1u << 30;
 
-/* pyramid tile: 64 x 16 outputs, 256 threads */
-constexpr int kPyrTW = 64, kPyrTH = 16;
+/* pyramid unit: one wavefront per strip of 248 columns (62 lanes x 4) x TH
+   rows of one level; 4 units per 256-thread workgroup.  The strip heights
+   are the autotuner's variants (TH + 6 walked rows fit one wavefront). */
+constexpr int kPyrStripW = 248;
+constexpr int kPyrVariants = 5;
+constexpr int kPyrTHs[kPyrVariants] = {15, 22, 29, 36, 50};
 /* detection tile: 64 x 16 pixels, 256 threads */
 constexpr int kDetTW = 64, kDetTH = 16;
 /* NMS block: one level-0 row segment of 1024 pixels, 4 per thread; at most
@@ -91,7 +95,7 @@ struct Tables {
 struct DetectLaunch {
     const FrameDev* frames;
     int nframes;
-    const Job* pyr_jobs; int pyr_njobs; int pyr_blocks;
+    const Job* pyr_jobs; int pyr_njobs; int pyr_blocks; int pyr_units; int pyr_variant;
     const Job* det_jobs; int det_njobs; int det_blocks;
     int nms_blocks;
     const int* nms_block_frame; /* frame of each NMS block */
@@ -116,6 +120,8 @@ struct StageEvents {
 
 /* orb_kernels.hip */
 hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s, const StageEvents* ev);
+/* the pyramid stage alone with strip-height variant `variant` (autotuner) */
+hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s);
 hipError_t launch_debug_sincos(const float* in, int n, float* s, float* c, hipStream_t st);
 hipError_t launch_debug_atan2(const float* yx, int n, float* out, hipStream_t st);
 

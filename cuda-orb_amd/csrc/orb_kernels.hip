@@ -64,6 +64,20 @@ __device__ __forceinline__ void dev_sincos(float a, float& sn, float& cs)
 
 /* ------------------------------------------------------------- utilities */
 
+/* Pointers read from FrameDev are generic; the global-space view makes the
+   compiler emit global_* instead of flat_* memory operations, which it can
+   wait on per class (vmcnt) without draining LDS traffic as well. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ORB_GLOBAL __attribute__((address_space(1)))
+#else
+#define ORB_GLOBAL /* the host pass only parses device code */
+#endif
+template <typename T>
+__device__ __forceinline__ ORB_GLOBAL T* as_global(T* p)
+{
+    return (ORB_GLOBAL T*)p;
+}
+
 __device__ __forceinline__ Job find_job(const Job* __restrict__ jobs, int n, int b)
 {
     int lo = 0, hi = n - 1;
@@ -87,22 +101,6 @@ __device__ __forceinline__ int reflect101(int i, int n)
 __device__ __forceinline__ const int* level_tab(const FrameDev& F, const int* tabs, int l)
 {
     return tabs + F.tab_off[l];
-}
-
-/* level-l pixel (x, y) computed from the input image */
-__device__ __forceinline__ int src_value(const FrameDev& F, const int* __restrict__ tabs, int l, int x, int y)
-{
-    if (l == 0) return F.img[(long long)y * F.ip + x];
-    if (F.ref_mode) return F.img[(long long)(y << l) * F.ip + (x << l)];
-    const int* t = level_tab(F, tabs, l);
-    const int tx = t[x], ty = t[F.lw[l] + y];
-    const int x0 = tx >> 8, fx = tx & 255, y0 = ty >> 8, fy = ty & 255;
-    const int x1 = min(x0 + 1, F.w0 - 1), y1 = min(y0 + 1, F.h0 - 1);
-    const uint8_t* r0 = F.img + (long long)y0 * F.ip;
-    const uint8_t* r1 = F.img + (long long)y1 * F.ip;
-    const int top = (256 - fx) * r0[x0] + fx * r0[x1];
-    const int bot = (256 - fx) * r1[x0] + fx * r1[x1];
-    return ((256 - fy) * top + fy * bot + 32768) >> 16;
 }
 
 __device__ __forceinline__ int to_l0(const FrameDev& F, const int* __restrict__ tabs, int l, int v, bool is_y)
@@ -136,186 +134,235 @@ __device__ __forceinline__ int wave_sum(int v)
 
 /* ---------------------------------------------------------------- pyramid */
 
-/* Workgroup b of n runs logical tile xcd_remap(b): the hardware deals
+/* Workgroup b of n runs logical unit xcd_remap(b): the hardware deals
    workgroups round-robin to the 8 XCDs, so this gives every XCD (and its L2)
-   one contiguous range of tiles, and vertically adjacent tiles, which share
-   halo rows, land on the same L2. */
+   one contiguous range of units, and neighbouring units, which share source
+   rows and halo rows, land on the same L2. */
 __device__ __forceinline__ int xcd_remap(int b, int n)
 {
     const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
     return x * q + min(x, r) + k;
 }
 
-/* staged level tile: rows y0-3 .. y0+TH+3, columns x0-4 .. x0+TW+4 */
-constexpr int kPyrSH = kPyrTH + 6, kPyrSW = kPyrTW + 8;
-/* LDS budget for the level-0 source rectangle of a bilinear tile */
-constexpr int kPyrSrcBytes = 20480;
-
-__device__ __forceinline__ int bilerp(int a, int b, int c, int d, int fx, int fy)
+/* value of the lane whose byte address (4 * lane) is given */
+__device__ __forceinline__ uint32_t lane_get(uint32_t v, int addr)
 {
-    const int top = (256 - fx) * a + fx * b;
-    const int bot = (256 - fx) * c + fx * d;
-    return ((256 - fy) * top + fy * bot + 32768) >> 16;
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
 }
 
-__global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
-                                                 int njobs, Tables T)
-{
-    __shared__ __align__(16) uint8_t s_raw[kPyrSH][kPyrSW];
-    __shared__ __align__(16) float s_h[kPyrSH][kPyrTW];
-    __shared__ __align__(16) uint8_t s_src[kPyrSrcBytes];
-    __shared__ int s_xt[kPyrSW], s_yt[kPyrSH];
-    __shared__ int s_rect[4];
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const Job jb = find_job(jobs, njobs, b);
-    const FrameDev& F = frames[jb.frame];
-    const int l = jb.level;
-    const int t = b - jb.first_block;
-    const int x0 = (t % jb.tiles_x) * kPyrTW, y0 = (t / jb.tiles_x) * kPyrTH;
-    const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
-    const int tid = threadIdx.x;
-    uint8_t* raw = F.raw + F.loff[l];
-    uint8_t* blur = F.blur + F.loff[l];
-    const bool inside = x0 < w;
+/* Loaded bytes of one level row for the 4 columns of a lane, kept apart from
+   their use so the loads of later rows are in flight while a row is computed.
+   MODE 0: level 0 (input copy), 1: scale-2 decimation, 2: bilinear. */
+template <int MODE>
+struct PyrRow {
+    uint8_t v[MODE == 2 ? 16 : 4];
+    uint32_t d;     /* level 0: the 4 pixels as one dword */
+    uint32_t pad;   /* level 0: the input's bytes in the row-padding columns */
+    int fy;
+};
 
-    if (inside) {
-        if (l == 0 || F.ref_mode) {
-            /* level 0, or scale-2 nearest decimation: direct reads of the input */
-            const int sh = l;
-            const bool vec = l == 0 && (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0 && x0 >= 4 &&
-                             x0 + kPyrSW - 4 <= w;
-            for (int i = tid; i < kPyrSH * (kPyrSW / 4); i += 256) {
-                const int r = i / (kPyrSW / 4), c4 = 4 * (i - r * (kPyrSW / 4));
-                const int yy = reflect101(y0 - 3 + r, h);
-                const uint8_t* row = F.img + (long long)(yy << sh) * F.ip;
-                if (vec) {
-                    *(uint32_t*)&s_raw[r][c4] = *(const uint32_t*)(row + x0 - 4 + c4);
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 4; k++) s_raw[r][c4 + k] = row[reflect101(x0 - 4 + c4 + k, w) << sh];
-                }
-            }
-        } else {
-            /* bilinear level: per-tile source tables, then the level-0 source
-               rectangle staged in LDS with dword loads */
-            const int* tab = level_tab(F, T.tabs, l);
-            if (tid < 4) s_rect[tid] = (tid & 1) ? -1 : 0x7fffffff;
-            __syncthreads();
-            if (tid < kPyrSW) {
-                const int v = tab[reflect101(x0 - 4 + tid, w)];
-                s_xt[tid] = v;
-                atomicMin(&s_rect[0], v >> 8);
-                atomicMax(&s_rect[1], min((v >> 8) + 1, F.w0 - 1));
-            } else if (tid >= 128 && tid < 128 + kPyrSH) {
-                const int r = tid - 128;
-                const int v = tab[w + reflect101(y0 - 3 + r, h)];
-                s_yt[r] = v;
-                atomicMin(&s_rect[2], v >> 8);
-                atomicMax(&s_rect[3], min((v >> 8) + 1, F.h0 - 1));
-            }
-            __syncthreads();
-            const int sx0 = s_rect[0] & ~3, sx1 = s_rect[1], sy0 = s_rect[2], sy1 = s_rect[3];
-            const int sw = ((sx1 - sx0 + 4) & ~3), shh = sy1 - sy0 + 1;
-            const bool staged = sw * shh <= kPyrSrcBytes && (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
-            if (staged) {
-                const int dw = sw >> 2;
-                for (int i = tid; i < dw * shh; i += 256) {
-                    const int r = i / dw, c = i - r * dw;
-                    *(uint32_t*)&s_src[r * sw + 4 * c] =
-                        *(const uint32_t*)(F.img + (long long)(sy0 + r) * F.ip + sx0 + 4 * c);
-                }
-                __syncthreads();
-            }
-            for (int i = tid; i < kPyrSH * kPyrSW; i += 256) {
-                const int r = i / kPyrSW, c = i - r * kPyrSW;
-                const int tx = s_xt[c], ty = s_yt[r];
-                const int xa = tx >> 8, fx = tx & 255, ya = ty >> 8, fy = ty & 255;
-                const int xb = min(xa + 1, F.w0 - 1), yb = min(ya + 1, F.h0 - 1);
-                int a, bb, cc, d;
-                if (staged) {
-                    const uint8_t* r0 = s_src + (ya - sy0) * sw - sx0;
-                    const uint8_t* r1 = s_src + (yb - sy0) * sw - sx0;
-                    a = r0[xa]; bb = r0[xb]; cc = r1[xa]; d = r1[xb];
-                } else {
-                    const uint8_t* r0 = F.img + (long long)ya * F.ip;
-                    const uint8_t* r1 = F.img + (long long)yb * F.ip;
-                    a = r0[xa]; bb = r0[xb]; cc = r1[xa]; d = r1[xb];
-                }
-                s_raw[r][c] = (uint8_t)bilerp(a, bb, cc, d, fx, fy);
-            }
-        }
-    }
-    __syncthreads();
-    /* raw level (and padding) as dwords: thread -> row tid/16, columns 4*(tid%16) */
-    {
-        const int r = tid >> 4, c4 = 4 * (tid & 15);
-        const int y = y0 + r, x = x0 + c4;
-        if (y < h) {
-            uint32_t v;
-            if (x + 4 <= w) {
-                v = *(const uint32_t*)&s_raw[r + 3][c4 + 4];
-            } else {
-                /* row padding: level 0 keeps the input's bytes, other levels 0 */
-                v = 0;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int xx = x + k;
-                    uint32_t byte;
-                    if (xx < w) byte = s_raw[r + 3][c4 + 4 + k];
-                    else byte = (l == 0 && xx < F.ip) ? F.img[(long long)y * F.ip + xx] : 0u;
-                    v |= byte << (8 * k);
-                }
-            }
-            *(uint32_t*)(raw + (long long)y * p + x) = v;
-            /* a tile wholly in the row padding: the blurred level keeps the same bytes */
-            if (!inside) *(uint32_t*)(blur + (long long)y * p + x) = v;
-        }
-    }
-    if (!inside) return;
-    /* horizontal 7-tap pass on integer pair sums: 4 outputs per work item */
-    for (int i = tid; i < kPyrSH * (kPyrTW / 4); i += 256) {
-        const int r = i >> 4, c4 = 4 * (i & 15);
-        const uint8_t* sr = &s_raw[r][c4 + 1];   /* columns x0 + c4 - 3 .. x0 + c4 + 6 */
-        int q[10];
-#pragma unroll
-        for (int k = 0; k < 10; k++) q[k] = sr[k];
-        float4 o;
-        float* of = (float*)&o;
+struct PyrLane {
+    const ORB_GLOBAL uint8_t* img;
+    long long ip;
+    int cx, w, l, h0;
+    uint32_t xa[4], xb[4];   /* unsigned column offsets: address = uniform row base + offset */
+    int fx[4];
+    bool dword_ok, any_pad;
+};
+
+template <int MODE>
+__device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRow<MODE>& R)
+{
+    if (MODE == 2) {
+        const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
+        const ORB_GLOBAL uint8_t* ra = P.img + ya * P.ip;
+        const ORB_GLOBAL uint8_t* rb = P.img + yb * P.ip;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            float v = __fmul_rn(T.k0, (float)q[k + 3]);
-            v = __fmaf_rn(T.k1, (float)(q[k + 2] + q[k + 4]), v);
-            v = __fmaf_rn(T.k2, (float)(q[k + 1] + q[k + 5]), v);
-            v = __fmaf_rn(T.k3, (float)(q[k] + q[k + 6]), v);
-            of[k] = v;
+            R.v[4 * k + 0] = ra[P.xa[k]];
+            R.v[4 * k + 1] = ra[P.xb[k]];
+            R.v[4 * k + 2] = rb[P.xa[k]];
+            R.v[4 * k + 3] = rb[P.xb[k]];
         }
-        *(float4*)&s_h[r][c4] = o;
-    }
-    __syncthreads();
-    /* vertical pass, round half to even, 4 pixels per thread written as a dword */
-    {
-        const int r = tid >> 4, c4 = 4 * (tid & 15);
-        const int y = y0 + r, x = x0 + c4;
-        if (y < h) {
-            float4 col[7];
+        R.fy = ty & 255;
+        R.pad = 0;
+    } else {
+        const ORB_GLOBAL uint8_t* row = P.img + (long long)(yr << (MODE == 1 ? P.l : 0)) * P.ip;
+        R.d = 0;
+        if (MODE == 0 && P.dword_ok) {
+            R.d = *(const ORB_GLOBAL uint32_t*)(row + (uint32_t)P.cx);
+        } else {
 #pragma unroll
-            for (int k = 0; k < 7; k++) col[k] = *(const float4*)&s_h[r + k][c4];
-            const float* cf = (const float*)col;
-            uint32_t v = 0;
+            for (int k = 0; k < 4; k++) R.v[k] = row[P.xa[k]];
+        }
+        R.pad = 0;
+        if (MODE == 0 && P.any_pad) {
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                float a = __fmul_rn(T.k0, cf[4 * 3 + k]);
-                a = __fmaf_rn(T.k1, __fadd_rn(cf[4 * 2 + k], cf[4 * 4 + k]), a);
-                a = __fmaf_rn(T.k2, __fadd_rn(cf[4 * 1 + k], cf[4 * 5 + k]), a);
-                a = __fmaf_rn(T.k3, __fadd_rn(cf[4 * 0 + k], cf[4 * 6 + k]), a);
-                uint32_t byte = (uint32_t)(uint8_t)__float2int_rn(a);
-                /* row padding keeps the raw level's bytes */
-                if (x + k >= w) byte = (l == 0 && x + k < F.ip) ? F.img[(long long)y * F.ip + x + k] : 0u;
-                v |= byte << (8 * k);
+                const int x = P.cx + k;
+                if (x >= P.w && x < P.ip) R.pad |= (uint32_t)row[(uint32_t)x] << (8 * k);
             }
-            *(uint32_t*)(blur + (long long)y * p + x) = v;
+        }
+        R.fy = 0;
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ uint32_t pyr_pixels(const PyrLane& P, const PyrRow<MODE>& R)
+{
+    if (MODE == 2) {
+        uint32_t hin = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int top = (256 - P.fx[k]) * (int)R.v[4 * k + 0] + P.fx[k] * (int)R.v[4 * k + 1];
+            const int bot = (256 - P.fx[k]) * (int)R.v[4 * k + 2] + P.fx[k] * (int)R.v[4 * k + 3];
+            hin |= (uint32_t)(((256 - R.fy) * top + R.fy * bot + 32768) >> 16) << (8 * k);
+        }
+        return hin;
+    }
+    if (MODE == 0 && P.dword_ok) return R.d;
+    return (uint32_t)R.v[0] | ((uint32_t)R.v[1] << 8) | ((uint32_t)R.v[2] << 16) | ((uint32_t)R.v[3] << 24);
+}
+
+/* One wavefront builds a strip of kPyrStripW columns x TH rows of one level:
+   lane i owns the 4 columns xs - 4 + 4i .. +3 (lanes 0 and 63 only feed the
+   horizontal taps of their neighbours).  The lane walks rows ys-3 .. ys+TH+2
+   (reflect-101 at the level's top and bottom): per row it forms the 4 level
+   pixels (copy of the input, scale-2 decimation, or 8-bit bilinear from the
+   input), takes its neighbours' pixels across lanes, runs the horizontal
+   7-tap pass and keeps the last 7 results in registers, from which the
+   vertical pass emits the blurred row 3 above.  The loads of a row are issued
+   two rows ahead.  Strips cover the whole pitch: row padding is written as
+   the input's bytes at level 0 and as 0 elsewhere, in both the raw and the
+   blurred level (orbd.cu:514-535, 1273-1338; blur out of place). */
+template <int MODE, int TH>
+__device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int ys, int h, int p,
+                                         ORB_GLOBAL uint8_t* raw, ORB_GLOBAL uint8_t* blur, int tyv, bool writer)
+{
+    const int lane = threadIdx.x & 63;
+    const int addr_up = ((lane + 63) & 63) << 2, addr_dn = ((lane + 1) & 63) << 2;
+    /* rows walked: the strip's rows plus 3 above and below, in steps of 7 */
+    const int nwalk = min(TH, h - ys) + 6;
+    auto row_ty = [&](int i) { return __builtin_amdgcn_readlane(tyv, min(i, nwalk - 1)); };
+    auto row_y = [&](int i) { return reflect101(ys - 3 + min(i, nwalk - 1), h); };
+    PyrRow<MODE> s0, s1;
+    pyr_load<MODE>(P, row_y(0), row_ty(0), s0);
+    pyr_load<MODE>(P, row_y(1), row_ty(1), s1);
+    float4 ring[7];
+    uint32_t padring[7];
+    for (int i0 = 0; i0 < nwalk; i0 += 7) {
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            const int i = i0 + j;
+            PyrRow<MODE> sn;
+            pyr_load<MODE>(P, row_y(i + 2), row_ty(i + 2), sn);
+            const uint32_t hin = pyr_pixels<MODE>(P, s0);
+            const uint32_t lf = lane_get(hin, addr_up), rt = lane_get(hin, addr_dn);
+            const int rr = ys - 3 + i;
+            /* raw output; padding columns keep the input's bytes at level 0, 0 elsewhere */
+            uint32_t padmask = 0;
+            if (P.any_pad) {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (P.cx + k >= P.w) padmask |= 0xffu << (8 * k);
+            }
+            padring[j] = s0.pad;
+            if (rr >= ys && rr < ys + TH && rr < h && writer)
+                *(ORB_GLOBAL uint32_t*)(raw + (long long)rr * p + (uint32_t)P.cx) = (hin & ~padmask) | s0.pad;
+            /* horizontal pass: bytes q1..q10 = columns cx-3 .. cx+6 */
+            int q[12];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                q[k] = (lf >> (8 * k)) & 255;
+                q[4 + k] = (hin >> (8 * k)) & 255;
+                q[8 + k] = (rt >> (8 * k)) & 255;
+            }
+            float4 hv;
+            float* hf = (float*)&hv;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float v = __fmul_rn(T.k0, (float)q[k + 4]);
+                v = __fmaf_rn(T.k1, (float)(q[k + 3] + q[k + 5]), v);
+                v = __fmaf_rn(T.k2, (float)(q[k + 2] + q[k + 6]), v);
+                v = __fmaf_rn(T.k3, (float)(q[k + 1] + q[k + 7]), v);
+                hf[k] = v;
+            }
+            ring[j] = hv;
+            /* vertical pass for row rr - 3 from ring rows rr-6 .. rr */
+            const int o = rr - 3;
+            if (o >= ys && o < ys + TH && o < h && writer) {
+                const float* c0 = (const float*)&ring[(j + 1) % 7];
+                const float* c1 = (const float*)&ring[(j + 2) % 7];
+                const float* c2 = (const float*)&ring[(j + 3) % 7];
+                const float* c3 = (const float*)&ring[(j + 4) % 7];
+                const float* c4 = (const float*)&ring[(j + 5) % 7];
+                const float* c5 = (const float*)&ring[(j + 6) % 7];
+                const float* c6 = (const float*)&ring[j];
+                uint32_t out = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float a = __fmul_rn(T.k0, c3[k]);
+                    a = __fmaf_rn(T.k1, __fadd_rn(c2[k], c4[k]), a);
+                    a = __fmaf_rn(T.k2, __fadd_rn(c1[k], c5[k]), a);
+                    a = __fmaf_rn(T.k3, __fadd_rn(c0[k], c6[k]), a);
+                    out |= (uint32_t)(uint8_t)__float2int_rn(a) << (8 * k);
+                }
+                *(ORB_GLOBAL uint32_t*)(blur + (long long)o * p + (uint32_t)P.cx) =
+                    (out & ~padmask) | padring[(j + 4) % 7];
+            }
+            s0 = s1;
+            s1 = sn;
         }
     }
+}
+
+template <int TH>
+__global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
+                                                 int njobs, int nunits, Tables T)
+{
+    static_assert(TH + 6 <= 64, "one lane per walked row holds its bilinear row entry");
+    const int lane = threadIdx.x & 63;
+    const int u = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+    if (u >= nunits) return;
+    const Job jb = find_job(jobs, njobs, u);
+    const FrameDev& F = frames[jb.frame];
+    const int l = jb.level;
+    const int t = u - jb.first_block;
+    const int xs = (t % jb.tiles_x) * kPyrStripW, ys = (t / jb.tiles_x) * TH;
+    const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
+    PyrLane P;
+    P.img = as_global(F.img);
+    P.ip = F.ip;
+    P.cx = xs - 4 + 4 * lane;
+    P.w = w;
+    P.l = l;
+    P.h0 = F.h0;
+    const int mode = l == 0 ? 0 : (F.ref_mode ? 1 : 2);
+    const int* tab = T.tabs + F.tab_off[l];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int xr = reflect101(P.cx + k, w);   /* reflect-101 column feeds the horizontal taps */
+        if (mode == 2) {
+            const int v = tab[xr];
+            P.xa[k] = (uint32_t)(v >> 8);
+            P.fx[k] = v & 255;
+            P.xb[k] = (uint32_t)min((v >> 8) + 1, F.w0 - 1);
+        } else {
+            P.xa[k] = (uint32_t)(xr << (mode == 1 ? l : 0));
+            P.xb[k] = P.xa[k];
+            P.fx[k] = 0;
+        }
+    }
+    P.dword_ok = mode == 0 && P.cx >= 0 && P.cx + 3 < w && (P.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
+    P.any_pad = P.cx + 3 >= w;
+    const bool writer = lane >= 1 && lane <= 62 && P.cx < p;
+    /* lane i holds the bilinear row entry of walked row i */
+    const int tyv = mode == 2 && lane < TH + 6 ? tab[w + reflect101(ys - 3 + lane, h)] : 0;
+    ORB_GLOBAL uint8_t* raw = as_global(F.raw + F.loff[l]);
+    ORB_GLOBAL uint8_t* blur = as_global(F.blur + F.loff[l]);
+    if (mode == 0) pyr_walk<0, TH>(P, T, ys, h, p, raw, blur, tyv, writer);
+    else if (mode == 1) pyr_walk<1, TH>(P, T, ys, h, p, raw, blur, tyv, writer);
+    else pyr_walk<2, TH>(P, T, ys, h, p, raw, blur, tyv, writer);
 }
 
 /* -------------------------------------------------------------- detection */
@@ -401,12 +448,39 @@ __device__ float harris_lds(const uint8_t* c, float scale)
     return __fmul_rn(__fmaf_rn(-0.04f, __int2float_rn(B), __int2float_rn(A)), scale);
 }
 
+/* Slot of this lane in a list that the wavefront appends to: one LDS atomic
+   per wavefront, lanes ordered by lane id; -1 where pred is false. */
+__device__ __forceinline__ int wave_append(bool pred, int* counter)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m == 0) return -1;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    const int before = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+    return pred ? base + before : -1;
+}
+
+/* FAST-9 + score of one detection tile, in three passes over the staged
+   level bytes:
+   1. every pixel: the compass test.  A contiguous 9-arc of the 16-ring
+      contains two compass points (ring 0/4/8/12) that are 4 apart, so a
+      corner has two such neighbouring compass points both darker than c-t
+      or both brighter than c+t; the rest are rejected here.
+   2. the compass survivors, compacted: the exact segment test of
+      orbd.cu:685-740 (quick test on opposite pairs, >8 consecutive of the
+      wrapped ring, darker arc first and exclusively).
+   3. the corners, compacted: Harris (or FAST) score and the 64-bit max into
+      the level-0 key map (orbd.cu:742-755). */
 __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
                                                 int njobs, Tables T, const uint32_t* __restrict__ epochs)
 {
     __shared__ __align__(16) uint8_t s_tile[kDetSH * kDetSW];
-    __shared__ uint16_t s_list[kDetTW * kDetTH];
-    __shared__ int s_n;
+    __shared__ uint16_t s_cand[kDetTW * kDetTH];
+    __shared__ uint16_t s_corner[kDetTW * kDetTH];
+    __shared__ int s_n[2];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const Job jb = find_job(jobs, njobs, b);
     const FrameDev& F = frames[jb.frame];
@@ -415,27 +489,51 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     const int x0 = (t % jb.tiles_x) * kDetTW, y0 = (t / jb.tiles_x) * kDetTH;
     const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
     const int tid = threadIdx.x;
-    if (tid == 0) s_n = 0;
+    if (tid < 2) s_n[tid] = 0;
     /* stage kDetSH rows x kDetSW bytes as dwords (every row start is dword aligned) */
     {
-        const uint8_t* base = F.raw + F.loff[l] + (long long)(y0 - 4) * p + (x0 - 4);
+        const ORB_GLOBAL uint8_t* base = as_global(F.raw) + F.loff[l] + (long long)(y0 - 4) * p + (x0 - 4);
         constexpr int dw_per_row = kDetSW / 4;
         for (int i = tid; i < kDetSH * dw_per_row; i += 256) {
             const int r = i / dw_per_row, c = i - r * dw_per_row;
-            *(uint32_t*)&s_tile[r * kDetSW + 4 * c] = *(const uint32_t*)(base + (long long)r * p + 4 * c);
+            *(uint32_t*)&s_tile[r * kDetSW + 4 * c] = *(const ORB_GLOBAL uint32_t*)(base + (long long)r * p + 4 * c);
         }
     }
     __syncthreads();
-    /* FAST-9 segment test; corners go to an LDS list */
     const int thr = T.threshold;
-    const int lx = tid & (kDetTW - 1);
+    /* pass 1: compass test on every pixel of the tile */
+    {
+        const int lx = tid & (kDetTW - 1);
+        const int ix = x0 + lx;
+        const bool col_ok = ix >= 3 && ix < w - 3;
 #pragma unroll
-    for (int rr = 0; rr < kDetTH / 4; rr++) {
-        const int ly = (tid >> 6) + 4 * rr;
-        const int ix = x0 + lx, iy = y0 + ly;
-        const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx + 4];
+        for (int rr = 0; rr < kDetTH / 4; rr++) {
+            const int ly = (tid >> 6) + 4 * rr;
+            const int iy = y0 + ly;
+            const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx + 4];
+            const int v = c[0];
+            const int q0 = c[RING_OFF(0, 3)], q4 = c[RING_OFF(3, 0)], q8 = c[RING_OFF(0, -3)],
+                      q12 = c[RING_OFF(-3, 0)];
+            const int lo = v - thr, hi = v + thr;
+            const bool d0 = q0 < lo, d4 = q4 < lo, d8 = q8 < lo, d12 = q12 < lo;
+            const bool b0 = q0 > hi, b4 = q4 > hi, b8 = q8 > hi, b12 = q12 > hi;
+            const bool dark = (d0 && d4) || (d4 && d8) || (d8 && d12) || (d12 && d0);
+            const bool bright = (b0 && b4) || (b4 && b8) || (b8 && b12) || (b12 && b0);
+            const bool cand = col_ok && iy >= 3 && iy < h - 3 && (dark || bright);
+            const int slot = wave_append(cand, &s_n[0]);
+            if (slot >= 0) s_cand[slot] = (uint16_t)((ly << 8) | lx);
+        }
+    }
+    __syncthreads();
+    /* pass 2: exact segment test on the candidates */
+    const int ncand = s_n[0];
+    for (int i0 = 0; i0 < ncand; i0 += 256) {
+        const int i = i0 + tid;
         bool corner = false;
-        if (ix >= 3 && ix < w - 3 && iy >= 3 && iy < h - 3) {
+        int e = 0;
+        if (i < ncand) {
+            e = s_cand[i];
+            const uint8_t* c = &s_tile[((e >> 8) + 4) * kDetSW + (e & 255) + 4];
             const int v = c[0];
             unsigned dark = 0, bright = 0;
 #pragma unroll
@@ -456,17 +554,16 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
             for (int k = 1; k < 9; k++) run &= r2 >> k;
             corner = run != 0;
         }
-        if (corner) {
-            const int slot = atomicAdd(&s_n, 1);
-            s_list[slot] = (uint16_t)((ly << 8) | lx);
-        }
+        const int slot = wave_append(corner, &s_n[1]);
+        if (slot >= 0) s_corner[slot] = (uint16_t)e;
     }
     __syncthreads();
-    const int n = s_n;
+    /* pass 3: score the corners into the key map */
+    const int n = s_n[1];
     if (n == 0) return;
     const unsigned long long ep = (unsigned long long)epochs[jb.frame] << kEpochShift;
     for (int i = tid; i < n; i += 256) {
-        const int e = s_list[i];
+        const int e = s_corner[i];
         const int ly = e >> 8, lx2 = e & 255;
         const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx2 + 4];
         float score;
@@ -483,7 +580,8 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
         const int X0 = to_l0(F, T.tabs, l, ix, false), Y0 = to_l0(F, T.tabs, l, iy, true);
         const unsigned long long key = ep | ((unsigned long long)__float_as_uint(score) << 3) |
                                        (unsigned long long)(7 - l);
-        atomicMax(F.keys + (long long)Y0 * F.p0 + X0, key);
+        __hip_atomic_fetch_max(as_global(F.keys) + (long long)Y0 * F.p0 + X0, key, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -527,7 +625,7 @@ __device__ __forceinline__ int block_scan(int v, int* s_tmp, int& total)
 /* radius <= 4 (every bilinear pyramid up to 8 levels at scale <= 1.25): per
    neighbour row, the 12 keys x-4 .. x+7 around the 4 pixels come in three
    aligned 32-byte loads and all comparisons run from registers */
-__device__ __forceinline__ uint32_t nms_rows4(const unsigned long long* __restrict__ rowp, long long p0, int xs,
+__device__ __forceinline__ uint32_t nms_rows4(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0, int xs,
                                               const unsigned long long* k4, uint32_t valid, uint32_t epoch,
                                               const int* rk, int R)
 {
@@ -536,7 +634,7 @@ __device__ __forceinline__ uint32_t nms_rows4(const unsigned long long* __restri
 #pragma unroll
     for (int k = 0; k < 4; k++) sk[k] = (uint32_t)(k4[k] >> 3) & 0x7fffffffu;
     for (int dy = -R; dy <= R; dy++) {
-        const ulonglong4* rp = (const ulonglong4*)(rowp + (long long)dy * p0 + xs - 4);
+        const ORB_GLOBAL ulonglong4* rp = (const ORB_GLOBAL ulonglong4*)(rowp + (long long)dy * p0 + xs - 4);
         unsigned long long nb[12];
 #pragma unroll
         for (int t = 0; t < 3; t++) {
@@ -565,7 +663,7 @@ __device__ __forceinline__ uint32_t nms_rows4(const unsigned long long* __restri
 
 /* any radius: the wavefront takes its candidates one at a time and spreads
    each neighbourhood over its 64 lanes */
-__device__ __forceinline__ uint32_t nms_wave(const unsigned long long* __restrict__ rowp, long long p0, int xs,
+__device__ __forceinline__ uint32_t nms_wave(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0, int xs,
                                              const unsigned long long* k4, uint32_t valid, uint32_t epoch,
                                              const int* __restrict__ radius)
 {
@@ -612,11 +710,11 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
     const int xs = F.nms_x0 + seg * kNmsSeg + 4 * threadIdx.x;
     const int xlo = F.border, xhi = F.w0 - F.border;
     const uint32_t epoch = epochs[f];
-    const unsigned long long* rowp = F.keys + (long long)y * F.p0;
+    const ORB_GLOBAL unsigned long long* rowp = as_global(F.keys) + (long long)y * F.p0;
     unsigned long long k4[4] = {0ull, 0ull, 0ull, 0ull};
     if (xs < xhi) {
         /* xs is a multiple of 4 below w0 <= p0 (a multiple of 128): 32-byte aligned, inside the row */
-        const ulonglong4 a = *(const ulonglong4*)(rowp + xs);
+        const ulonglong4 a = *(const ORB_GLOBAL ulonglong4*)(rowp + xs);
         k4[0] = a.x;
         k4[1] = a.y;
         k4[2] = a.z;
@@ -638,7 +736,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
     }
     int total;
     int pos = block_scan<256>(__popc(surv), s_tmp, total);
-    Candidate* out = cand + (long long)blockIdx.x * kNmsCap;
+    ORB_GLOBAL Candidate* out = as_global(cand) + (long long)blockIdx.x * kNmsCap;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if (surv & (1u << k)) {
@@ -654,7 +752,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
 
 /* ---------------------------------------------------------- select + emit */
 
-__device__ __forceinline__ void emit_point(orb::OrbPoint* o, Candidate c)
+__device__ __forceinline__ void emit_point(ORB_GLOBAL orb::OrbPoint* o, Candidate c)
 {
     o->x = (int)(c.pos & 0x3fffu);
     o->y = (int)((c.pos >> 14) & 0x3fffu);
@@ -693,7 +791,7 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
         for (int b = b0; b < b1; b++) {
             const int n = blk_count[first + b];
             const Candidate* cb = cand + (long long)(first + b) * kNmsCap;
-            for (int i = 0; i < n; i++) emit_point(F.pts + out++, cb[i]);
+            for (int i = 0; i < n; i++) emit_point(as_global(F.pts) + out++, cb[i]);
         }
     } else {
         uint32_t prefix = 0, pmask = 0;
@@ -746,16 +844,16 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
                 const Candidate c = cb[i];
                 bool kept = c.score > thr;
                 if (c.score == thr) kept = eq_before++ < need;
-                if (kept) emit_point(F.pts + out++, c);
+                if (kept) emit_point(as_global(F.pts) + out++, c);
             }
         }
         total = cap;
     }
-    if (tid == 0) *F.count = total;
+    if (tid == 0) *as_global(F.count) = total;
     /* advance the epoch; on wrap-around clear this frame's key map */
     if (epoch + 1 >= kEpochLimit) {
         const long long n = (long long)F.h0 * F.p0;
-        for (long long i = tid; i < n; i += 1024) F.keys[i] = 0ull;
+        for (long long i = tid; i < n; i += 1024) as_global(F.keys)[i] = 0ull;
         if (tid == 0) epochs[f] = 1;
     } else if (tid == 0) {
         epochs[f] = epoch + 1;
@@ -799,10 +897,10 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
     if (threadIdx.x < 17) s_umax[threadIdx.x] = T.umax[threadIdx.x];
     __syncthreads();
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = *F.count;
+    const int n = *as_global(F.count);
     const int hk = T.half_patch, side = 2 * hk + 1;
     for (int pi = lb * kDescPerBlock + wid; pi < n; pi += blocks_per_frame * kDescPerBlock) {
-        orb::OrbPoint* P = F.pts + pi;
+        ORB_GLOBAL orb::OrbPoint* P = as_global(F.pts) + pi;
         const int l = P->octave;
         const int cx = from_l0(F, T.tabs, l, P->x, false), cy = from_l0(F, T.tabs, l, P->y, true);
         const int p = F.lp[l];
@@ -810,20 +908,20 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
 
         /* stage both patches with dword loads (every row shares the keypoint's
            alignment because the pitch is a multiple of 128) */
-        const uint8_t* ra = F.raw + co - 15 - 15LL * p;
+        const ORB_GLOBAL uint8_t* ra = as_global(F.raw) + co - 15 - 15LL * p;
         const int ro = (int)((uintptr_t)ra & 3);
-        const uint8_t* rb = ra - ro;
+        const ORB_GLOBAL uint8_t* rb = ra - ro;
         for (int i = lane; i < 31 * (kRawRow / 4); i += 64) {
             const int r = i / (kRawRow / 4), d = i - r * (kRawRow / 4);
-            *(uint32_t*)&s_raw[wid][r * kRawRow + 4 * d] = *(const uint32_t*)(rb + (long long)r * p + 4 * d);
+            *(uint32_t*)&s_raw[wid][r * kRawRow + 4 * d] = *(const ORB_GLOBAL uint32_t*)(rb + (long long)r * p + 4 * d);
         }
-        const uint8_t* ba = F.blur + co - 18 - 18LL * p;
+        const ORB_GLOBAL uint8_t* ba = as_global(F.blur) + co - 18 - 18LL * p;
         const int bo = (int)((uintptr_t)ba & 3);
-        const uint8_t* bb = ba - bo;
+        const ORB_GLOBAL uint8_t* bb = ba - bo;
         if (compute_desc) {
             for (int i = lane; i < 37 * (kBlurRow / 4); i += 64) {
                 const int r = i / (kBlurRow / 4), d = i - r * (kBlurRow / 4);
-                *(uint32_t*)&s_blur[wid][r * kBlurRow + 4 * d] = *(const uint32_t*)(bb + (long long)r * p + 4 * d);
+                *(uint32_t*)&s_blur[wid][r * kBlurRow + 4 * d] = *(const ORB_GLOBAL uint32_t*)(bb + (long long)r * p + 4 * d);
             }
         }
         wave_sync_lds();
@@ -849,7 +947,7 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
             const uint8_t* bp = &s_blur[wid][18 * kBlurRow + bo + 18];
             if (T.wta_k == 2) {
                 /* bit k of the descriptor compares points 2k and 2k+1 (orbd.cu:1351-1361) */
-                unsigned long long* outd = (unsigned long long*)(F.desc + (long long)pi * 32);
+                ORB_GLOBAL unsigned long long* outd = (ORB_GLOBAL unsigned long long*)(as_global(F.desc) + (long long)pi * 32);
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const int k = lane + 64 * j;
@@ -883,7 +981,7 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
                 wave_sync_lds();
                 if (lane < 32) {
                     const uint8_t* sc = s_code[wid] + 4 * lane;
-                    F.desc[(long long)pi * 32 + lane] = (uint8_t)(sc[0] | (sc[1] << 2) | (sc[2] << 4) | (sc[3] << 6));
+                    as_global(F.desc)[(long long)pi * 32 + lane] = (uint8_t)(sc[0] | (sc[1] << 2) | (sc[2] << 4) | (sc[3] << 6));
                 }
             }
         }
@@ -912,12 +1010,32 @@ __global__ void k_debug_atan2(const float* yx, int n, float* out)
 
 /* ------------------------------------------------------------- launchers */
 
+hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s)
+{
+    switch (variant) {
+#define ORB_PYR_CASE(v)                                                                                     \
+    case v:                                                                                                 \
+        hipLaunchKernelGGL(k_pyramid<kPyrTHs[v]>, dim3(d.pyr_blocks), dim3(256), 0, s, d.frames, d.pyr_jobs, \
+                           d.pyr_njobs, d.pyr_units, d.t);                                                  \
+        break;
+        ORB_PYR_CASE(0)
+        ORB_PYR_CASE(1)
+        ORB_PYR_CASE(2)
+        ORB_PYR_CASE(3)
+        ORB_PYR_CASE(4)
+#undef ORB_PYR_CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s, const StageEvents* ev)
 {
     (void)hipGetLastError(); /* report only errors of these launches */
     if (d.pyr_blocks > 0) {
         ev_start(ev, kStPyramid, s);
-        hipLaunchKernelGGL(k_pyramid, dim3(d.pyr_blocks), dim3(256), 0, s, d.frames, d.pyr_jobs, d.pyr_njobs, d.t);
+        hipError_t e = launch_pyramid(d, d.pyr_variant, s);
+        if (e != hipSuccess) return e;
         ev_stop(ev, kStPyramid, s);
     }
     if (d.det_blocks > 0) {

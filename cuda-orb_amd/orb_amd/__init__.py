@@ -32,6 +32,7 @@ EXPORTS = [
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
     "orb_match_top2", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
+    "orb_tune_set", "orb_tune_get", "orb_debug_pyramid",
     "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
     "orb_lindex_logits", "orb_lindex_grads", "orb_lindex_num_params", "orb_lindex_get_weights",
     "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_save", "orb_lindex_load", "orb_lindex_set_probe",
@@ -40,6 +41,8 @@ EXPORTS = [
 ORB_LINDEX_MLP = 0
 ORB_LINDEX_CENTROIDS = 1
 
+TUNE_PYRAMID = 0
+PYRAMID_STRIP_HEIGHTS = (15, 22, 29, 36, 50)
 NUM_STAGES = 8
 STAGES = ["pyramid", "detect", "nms", "select", "emit", "describe", "match", "match_merge"]
 
@@ -93,6 +96,9 @@ def lib():
         L.orb_debug_atan2.argtypes = [P, V, I, V]
         L.orb_profile_enable.argtypes = [P, ctypes.c_uint]
         L.orb_profile_read.argtypes = [P, V, V]
+        L.orb_tune_set.argtypes = [P, I, I]
+        L.orb_tune_get.argtypes = [P, I, V, V, V]
+        L.orb_debug_pyramid.argtypes = [P, I, I, V, ctypes.c_size_t, V]
         L.orb_lindex_create.argtypes = [P, ctypes.POINTER(LindexParams), ctypes.POINTER(P)]
         L.orb_lindex_destroy.argtypes = [P]
         L.orb_lindex_build.argtypes = [P, V, I, I]
@@ -253,6 +259,19 @@ class Context:
         self._check(lib().orb_profile_read(self.h, _np_ptr(ms), _np_ptr(cnt)), "orb_profile_read")
         return {STAGES[i]: (float(ms[i]), int(cnt[i])) for i in range(NUM_STAGES)}
 
+    def tune_set(self, stage, variant):
+        """Force tile-shape variant `variant` of a tuned stage (-1: autotune)."""
+        self._check(lib().orb_tune_set(self.h, stage, variant), "orb_tune_set")
+
+    def tune_get(self, stage=0):
+        """(variant in use or -1, measured mean ms per variant) of a tuned stage."""
+        v = ctypes.c_int(-1)
+        n = ctypes.c_int(0)
+        ms = np.zeros(16, np.float32)
+        self._check(lib().orb_tune_get(self.h, stage, ctypes.byref(v), ctypes.byref(n), _np_ptr(ms)),
+                    "orb_tune_get")
+        return v.value, [float(x) for x in ms[: n.value]]
+
     def workspace_bytes(self):
         return lib().orb_workspace_bytes(self.h)
 
@@ -263,6 +282,16 @@ class Context:
         self._check(lib().orb_debug_sincos(self.h, _dptr(angles), angles.numel(), _dptr(s), _dptr(c)),
                     "orb_debug_sincos")
         return s, c
+
+    def debug_pyramid(self, frame=0, which=0):
+        """uint8 tensor: the frame's flat raw (0) or blurred (1) pyramid of the last detect call."""
+        self._bind_stream()
+        n = ctypes.c_size_t(0)
+        self._check(lib().orb_debug_pyramid(self.h, frame, which, None, 0, ctypes.byref(n)), "orb_debug_pyramid")
+        out = self._torch.empty(n.value, dtype=self._torch.uint8, device="cuda")
+        self._check(lib().orb_debug_pyramid(self.h, frame, which, _dptr(out), n.value, ctypes.byref(n)),
+                    "orb_debug_pyramid")
+        return out
 
     def debug_atan2(self, yx):
         self._bind_stream()

@@ -210,10 +210,27 @@ int orb_lindex_load(orb_lindex* lx, const char* path);
 int orb_profile_enable(orb_ctx* ctx, unsigned stage_mask);
 int orb_profile_read(orb_ctx* ctx, double* ms_sum, int* counts);
 
+/* Autotuner (no counterpart in the reference, whose launch shapes are fixed,
+   orbd.cu:1564-1793).  Tile shapes of a stage are chosen per list of frame
+   sizes on its first call: every variant is timed on the context's stream
+   and the fastest kept; all variants give identical results.
+   orb_tune_set forces a variant (-1 restores tuning); orb_tune_get reports
+   the variant in use for the current frame set (-1 before its first call),
+   the number of variants and, if ms is not null, the measured mean
+   milliseconds per variant (0 where not measured). */
+#define ORB_TUNE_PYRAMID 0     /* pyramid strip height */
+int orb_tune_set(orb_ctx* ctx, int stage, int variant);
+int orb_tune_get(orb_ctx* ctx, int stage, int* variant, int* nvariants, float* ms);
+
 /* Device self-checks of the math helpers, for tests: n angles in d_in give
    sin, cos (own polynomials) and n (y, x) pairs give the centroid atan2. */
 int orb_debug_sincos(orb_ctx* ctx, const float* d_in, int n, float* d_sin, float* d_cos);
 int orb_debug_atan2(orb_ctx* ctx, const float* d_yx, int n, float* d_out);
+/* For tests: copy frame `frame`'s flat pyramid of the last detect call
+   (which 0: raw levels, 1: blurred levels; all levels with their row
+   padding, *nbytes bytes) into d_out of cap bytes.  With d_out null only
+   *nbytes is set. */
+int orb_debug_pyramid(orb_ctx* ctx, int frame, int which, uint8_t* d_out, size_t cap, size_t* nbytes);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,44 @@ def test_input_padding_bytes_follow_the_flat_layout(torch, ctx, oracle):
     assert np.array_equal(got_d, desc[: n.value])
 
 
+PYR_CASES = [
+    # w, h, noctaves, scale: bilinear levels, scale-2 decimation, odd sizes and a pitch equal to the width
+    (640, 480, 8, 1.2),
+    (1920, 1080, 8, 1.2),
+    (333, 257, 8, 1.2),
+    (1280, 720, 8, 1.25),
+    (600, 400, 5, 2.0),
+    (1024, 200, 8, 1.2),
+]
+
+
+@pytest.mark.parametrize("w,h,nl,scale", PYR_CASES)
+def test_pyramid_bytes_every_strip_variant(torch, ctx, oracle, w, h, nl, scale):
+    """The raw and blurred flat pyramids, row padding included, are byte-identical to the oracle's
+    for every strip height the autotuner may pick; the input's own padding bytes are noise."""
+    import orb_amd
+    print(f"pyramid {w}x{h} levels={nl} scale={scale}", flush=True)
+    img = orb_amd.synth_image(11, w, h, 20)
+    img[:, w:] = np.random.default_rng(3).integers(0, 256, (h, img.shape[1] - w), dtype=np.uint8)
+    g = oracle.geometry(w, h, nl, scale)
+    want_raw, want_blur = oracle.pyramid(g, img)
+    ctx.init(noctaves=nl, max_pts=500, scale_factor=scale)
+    try:
+        for v in range(len(orb_amd.PYRAMID_STRIP_HEIGHTS)):
+            ctx.tune_set(orb_amd.TUNE_PYRAMID, v)
+            gpu_detect(torch, ctx, img, w, h, 500)
+            raw = ctx.debug_pyramid(0, 0).cpu().numpy()
+            blur = ctx.debug_pyramid(0, 1).cpu().numpy()
+            for name, got, want in (("raw", raw, want_raw), ("blur", blur, want_blur)):
+                bad = np.nonzero(got != want)[0]
+                assert len(bad) == 0, f"variant {v} {name}: {len(bad)} bytes differ, first at {bad[:5]}"
+    finally:
+        ctx.tune_set(orb_amd.TUNE_PYRAMID, -1)
+    gpu_detect(torch, ctx, img, w, h, 500)
+    chosen, ms = ctx.tune_get(orb_amd.TUNE_PYRAMID)
+    assert 0 <= chosen < len(ms) and all(t > 0 for t in ms)
+
+
 def test_constant_image_gives_no_points(torch, ctx):
     ctx.init(noctaves=8, max_pts=100, scale_factor=1.2)
     img = np.full((480, 640), 77, np.uint8)
