@@ -136,6 +136,7 @@ struct FrameSet {
 
 struct orb_ctx {
     int device = 0;
+    int cu_count = 256;        /* compute units of the device (persistent grids) */
     hipStream_t stream = nullptr;
     bool inited = false;
     orb_params prm{};
@@ -641,6 +642,9 @@ int orb_ctx_create(orb_ctx** out, int device)
     if (e != hipSuccess) return fail(nullptr, ORB_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
     orb_ctx* c = new orb_ctx();
     c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->cu_count = cus;
     e = hipMalloc((void**)&c->d_pattern, 512 * sizeof(int2));
     if (e != hipSuccess) {
         delete c;

@@ -37,8 +37,8 @@ constexpr uint32_t kEpochLimit = 1u << 30;
 constexpr int kPyrStripW = 248;
 constexpr int kPyrVariants = 5;
 constexpr int kPyrTHs[kPyrVariants] = {15, 22, 29, 36, 50};
-/* detection tile: 64 x 16 pixels, 256 threads */
-constexpr int kDetTW = 64, kDetTH = 16;
+/* detection tile: 128 x 32 pixels, 256 threads */
+constexpr int kDetTW = 128, kDetTH = 32;
 /* NMS block: one level-0 row segment of 1024 pixels, 4 per thread; at most
    every other pixel of a row survives, so 512 candidates fit a segment */
 constexpr int kNmsSeg = 1024;

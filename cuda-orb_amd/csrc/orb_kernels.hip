@@ -152,22 +152,31 @@ __device__ __forceinline__ uint32_t lane_get(uint32_t v, int addr)
 
 /* Loaded bytes of one level row for the 4 columns of a lane, kept apart from
    their use so the loads of later rows are in flight while a row is computed.
-   MODE 0: level 0 (input copy), 1: scale-2 decimation, 2: bilinear. */
+   MODE 0: level 0 from a dword-aligned input (dword loads + byte permute);
+   MODE 1: byte gathers (level 0 of an unaligned input, or scale-2
+   decimation); MODE 2: bilinear. */
 template <int MODE>
 struct PyrRow {
-    uint8_t v[MODE == 2 ? 16 : 4];
-    uint32_t d;     /* level 0: the 4 pixels as one dword */
-    uint32_t pad;   /* level 0: the input's bytes in the row-padding columns */
+    uint32_t v[MODE == 2 ? 16 : (MODE == 1 ? 8 : 3)];
     int fy;
 };
 
+/* Per-lane constants of a strip.  All memory goes through buffer resources:
+   the row base is a wave-uniform scalar offset and the column a 32-bit
+   vector offset, and an offset at or above the resource's size (kOob) reads
+   0 and drops stores, which replaces branches around edge rows and lanes.
+   kOob exceeds every level and image size (< 2^28 bytes), and a row offset
+   and a column offset that are both kOob still sum below 2^32. */
+constexpr uint32_t kOob = 0x40000000u;
+
 struct PyrLane {
-    const ORB_GLOBAL uint8_t* img;
-    long long ip;
-    int cx, w, l, h0;
-    uint32_t xa[4], xb[4];   /* unsigned column offsets: address = uniform row base + offset */
+    __amdgpu_buffer_rsrc_t src;  /* the input image, h0 * ip bytes */
+    int ip, l, h0;
+    uint32_t xa[4], xb[4];       /* source byte offsets in a row (MODE 1: xb = the padding columns' own bytes) */
     int fx[4];
-    bool dword_ok, any_pad;
+    uint32_t base4, sel, direct; /* MODE 0: reflect window start, byte selector, the lane's own dword */
+    uint32_t padmask;            /* bytes of the lane's dword that lie in the row padding */
+    uint32_t ipmask;             /* level 0: padding bytes that exist in the input row */
 };
 
 template <int MODE>
@@ -175,53 +184,54 @@ __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRo
 {
     if (MODE == 2) {
         const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
-        const ORB_GLOBAL uint8_t* ra = P.img + ya * P.ip;
-        const ORB_GLOBAL uint8_t* rb = P.img + yb * P.ip;
+        const int sa = ya * P.ip, sb = yb * P.ip;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            R.v[4 * k + 0] = ra[P.xa[k]];
-            R.v[4 * k + 1] = ra[P.xb[k]];
-            R.v[4 * k + 2] = rb[P.xa[k]];
-            R.v[4 * k + 3] = rb[P.xb[k]];
+            R.v[4 * k + 0] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xa[k], sa, 0);
+            R.v[4 * k + 1] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], sa, 0);
+            R.v[4 * k + 2] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xa[k], sb, 0);
+            R.v[4 * k + 3] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], sb, 0);
         }
         R.fy = ty & 255;
-        R.pad = 0;
+    } else if (MODE == 1) {
+        const int so = (yr << P.l) * P.ip;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            R.v[k] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xa[k], so, 0);
+            R.v[4 + k] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], so, 0);
+        }
+        R.fy = 0;
     } else {
-        const ORB_GLOBAL uint8_t* row = P.img + (long long)(yr << (MODE == 1 ? P.l : 0)) * P.ip;
-        R.d = 0;
-        if (MODE == 0 && P.dword_ok) {
-            R.d = *(const ORB_GLOBAL uint32_t*)(row + (uint32_t)P.cx);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; k++) R.v[k] = row[P.xa[k]];
-        }
-        R.pad = 0;
-        if (MODE == 0 && P.any_pad) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int x = P.cx + k;
-                if (x >= P.w && x < P.ip) R.pad |= (uint32_t)row[(uint32_t)x] << (8 * k);
-            }
-        }
+        const int so = yr * P.ip;
+        R.v[0] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.base4, so, 0);
+        R.v[1] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.base4 + 4, so, 0);
+        R.v[2] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.direct, so, 0);
         R.fy = 0;
     }
 }
 
+/* the 4 level pixels (reflect-101 columns) and the raw output dword (row
+   padding: the input's bytes at level 0, 0 elsewhere) */
 template <int MODE>
-__device__ __forceinline__ uint32_t pyr_pixels(const PyrLane& P, const PyrRow<MODE>& R)
+__device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>& R, uint32_t& hin, uint32_t& out)
 {
     if (MODE == 2) {
-        uint32_t hin = 0;
+        hin = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const int top = (256 - P.fx[k]) * (int)R.v[4 * k + 0] + P.fx[k] * (int)R.v[4 * k + 1];
             const int bot = (256 - P.fx[k]) * (int)R.v[4 * k + 2] + P.fx[k] * (int)R.v[4 * k + 3];
             hin |= (uint32_t)(((256 - R.fy) * top + R.fy * bot + 32768) >> 16) << (8 * k);
         }
-        return hin;
+        out = hin & ~P.padmask;
+    } else if (MODE == 1) {
+        hin = R.v[0] | (R.v[1] << 8) | (R.v[2] << 16) | (R.v[3] << 24);
+        const uint32_t pad = R.v[4] | (R.v[5] << 8) | (R.v[6] << 16) | (R.v[7] << 24);
+        out = (hin & ~P.padmask) | pad;
+    } else {
+        hin = __builtin_amdgcn_perm(R.v[1], R.v[0], P.sel);
+        out = (hin & ~P.padmask) | (R.v[2] & P.padmask & P.ipmask);
     }
-    if (MODE == 0 && P.dword_ok) return R.d;
-    return (uint32_t)R.v[0] | ((uint32_t)R.v[1] << 8) | ((uint32_t)R.v[2] << 16) | ((uint32_t)R.v[3] << 24);
 }
 
 /* One wavefront builds a strip of kPyrStripW columns x TH rows of one level:
@@ -232,43 +242,40 @@ __device__ __forceinline__ uint32_t pyr_pixels(const PyrLane& P, const PyrRow<MO
    input), takes its neighbours' pixels across lanes, runs the horizontal
    7-tap pass and keeps the last 7 results in registers, from which the
    vertical pass emits the blurred row 3 above.  The loads of a row are issued
-   two rows ahead.  Strips cover the whole pitch: row padding is written as
-   the input's bytes at level 0 and as 0 elsewhere, in both the raw and the
-   blurred level (orbd.cu:514-535, 1273-1338; blur out of place). */
+   two rows ahead; stores outside the strip's rows or lanes go to kOob.
+   Strips cover the whole pitch: row padding is written as the input's bytes
+   at level 0 and as 0 elsewhere, in both the raw and the blurred level
+   (orbd.cu:514-535, 1273-1338; blur out of place). */
 template <int MODE, int TH>
 __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int ys, int h, int p,
-                                         ORB_GLOBAL uint8_t* raw, ORB_GLOBAL uint8_t* blur, int tyv, bool writer)
+                                         __amdgpu_buffer_rsrc_t raw, __amdgpu_buffer_rsrc_t blur, int tyv,
+                                         uint32_t cxw)
 {
     const int lane = threadIdx.x & 63;
     const int addr_up = ((lane + 63) & 63) << 2, addr_dn = ((lane + 1) & 63) << 2;
     /* rows walked: the strip's rows plus 3 above and below, in steps of 7 */
     const int nwalk = min(TH, h - ys) + 6;
+    const int row_end = min(ys + TH, h);
     auto row_ty = [&](int i) { return __builtin_amdgcn_readlane(tyv, min(i, nwalk - 1)); };
     auto row_y = [&](int i) { return reflect101(ys - 3 + min(i, nwalk - 1), h); };
     PyrRow<MODE> s0, s1;
     pyr_load<MODE>(P, row_y(0), row_ty(0), s0);
     pyr_load<MODE>(P, row_y(1), row_ty(1), s1);
     float4 ring[7];
-    uint32_t padring[7];
+    uint32_t outring[7];
     for (int i0 = 0; i0 < nwalk; i0 += 7) {
 #pragma unroll
         for (int j = 0; j < 7; j++) {
             const int i = i0 + j;
             PyrRow<MODE> sn;
             pyr_load<MODE>(P, row_y(i + 2), row_ty(i + 2), sn);
-            const uint32_t hin = pyr_pixels<MODE>(P, s0);
+            uint32_t hin, rawout;
+            pyr_pixels<MODE>(P, s0, hin, rawout);
             const uint32_t lf = lane_get(hin, addr_up), rt = lane_get(hin, addr_dn);
             const int rr = ys - 3 + i;
-            /* raw output; padding columns keep the input's bytes at level 0, 0 elsewhere */
-            uint32_t padmask = 0;
-            if (P.any_pad) {
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (P.cx + k >= P.w) padmask |= 0xffu << (8 * k);
-            }
-            padring[j] = s0.pad;
-            if (rr >= ys && rr < ys + TH && rr < h && writer)
-                *(ORB_GLOBAL uint32_t*)(raw + (long long)rr * p + (uint32_t)P.cx) = (hin & ~padmask) | s0.pad;
+            outring[j] = rawout;
+            const uint32_t roff = (rr >= ys && rr < row_end) ? (uint32_t)(rr * p) : kOob;
+            __builtin_amdgcn_raw_buffer_store_b32(rawout, raw, roff + cxw, 0, 0);
             /* horizontal pass: bytes q1..q10 = columns cx-3 .. cx+6 */
             int q[12];
 #pragma unroll
@@ -290,26 +297,26 @@ __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int 
             ring[j] = hv;
             /* vertical pass for row rr - 3 from ring rows rr-6 .. rr */
             const int o = rr - 3;
-            if (o >= ys && o < ys + TH && o < h && writer) {
-                const float* c0 = (const float*)&ring[(j + 1) % 7];
-                const float* c1 = (const float*)&ring[(j + 2) % 7];
-                const float* c2 = (const float*)&ring[(j + 3) % 7];
-                const float* c3 = (const float*)&ring[(j + 4) % 7];
-                const float* c4 = (const float*)&ring[(j + 5) % 7];
-                const float* c5 = (const float*)&ring[(j + 6) % 7];
-                const float* c6 = (const float*)&ring[j];
-                uint32_t out = 0;
+            const float* c0 = (const float*)&ring[(j + 1) % 7];
+            const float* c1 = (const float*)&ring[(j + 2) % 7];
+            const float* c2 = (const float*)&ring[(j + 3) % 7];
+            const float* c3 = (const float*)&ring[(j + 4) % 7];
+            const float* c4 = (const float*)&ring[(j + 5) % 7];
+            const float* c5 = (const float*)&ring[(j + 6) % 7];
+            const float* c6 = (const float*)&ring[j];
+            uint32_t out = 0;
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    float a = __fmul_rn(T.k0, c3[k]);
-                    a = __fmaf_rn(T.k1, __fadd_rn(c2[k], c4[k]), a);
-                    a = __fmaf_rn(T.k2, __fadd_rn(c1[k], c5[k]), a);
-                    a = __fmaf_rn(T.k3, __fadd_rn(c0[k], c6[k]), a);
-                    out |= (uint32_t)(uint8_t)__float2int_rn(a) << (8 * k);
-                }
-                *(ORB_GLOBAL uint32_t*)(blur + (long long)o * p + (uint32_t)P.cx) =
-                    (out & ~padmask) | padring[(j + 4) % 7];
+            for (int k = 0; k < 4; k++) {
+                float a = __fmul_rn(T.k0, c3[k]);
+                a = __fmaf_rn(T.k1, __fadd_rn(c2[k], c4[k]), a);
+                a = __fmaf_rn(T.k2, __fadd_rn(c1[k], c5[k]), a);
+                a = __fmaf_rn(T.k3, __fadd_rn(c0[k], c6[k]), a);
+                out |= (uint32_t)(uint8_t)__float2int_rn(a) << (8 * k);
             }
+            /* padding columns of the blurred row keep the raw row's bytes */
+            out = (out & ~P.padmask) | (outring[(j + 4) % 7] & P.padmask);
+            const uint32_t boff = (o >= ys && o < row_end) ? (uint32_t)(o * p) : kOob;
+            __builtin_amdgcn_raw_buffer_store_b32(out, blur, boff + cxw, 0, 0);
             s0 = s1;
             s1 = sn;
         }
@@ -330,39 +337,54 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
     const int t = u - jb.first_block;
     const int xs = (t % jb.tiles_x) * kPyrStripW, ys = (t / jb.tiles_x) * TH;
     const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
+    const int cx = xs - 4 + 4 * lane;
     PyrLane P;
-    P.img = as_global(F.img);
     P.ip = F.ip;
-    P.cx = xs - 4 + 4 * lane;
-    P.w = w;
     P.l = l;
     P.h0 = F.h0;
-    const int mode = l == 0 ? 0 : (F.ref_mode ? 1 : 2);
+    P.src = __builtin_amdgcn_make_buffer_rsrc((void*)F.img, 0, F.h0 * F.ip, 0x00020000);
+    const bool aligned = (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
+    const int mode = l == 0 ? (aligned ? 0 : 1) : (F.ref_mode ? 1 : 2);
     const int* tab = T.tabs + F.tab_off[l];
+    P.padmask = 0;
+    P.ipmask = 0;
+    int xr[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const int xr = reflect101(P.cx + k, w);   /* reflect-101 column feeds the horizontal taps */
+        const int x = cx + k;
+        xr[k] = reflect101(x, w);   /* the reflect-101 column feeds the horizontal taps */
+        if (x >= w) P.padmask |= 0xffu << (8 * k);
+        if (l == 0 && x >= w && x < F.ip) P.ipmask |= 0xffu << (8 * k);
         if (mode == 2) {
-            const int v = tab[xr];
+            const int v = tab[xr[k]];
             P.xa[k] = (uint32_t)(v >> 8);
             P.fx[k] = v & 255;
             P.xb[k] = (uint32_t)min((v >> 8) + 1, F.w0 - 1);
         } else {
-            P.xa[k] = (uint32_t)(xr << (mode == 1 ? l : 0));
-            P.xb[k] = P.xa[k];
+            P.xa[k] = (uint32_t)(xr[k] << (mode == 1 ? l : 0));
+            /* MODE 1 padding bytes (level 0 only): the input byte at the column itself */
+            P.xb[k] = (l == 0 && x >= w && x < F.ip) ? (uint32_t)x : kOob;
             P.fx[k] = 0;
         }
     }
-    P.dword_ok = mode == 0 && P.cx >= 0 && P.cx + 3 < w && (P.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
-    P.any_pad = P.cx + 3 >= w;
-    const bool writer = lane >= 1 && lane <= 62 && P.cx < p;
+    if (mode == 0) {
+        const int lo = min(min(xr[0], xr[1]), min(xr[2], xr[3]));
+        P.base4 = (uint32_t)(lo & ~3);
+        P.sel = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) P.sel |= (uint32_t)(xr[k] - (lo & ~3)) << (8 * k);
+        P.direct = cx >= 0 ? (uint32_t)cx : kOob;
+    }
+    /* lanes 1..62 inside the pitch write; the others store to kOob */
+    const uint32_t cxw = (lane >= 1 && lane <= 62 && cx < p) ? (uint32_t)cx : kOob;
     /* lane i holds the bilinear row entry of walked row i */
     const int tyv = mode == 2 && lane < TH + 6 ? tab[w + reflect101(ys - 3 + lane, h)] : 0;
-    ORB_GLOBAL uint8_t* raw = as_global(F.raw + F.loff[l]);
-    ORB_GLOBAL uint8_t* blur = as_global(F.blur + F.loff[l]);
-    if (mode == 0) pyr_walk<0, TH>(P, T, ys, h, p, raw, blur, tyv, writer);
-    else if (mode == 1) pyr_walk<1, TH>(P, T, ys, h, p, raw, blur, tyv, writer);
-    else pyr_walk<2, TH>(P, T, ys, h, p, raw, blur, tyv, writer);
+    const int lbytes = h * p;
+    __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc((void*)(F.raw + F.loff[l]), 0, lbytes, 0x00020000);
+    __amdgpu_buffer_rsrc_t blur = __builtin_amdgcn_make_buffer_rsrc((void*)(F.blur + F.loff[l]), 0, lbytes, 0x00020000);
+    if (mode == 0) pyr_walk<0, TH>(P, T, ys, h, p, raw, blur, tyv, cxw);
+    else if (mode == 1) pyr_walk<1, TH>(P, T, ys, h, p, raw, blur, tyv, cxw);
+    else pyr_walk<2, TH>(P, T, ys, h, p, raw, blur, tyv, cxw);
 }
 
 /* -------------------------------------------------------------- detection */
@@ -490,13 +512,22 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
     const int tid = threadIdx.x;
     if (tid < 2) s_n[tid] = 0;
-    /* stage kDetSH rows x kDetSW bytes as dwords (every row start is dword aligned) */
+    /* stage kDetSH rows x kDetSW bytes as dwords (every row start is dword
+       aligned): all loads first, then the LDS writes */
     {
         const ORB_GLOBAL uint8_t* base = as_global(F.raw) + F.loff[l] + (long long)(y0 - 4) * p + (x0 - 4);
-        constexpr int dw_per_row = kDetSW / 4;
-        for (int i = tid; i < kDetSH * dw_per_row; i += 256) {
+        constexpr int dw_per_row = kDetSW / 4, ndw = kDetSH * dw_per_row, per = (ndw + 255) / 256;
+        uint32_t v[per];
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            const int i = tid + 256 * k;
             const int r = i / dw_per_row, c = i - r * dw_per_row;
-            *(uint32_t*)&s_tile[r * kDetSW + 4 * c] = *(const ORB_GLOBAL uint32_t*)(base + (long long)r * p + 4 * c);
+            if (i < ndw) v[k] = *(const ORB_GLOBAL uint32_t*)(base + (long long)r * p + 4 * c);
+        }
+#pragma unroll
+        for (int k = 0; k < per; k++) {
+            const int i = tid + 256 * k;
+            if (i < ndw) *(uint32_t*)&s_tile[4 * i] = v[k];
         }
     }
     __syncthreads();
@@ -507,8 +538,8 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
         const int ix = x0 + lx;
         const bool col_ok = ix >= 3 && ix < w - 3;
 #pragma unroll
-        for (int rr = 0; rr < kDetTH / 4; rr++) {
-            const int ly = (tid >> 6) + 4 * rr;
+        for (int rr = 0; rr < kDetTH * kDetTW / 256; rr++) {
+            const int ly = tid / kDetTW + (256 / kDetTW) * rr;
             const int iy = y0 + ly;
             const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx + 4];
             const int v = c[0];

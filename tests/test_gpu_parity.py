@@ -115,6 +115,7 @@ PYR_CASES = [
     (333, 257, 8, 1.2),
     (1280, 720, 8, 1.25),
     (600, 400, 5, 2.0),
+    (1920, 1080, 5, 2.0),
     (1024, 200, 8, 1.2),
 ]
 
@@ -138,7 +139,8 @@ def test_pyramid_bytes_every_strip_variant(torch, ctx, oracle, w, h, nl, scale):
             blur = ctx.debug_pyramid(0, 1).cpu().numpy()
             for name, got, want in (("raw", raw, want_raw), ("blur", blur, want_blur)):
                 bad = np.nonzero(got != want)[0]
-                assert len(bad) == 0, f"variant {v} {name}: {len(bad)} bytes differ, first at {bad[:5]}"
+                where = [(int(np.searchsorted(np.array(g.off[:g.nlevels]), i, "right") - 1), int(i)) for i in bad[:5]]
+                assert len(bad) == 0, f"variant {v} {name}: {len(bad)} bytes differ, first (level, offset) {where}"
     finally:
         ctx.tune_set(orb_amd.TUNE_PYRAMID, -1)
     gpu_detect(torch, ctx, img, w, h, 500)
