@@ -26,7 +26,7 @@
 namespace orbk
 {
 
-constexpr int kGranule = 64;          /* database entries per chunk granule */
+constexpr int kGranule = 64;          /* database entries per chunk granule (popcount: 16 per wavefront) */
 constexpr int kMinChunkGranules = 2;  /* at least 128 entries per chunk: bounds the merge work */
 constexpr int kPopQ = 64;             /* popcount: queries per wavefront */
 constexpr int kMfmaQ = 256;           /* MFMA: queries per workgroup */
@@ -54,17 +54,20 @@ __host__ __device__ inline Plan make_plan(int nq, int ndb, int qtile, int slots)
 
 /* ----------------------------------------------------------- popcount */
 
+/* A workgroup unit is 64 queries x one chunk; its 4 wavefronts scan a
+   quarter of the chunk each and merge through LDS, so small problems still
+   spread over many wavefronts without multiplying the merge work. */
 __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q, int nq_cap,
                                                    const int* __restrict__ d_nq, const uint4* __restrict__ db,
                                                    int ndb_cap, const int* __restrict__ d_ndb, int slots,
                                                    uint2* __restrict__ partial)
 {
+    __shared__ uint2 s_k[3][64];
     const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
     const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
     const Plan P = make_plan(nq, ndb, kPopQ, slots);
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    for (int u = wave; u < P.qunits * P.chunks; u += slots) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int u = blockIdx.x; u < P.qunits * P.chunks; u += gridDim.x) {
         const int qu = u % P.qunits, c = u / P.qunits;
         const int qi = qu * kPopQ + lane;
         uint4 qa = make_uint4(0, 0, 0, 0), qb = qa;
@@ -73,7 +76,8 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
             qa = qp[0];
             qb = qp[1];
         }
-        const int j0 = c * P.chunk, j1 = min(ndb, j0 + P.chunk);
+        const int sub = P.chunk / 4;
+        const int j0 = min(ndb, c * P.chunk + wid * sub), j1 = min(ndb, j0 + sub);
         uint32_t k1 = kNoKey, k2 = kNoKey;
         auto pair = [&](uint4 a, uint4 b, int j) {
             uint32_t d = __popc(qa.x ^ a.x);
@@ -101,7 +105,17 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
             const int ju = __builtin_amdgcn_readfirstlane(j);
             pair(db[2 * ju], db[2 * ju + 1], ju);
         }
-        if (qi < nq) partial[(long long)c * nq + qi] = make_uint2(k1, k2);
+        if (wid > 0) s_k[wid - 1][lane] = make_uint2(k1, k2);
+        __syncthreads();
+        if (wid == 0) {
+#pragma unroll
+            for (int w = 0; w < 3; w++) {
+                top2_push(k1, k2, s_k[w][lane].x);
+                top2_push(k1, k2, s_k[w][lane].y);
+            }
+            if (qi < nq) partial[(long long)c * nq + qi] = make_uint2(k1, k2);
+        }
+        __syncthreads();
     }
 }
 
@@ -248,7 +262,7 @@ __global__ __launch_bounds__(256) void k_match_merge(int nq_cap, const int* __re
 
 /* ---------------------------------------------------------- launchers */
 
-constexpr int kPopBlocks = 2048;   /* 8192 wavefront slots */
+constexpr int kPopBlocks = 2048;   /* workgroup slots of the popcount plan */
 constexpr int kMfmaBlocks = 1024;  /* workgroup slots */
 
 size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk)
@@ -256,7 +270,7 @@ size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk
     (void)ndb;
     /* chunks * nq <= (slots / qunits + 1) * qunits * qtile <= (slots + qunits) * qtile */
     const int qtile = method == 1 ? kMfmaQ : kPopQ;
-    const int slots = method == 1 ? kMfmaBlocks : kPopBlocks * 4;
+    const int slots = method == 1 ? kMfmaBlocks : kPopBlocks;
     const long long qunits = (nq + qtile - 1) / qtile;
     *nchunks = 0;
     *chunk = 0;
@@ -278,12 +292,11 @@ hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const Stag
 {
     (void)hipGetLastError(); /* report only errors of these launches */
     if (m.nq <= 0) return hipSuccess;
-    const int slots = kPopBlocks * 4;
+    const int slots = kPopBlocks;
     /* enough workgroups for the units the capacities allow, at most kPopBlocks */
     const Plan P = make_plan(m.nq, m.ndb, kPopQ, slots);
     const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
-    int blocks = (int)((units + 3) / 4);
-    if (blocks > kPopBlocks) blocks = kPopBlocks;
+    int blocks = (int)(units < kPopBlocks ? units : kPopBlocks);
     if (blocks < 1) blocks = 1;
     ev_start(ev, kStMatch, s);
     hipLaunchKernelGGL(k_match_pop, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db, m.ndb,
