@@ -942,17 +942,35 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
         const ORB_GLOBAL uint8_t* ra = as_global(F.raw) + co - 15 - 15LL * p;
         const int ro = (int)((uintptr_t)ra & 3);
         const ORB_GLOBAL uint8_t* rb = ra - ro;
-        for (int i = lane; i < 31 * (kRawRow / 4); i += 64) {
-            const int r = i / (kRawRow / 4), d = i - r * (kRawRow / 4);
-            *(uint32_t*)&s_raw[wid][r * kRawRow + 4 * d] = *(const ORB_GLOBAL uint32_t*)(rb + (long long)r * p + 4 * d);
-        }
         const ORB_GLOBAL uint8_t* ba = as_global(F.blur) + co - 18 - 18LL * p;
         const int bo = (int)((uintptr_t)ba & 3);
         const ORB_GLOBAL uint8_t* bb = ba - bo;
-        if (compute_desc) {
-            for (int i = lane; i < 37 * (kBlurRow / 4); i += 64) {
+        {
+            /* every load of both patches in flight before the LDS writes */
+            constexpr int nraw = 31 * (kRawRow / 4), nblur = 37 * (kBlurRow / 4);
+            constexpr int praw = (nraw + 63) / 64, pblur = (nblur + 63) / 64;
+            uint32_t vr[praw], vb[pblur];
+#pragma unroll
+            for (int k = 0; k < praw; k++) {
+                const int i = lane + 64 * k;
+                const int r = i / (kRawRow / 4), d = i - r * (kRawRow / 4);
+                if (i < nraw) vr[k] = *(const ORB_GLOBAL uint32_t*)(rb + (long long)r * p + 4 * d);
+            }
+#pragma unroll
+            for (int k = 0; k < pblur; k++) {
+                const int i = lane + 64 * k;
                 const int r = i / (kBlurRow / 4), d = i - r * (kBlurRow / 4);
-                *(uint32_t*)&s_blur[wid][r * kBlurRow + 4 * d] = *(const ORB_GLOBAL uint32_t*)(bb + (long long)r * p + 4 * d);
+                if (compute_desc && i < nblur) vb[k] = *(const ORB_GLOBAL uint32_t*)(bb + (long long)r * p + 4 * d);
+            }
+#pragma unroll
+            for (int k = 0; k < praw; k++) {
+                const int i = lane + 64 * k;
+                if (i < nraw) *(uint32_t*)&s_raw[wid][4 * i] = vr[k];
+            }
+#pragma unroll
+            for (int k = 0; k < pblur; k++) {
+                const int i = lane + 64 * k;
+                if (compute_desc && i < nblur) *(uint32_t*)&s_blur[wid][4 * i] = vb[k];
             }
         }
         wave_sync_lds();
