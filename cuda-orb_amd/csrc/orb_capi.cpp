@@ -120,7 +120,7 @@ struct FrameSet {
     FrameDev* d_frames = nullptr;
     Job* d_pyr[kPyrVariants] = {};
     Job* d_det = nullptr;
-    int pyr_njobs = 0, pyr_units[kPyrVariants] = {}, det_njobs = 0, det_blocks = 0;
+    int pyr_njobs[kPyrVariants] = {}, pyr_units[kPyrVariants] = {}, det_njobs = 0, det_blocks = 0;
     /* autotuner: measured ms per pyramid variant and the choice (-1: not tuned) */
     float pyr_ms[kPyrVariants] = {};
     int pyr_choice = -1;
@@ -300,17 +300,28 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
             F.tab_off[l] = tab_ints;
             if (!g.ref && l > 0) tab_ints += 2 * (g.w[l] + g.h[l]);
             if (g.radius[l] > rmax) rmax = g.radius[l];
-            /* pyramid strips cover the whole pitch (padding written too) */
-            const int ptx = (g.p[l] + kPyrStripW - 1) / kPyrStripW;
-            for (int v = 0; v < kPyrVariants; v++) {
-                pyr_jobs[v].push_back(Job{f, l, ptx, pyr_units[v]});
-                pyr_units[v] += ptx * ((g.h[l] + kPyrTHs[v] - 1) / kPyrTHs[v]);
-            }
+
             if (g.w[l] > 6 && g.h[l] > 6) {
                 /* tiles cover [0, w-3) x [0, h-3); pixels outside [3, w-3) x [3, h-3) are masked */
                 const int dtx = (g.w[l] - 3 + kDetTW - 1) / kDetTW, dty = (g.h[l] - 3 + kDetTH - 1) / kDetTH;
-                det_jobs.push_back(Job{f, l, dtx, det_blocks});
+                det_jobs.push_back(Job{f, l, dtx, det_blocks, 0});
                 det_blocks += dtx * dty;
+            }
+        }
+        /* pyramid strips cover the whole pitch (padding written too).  Units
+           go in kPyrBands horizontal bands of the frame, each band holding the
+           strips of every level over it, so the contiguous unit range an XCD
+           runs reads one band of the input into its L2 */
+        for (int v = 0; v < kPyrVariants; v++) {
+            for (int band = 0; band < kPyrBands; band++) {
+                for (int l = 0; l < g.L; l++) {
+                    const int ptx = (g.p[l] + kPyrStripW - 1) / kPyrStripW;
+                    const int nch = (g.h[l] + kPyrTHs[v] - 1) / kPyrTHs[v];
+                    const int c0 = band * nch / kPyrBands, c1 = (band + 1) * nch / kPyrBands;
+                    if (c1 <= c0) continue;
+                    pyr_jobs[v].push_back(Job{f, l, ptx, pyr_units[v], c0});
+                    pyr_units[v] += ptx * (c1 - c0);
+                }
             }
         }
         /* flat-pyramid guard: covers the deepest stencil / patch overhang
@@ -399,7 +410,7 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     fs.border = border;
     fs.cap = P.max_pts;
     fs.nframes = nf;
-    fs.pyr_njobs = (int)pyr_jobs[0].size();
+    for (int v = 0; v < kPyrVariants; v++) fs.pyr_njobs[v] = (int)pyr_jobs[v].size();
     for (int v = 0; v < kPyrVariants; v++) fs.pyr_units[v] = pyr_units[v];
     fs.det_njobs = (int)det_jobs.size();
     fs.det_blocks = det_blocks;
@@ -456,6 +467,7 @@ void set_pyramid_variant(const FrameSet& fs, DetectLaunch& d, int v)
 {
     d.pyr_variant = v;
     d.pyr_jobs = fs.d_pyr[v];
+    d.pyr_njobs = fs.pyr_njobs[v];
     d.pyr_units = fs.pyr_units[v];
     d.pyr_blocks = (fs.pyr_units[v] + 3) / 4;
 }
@@ -532,7 +544,7 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     memset(&d, 0, sizeof(d));
     d.frames = fs.d_frames;
     d.nframes = nf;
-    d.pyr_njobs = fs.pyr_njobs;
+
     d.det_jobs = fs.d_det;
     d.det_njobs = fs.det_njobs;
     d.det_blocks = fs.det_blocks;

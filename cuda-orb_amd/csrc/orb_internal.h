@@ -35,6 +35,7 @@ constexpr uint32_t kEpochLimit = 1u << 30;
    rows of one level; 4 units per 256-thread workgroup.  The strip heights
    are the autotuner's variants (TH + 6 walked rows fit one wavefront). */
 constexpr int kPyrStripW = 248;
+constexpr int kPyrBands = 8;         /* horizontal bands per frame in the unit order */
 constexpr int kPyrVariants = 5;
 constexpr int kPyrTHs[kPyrVariants] = {15, 22, 29, 36, 50};
 /* detection tile: 128 x 32 pixels, 256 threads */
@@ -72,9 +73,10 @@ struct FrameDev {
     int rmax;                  /* largest NMS radius of the frame's levels */
 };
 
-/* A contiguous range of workgroups covering one (frame, level) region. */
+/* A contiguous range of work units covering one (frame, level) region:
+   tiles_x units per row of tiles, tile rows from row0. */
 struct Job {
-    int frame, level, tiles_x, first_block;
+    int frame, level, tiles_x, first_block, row0;
 };
 
 /* NMS survivor: pos = level << 28 | y << 14 | x (level-0 coordinates), score = float bits */

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
     const FrameDev& F = frames[jb.frame];
     const int l = jb.level;
     const int t = u - jb.first_block;
-    const int xs = (t % jb.tiles_x) * kPyrStripW, ys = (t / jb.tiles_x) * TH;
+    const int xs = (t % jb.tiles_x) * kPyrStripW, ys = (jb.row0 + t / jb.tiles_x) * TH;
     const int w = F.lw[l], h = F.lh[l], p = F.lp[l];
     const int cx = xs - 4 + 4 * lane;
     PyrLane P;
@@ -733,9 +733,11 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
                                              int* __restrict__ blk_count)
 {
     __shared__ int s_tmp[8];
-    const int f = block_frame[blockIdx.x];
+    /* consecutive rows on one XCD: their shared neighbour rows hit its L2 */
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int f = block_frame[bid];
     const FrameDev& F = frames[f];
-    const int b = blockIdx.x - F.nms_first;
+    const int b = bid - F.nms_first;
     const int row = b / F.nms_segs, seg = b - row * F.nms_segs;
     const int y = F.border + row;
     const int xs = F.nms_x0 + seg * kNmsSeg + 4 * threadIdx.x;
@@ -767,7 +769,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
     }
     int total;
     int pos = block_scan<256>(__popc(surv), s_tmp, total);
-    ORB_GLOBAL Candidate* out = as_global(cand) + (long long)blockIdx.x * kNmsCap;
+    ORB_GLOBAL Candidate* out = as_global(cand) + (long long)bid * kNmsCap;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if (surv & (1u << k)) {
@@ -778,7 +780,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
             out[pos++] = c;
         }
     }
-    if (threadIdx.x == 0) blk_count[blockIdx.x] = total;
+    if (threadIdx.x == 0) blk_count[bid] = total;
 }
 
 /* ---------------------------------------------------------- select + emit */
@@ -921,8 +923,10 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
     __shared__ uint8_t s_code[kDescPerBlock][128];
     __shared__ __align__(16) uint8_t s_raw[kDescPerBlock][31 * kRawRow];
     __shared__ __align__(16) uint8_t s_blur[kDescPerBlock][37 * kBlurRow];
-    const int f = blockIdx.x / blocks_per_frame;
-    const int lb = blockIdx.x - f * blocks_per_frame;
+    /* consecutive keypoints (raster neighbours, overlapping patches) on one XCD */
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int f = bid / blocks_per_frame;
+    const int lb = bid - f * blocks_per_frame;
     const FrameDev& F = frames[f];
     for (int i = threadIdx.x; i < 512; i += 256) s_pat[i] = T.pattern[i];
     if (threadIdx.x < 17) s_umax[threadIdx.x] = T.umax[threadIdx.x];
