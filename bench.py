@@ -63,6 +63,32 @@ def stage_bytes(stage, frames, npts):
     return tot
 
 
+COUNTERS_FILE = ROOT / "profiles" / "orb_1080p_pair_hbm_counters.md"
+
+
+def stage_kernel(stage, strip_height):
+    """rocprofv3 kernel name of a stage's launch."""
+    return {"pyramid": f"void k_pyramid<{strip_height}>", "detect": "k_detect", "nms": "k_nms", "select": "k_select",
+            "describe": "k_describe", "match": "k_match_pop", "match_merge": "k_match_merge"}.get(stage)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (rows: | kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | bytes |), or None."""
+    try:
+        text = COUNTERS_FILE.read_text()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == kernel:
+            try:
+                return int(float(cells[4]))
+            except ValueError:
+                return None
+    return None
+
+
 def dense_match(ctx, torch, dev, reps=3):
     """65536 queries against 262144 random 256-bit descriptors (queries = database rows with
     Binomial(256, 0.05) bit flips): INT8-MFMA and popcount matchers, timed with events on the
@@ -367,6 +393,20 @@ def main():
     ms_step = (t1 - t0) * 1e3 / args.steps
     dom_ms = dom[0] / max(dom[1], 1)
 
+    # the same step recorded once into a HIP graph and replayed (kernel-only protocol)
+    ctx.capture_begin()
+    step()
+    graph = ctx.capture_end()
+    graph.launch()
+    torch.cuda.synchronize()
+    g0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    torch.cuda.synchronize()
+    g1 = time.perf_counter()
+    graph.close()
+    graph_ms = (g1 - g0) * 1e3 / args.steps
+
     if dominant in ("match", "match_merge"):
         pairs = n1 * n2
         achieved = 2 * 256 * pairs / (dom_ms * 1e-3) / 1e12
@@ -379,8 +419,10 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
                 "bytes_per_launch": nbytes}
-    roof["kernel"] = dominant
+    strip_h = orb_amd.PYRAMID_STRIP_HEIGHTS[ctx.tune_get(orb_amd.TUNE_PYRAMID)[0]]
+    roof["kernel"] = stage_kernel(dominant, strip_h)
     roof["avg_launch_ms"] = round(dom_ms, 5)
+    roof["traffic"] = measured_traffic(roof["kernel"])
 
     out = {
         "metric": METRIC,
@@ -398,6 +440,7 @@ def main():
         "config": {"workload": "1920x1080 pair: detect+describe both (8 levels @1.2, FAST 20, Harris, "
                                "<=10000 kp) + exact popcount BF match",
                    "frames_per_step": 2, "keypoints": [n1, n2], "match_pairs": n1 * n2},
+        "graph_replay": {"ms_per_step": round(graph_ms, 4), "frames_per_s": round(2e3 / graph_ms, 2)},
         "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "match_gpairs_per_s": round(n1 * n2 / (stage_ms["match"] + stage_ms["match_merge"]) / 1e6, 2)
         if stage_ms["match"] > 0 else None,

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -110,6 +111,8 @@ int resize_tab(int i, int S, int D)
 size_t round256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct FrameSet {
+    int pins = 0;              /* graphs that recorded launches on this set's buffers */
+    unsigned long long last_use = 0;
     std::vector<int> ws, hs;
     int noctaves = 0, border = 0, cap = 0;
     double scale = 0;
@@ -142,12 +145,24 @@ struct orb_ctx {
     orb_params prm{};
     int2* d_pattern = nullptr;
     int pyr_forced = -1;       /* pyramid variant forced by orb_tune_set, or -1 */
+    /* graph capture: calls are recorded on capture_stream instead of run */
+    bool capturing = false;
+    hipStream_t user_stream = nullptr;
+    hipStream_t capture_stream = nullptr;
     float gauss[4] = {};
     float harris_scale = 0.f;
     int umax[32] = {};
-    FrameSet fs;
+    /* frame sets by frame sizes (and, for sets a graph recorded, buffer
+       pointers); fs points at the one of the current call */
+    std::vector<FrameSet*> sets;
+    FrameSet* fs = nullptr;
+    unsigned long long use_clock = 0;
     uint32_t* d_match_partial = nullptr;
     size_t match_partial_cap = 0;
+    int partial_pins = 0;                  /* graphs that recorded the matcher workspace */
+    std::vector<uint32_t*> retired_partial;/* replaced workspaces still named by a graph */
+    std::vector<FrameSet*> captured_sets;  /* during capture: sets the recorded calls use */
+    bool captured_partial = false;
     std::string err;
     /* stage timing: events recorded around launches while prof_mask != 0 */
     unsigned prof_mask = 0;
@@ -242,23 +257,79 @@ void make_pattern(int wta_k, int32_t* xy)
     }
 }
 
-void free_frameset(FrameSet& fs)
+void free_frameset(FrameSet* fs)
 {
-    if (fs.arena) (void)hipFree(fs.arena);
-    fs = FrameSet();
+    if (fs->arena) (void)hipFree(fs->arena);
+    delete fs;
 }
 
-/* Build geometry, job tables and buffers for a list of frame sizes. */
-int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int>& hs)
+constexpr int kMaxFreeSets = 4;   /* unpinned frame sets kept for reuse */
+
+struct CallFrames {
+    int nf;
+    const uint8_t* const* imgs;
+    const int* pitches;
+    orb::OrbPoint* const* pts;
+    uint8_t* const* desc;
+    int* d_counts;
+    int compute_desc;
+};
+
+bool same_buffers(const FrameSet& fs, const CallFrames& cf)
 {
-    FrameSet& fs = c->fs;
+    for (int f = 0; f < cf.nf; f++) {
+        const FrameDev& F = fs.hframes[f];
+        if (F.img != cf.imgs[f] || F.ip != cf.pitches[f] || F.pts != cf.pts[f] ||
+            F.desc != (cf.compute_desc ? cf.desc[f] : nullptr) || F.count != cf.d_counts + f)
+            return false;
+    }
+    return true;
+}
+
+/* Build geometry, job tables and buffers for a list of frame sizes, or
+   reuse a set built before. */
+int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int>& hs, const CallFrames& cf)
+{
     const orb_params& P = c->prm;
     const int border = P.edge_threshold < 3 ? 3 : P.edge_threshold;
-    if (fs.arena && fs.ws == ws && fs.hs == hs && fs.noctaves == P.noctaves && fs.scale == P.scale_factor &&
-        fs.border == border && fs.cap == P.max_pts)
+    /* a set with these sizes; one a graph recorded only with the same buffers */
+    FrameSet* pick = nullptr;
+    for (FrameSet* s : c->sets) {
+        if (!(s->ws == ws && s->hs == hs && s->noctaves == P.noctaves && s->scale == P.scale_factor &&
+              s->border == border && s->cap == P.max_pts))
+            continue;
+        const bool same = same_buffers(*s, cf);
+        if (s->pins > 0 && !same) continue;
+        if (!pick || (same && !same_buffers(*pick, cf))) pick = s;
+    }
+    if (pick) {
+        c->fs = pick;
+        pick->last_use = ++c->use_clock;
         return ORB_OK;
+    }
+    if (c->capturing)
+        return fail(c, ORB_E_STATE, "new frame sizes or parameters during graph capture: make the same call once "
+                                    "before orb_capture_begin");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    free_frameset(fs);
+    /* keep at most kMaxFreeSets unpinned sets: drop the least recently used */
+    for (;;) {
+        int nfree = 0;
+        FrameSet* lru = nullptr;
+        for (FrameSet* s : c->sets)
+            if (s->pins == 0) {
+                nfree++;
+                if (!lru || s->last_use < lru->last_use) lru = s;
+            }
+        if (nfree < kMaxFreeSets || !lru) break;
+        c->sets.erase(std::find(c->sets.begin(), c->sets.end(), lru));
+        if (c->fs == lru) c->fs = nullptr;
+        free_frameset(lru);
+    }
+    FrameSet* nfs = new FrameSet();
+    c->sets.push_back(nfs);
+    c->fs = nfs;
+    nfs->last_use = ++c->use_clock;
+    FrameSet& fs = *nfs;
     const int nf = (int)ws.size();
     std::vector<Geom> geo(nf);
     for (int f = 0; f < nf; f++) {
@@ -423,7 +494,7 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
 /* Events for the stages of one call (null when timing is off). */
 const StageEvents* take_events(orb_ctx* c, StageEvents& ev, unsigned stages)
 {
-    const unsigned m = c->prof_mask & stages;
+    const unsigned m = c->capturing ? 0u : c->prof_mask & stages;
     if (!m) return nullptr;
     for (int s = 0; s < kNumStages; s++) {
         ev.on[s] = (m >> s) & 1u;
@@ -448,7 +519,7 @@ Tables make_tables(const orb_ctx* c)
 {
     Tables t;
     memset(&t, 0, sizeof(t));
-    t.tabs = c->fs.d_tabs;
+    t.tabs = c->fs->d_tabs;
     t.pattern = c->d_pattern;
     t.k0 = c->gauss[0];
     t.k1 = c->gauss[1];
@@ -480,7 +551,7 @@ void set_pyramid_variant(const FrameSet& fs, DetectLaunch& d, int v)
 int tune_pyramid(orb_ctx* c, DetectLaunch& d)
 {
     constexpr int kTuneReps = 3;
-    FrameSet& fs = c->fs;
+    FrameSet& fs = *c->fs;
     hipEvent_t a = nullptr, b = nullptr;
     HIP_TRY(c, hipEventCreate(&a));
     hipError_t e = hipEventCreate(&b);
@@ -519,9 +590,12 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
         if (compute_desc && !desc[f]) return fail(c, ORB_E_ARG, "frame %d: null descriptor buffer", f);
         if (pitches[f] < ws[f]) return fail(c, ORB_E_ARG, "frame %d: pitch %d < width %d", f, pitches[f], ws[f]);
     }
-    int rc = build_frameset(c, vw, vh);
+    const CallFrames cf{nf, imgs, pitches, pts, desc, d_counts, compute_desc};
+    int rc = build_frameset(c, vw, vh, cf);
     if (rc != ORB_OK) return rc;
-    FrameSet& fs = c->fs;
+    FrameSet& fs = *c->fs;
+    if (c->capturing && std::find(c->captured_sets.begin(), c->captured_sets.end(), c->fs) == c->captured_sets.end())
+        c->captured_sets.push_back(c->fs);
     bool changed = false;
     for (int f = 0; f < nf; f++) {
         FrameDev& F = fs.hframes[f];
@@ -536,6 +610,9 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
         }
     }
     if (changed) {
+        if (c->capturing)
+            return fail(c, ORB_E_STATE, "new image, point or descriptor pointers during graph capture: make the same "
+                                        "call once before orb_capture_begin");
         HIP_TRY(c, hipMemcpyAsync(fs.d_frames, fs.hframes.data(), sizeof(FrameDev) * nf, hipMemcpyHostToDevice,
                                   c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -560,6 +637,7 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     int variant = c->pyr_forced;
     if (variant < 0) {
         if (fs.pyr_choice < 0) {
+            if (c->capturing) return fail(c, ORB_E_STATE, "the autotuner has not run for these frame sizes");
             rc = tune_pyramid(c, d);
             if (rc != ORB_OK) return rc;
         }
@@ -575,9 +653,16 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
 
 int ensure_partial(orb_ctx* c, size_t words)
 {
+    if (c->capturing) c->captured_partial = true;
     if (words <= c->match_partial_cap) return ORB_OK;
+    if (c->capturing)
+        return fail(c, ORB_E_STATE, "matcher workspace grows during graph capture: make the same call once before "
+                                    "orb_capture_begin");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->d_match_partial) (void)hipFree(c->d_match_partial);
+    if (c->d_match_partial) {
+        if (c->partial_pins > 0) c->retired_partial.push_back(c->d_match_partial); /* a graph still names it */
+        else (void)hipFree(c->d_match_partial);
+    }
     c->d_match_partial = nullptr;
     c->match_partial_cap = 0;
     HIP_TRY(c, hipMalloc((void**)&c->d_match_partial, words * sizeof(uint32_t)));
@@ -625,6 +710,7 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
 int orb__ctx_stream(orb_ctx* c, void** stream)
 {
     if (!c) return ORB_E_ARG;
+    if (c->capturing) return fail(c, ORB_E_STATE, "learned-index calls cannot be recorded into a graph");
     *stream = (void*)c->stream;
     return ORB_OK;
 }
@@ -671,13 +757,16 @@ int orb_ctx_destroy(orb_ctx* c)
     if (!c) return ORB_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    free_frameset(c->fs);
+    for (FrameSet* s : c->sets) free_frameset(s);
+    c->sets.clear();
+    for (uint32_t* p : c->retired_partial) (void)hipFree(p);
     for (auto& r : c->recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->d_pattern) (void)hipFree(c->d_pattern);
+    if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     if (c->d_match_partial) (void)hipFree(c->d_match_partial);
     delete c;
     return ORB_OK;
@@ -686,7 +775,86 @@ int orb_ctx_destroy(orb_ctx* c)
 int orb_set_stream(orb_ctx* c, void* stream)
 {
     if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (c->capturing) {
+        if ((hipStream_t)stream == c->user_stream) return ORB_OK;
+        return fail(c, ORB_E_STATE, "stream change during graph capture");
+    }
     c->stream = (hipStream_t)stream;
+    return ORB_OK;
+}
+
+struct orb_graph {
+    orb_ctx* ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<FrameSet*> sets;   /* pinned: their buffers stay as recorded */
+    bool partial = false;          /* pins the matcher workspace */
+};
+
+int orb_capture_begin(orb_ctx* c)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (c->capturing) return fail(c, ORB_E_STATE, "capture already in progress");
+    if (!c->capture_stream) HIP_TRY(c, hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking));
+    HIP_TRY(c, hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal));
+    c->user_stream = c->stream;
+    c->stream = c->capture_stream;
+    c->capturing = true;
+    c->captured_sets.clear();
+    c->captured_partial = false;
+    return ORB_OK;
+}
+
+int orb_capture_end(orb_ctx* c, orb_graph** out)
+{
+    if (!c || !out) return fail(c, ORB_E_ARG, "null argument");
+    *out = nullptr;
+    if (!c->capturing) return fail(c, ORB_E_STATE, "no capture in progress");
+    c->capturing = false;
+    c->stream = c->user_stream;
+    hipGraph_t g = nullptr;
+    HIP_TRY(c, hipStreamEndCapture(c->capture_stream, &g));
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(c, e, "hipGraphInstantiate");
+    }
+    orb_graph* gr = new orb_graph();
+    gr->ctx = c;
+    gr->graph = g;
+    gr->exec = x;
+    gr->sets = c->captured_sets;
+    for (FrameSet* s : gr->sets) s->pins++;
+    gr->partial = c->captured_partial;
+    if (gr->partial) c->partial_pins++;
+    c->captured_sets.clear();
+    *out = gr;
+    return ORB_OK;
+}
+
+int orb_graph_launch(orb_graph* g)
+{
+    if (!g) return fail(nullptr, ORB_E_ARG, "null graph");
+    orb_ctx* c = g->ctx;
+    if (c->capturing) return fail(c, ORB_E_STATE, "graph launch during capture");
+    HIP_TRY(c, hipGraphLaunch(g->exec, c->stream));
+    return ORB_OK;
+}
+
+int orb_graph_destroy(orb_graph* g)
+{
+    if (!g) return ORB_OK;
+    orb_ctx* c = g->ctx;
+    (void)hipStreamSynchronize(c->stream); /* replays in flight still use the pinned buffers */
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    for (FrameSet* s : g->sets) s->pins--;
+    if (g->partial && --c->partial_pins == 0) {
+        for (uint32_t* p : c->retired_partial) (void)hipFree(p);
+        c->retired_partial.clear();
+    }
+    delete g;
     return ORB_OK;
 }
 
@@ -781,7 +949,10 @@ const char* orb_last_error(const orb_ctx* c)
 
 size_t orb_workspace_bytes(const orb_ctx* c)
 {
-    return c ? c->fs.arena_bytes + c->match_partial_cap * sizeof(uint32_t) : 0;
+    if (!c) return 0;
+    size_t b = c->match_partial_cap * sizeof(uint32_t);
+    for (const FrameSet* s : c->sets) b += s->arena_bytes;
+    return b;
 }
 
 int orb_profile_enable(orb_ctx* c, unsigned stage_mask)
@@ -827,9 +998,9 @@ int orb_tune_get(orb_ctx* c, int stage, int* variant, int* nvariants, float* ms)
     if (!c || !variant || !nvariants) return fail(c, ORB_E_ARG, "null argument");
     if (stage != ORB_TUNE_PYRAMID) return fail(c, ORB_E_ARG, "unknown tuning stage %d", stage);
     *nvariants = kPyrVariants;
-    *variant = c->pyr_forced >= 0 ? c->pyr_forced : c->fs.pyr_choice;
+    *variant = c->pyr_forced >= 0 ? c->pyr_forced : (c->fs ? c->fs->pyr_choice : -1);
     if (ms)
-        for (int v = 0; v < kPyrVariants; v++) ms[v] = c->fs.pyr_ms[v];
+        for (int v = 0; v < kPyrVariants; v++) ms[v] = c->fs ? c->fs->pyr_ms[v] : 0.f;
     return ORB_OK;
 }
 
@@ -850,9 +1021,10 @@ int orb_debug_atan2(orb_ctx* c, const float* yx, int n, float* out)
 int orb_debug_pyramid(orb_ctx* c, int frame, int which, uint8_t* d_out, size_t cap, size_t* nbytes)
 {
     if (!c || !nbytes) return fail(c, ORB_E_ARG, "null argument");
-    if (frame < 0 || frame >= c->fs.nframes) return fail(c, ORB_E_STATE, "no frame %d in the last call", frame);
+    if (!c->fs || frame < 0 || frame >= c->fs->nframes)
+        return fail(c, ORB_E_STATE, "no frame %d in the last call", frame);
     if (which != 0 && which != 1) return fail(c, ORB_E_ARG, "which must be 0 (raw) or 1 (blurred)");
-    const FrameDev& F = c->fs.hframes[frame];
+    const FrameDev& F = c->fs->hframes[frame];
     *nbytes = (size_t)F.total;
     if (!d_out) return ORB_OK;
     if (cap < (size_t)F.total) return fail(c, ORB_E_CAPACITY, "buffer of %zu bytes < %lld", cap, F.total);

@@ -32,7 +32,8 @@ EXPORTS = [
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
     "orb_match_top2", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
-    "orb_tune_set", "orb_tune_get", "orb_debug_pyramid",
+    "orb_tune_set", "orb_tune_get", "orb_debug_pyramid", "orb_capture_begin", "orb_capture_end",
+    "orb_graph_launch", "orb_graph_destroy",
     "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
     "orb_lindex_logits", "orb_lindex_grads", "orb_lindex_num_params", "orb_lindex_get_weights",
     "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_save", "orb_lindex_load", "orb_lindex_set_probe",
@@ -99,6 +100,10 @@ def lib():
         L.orb_tune_set.argtypes = [P, I, I]
         L.orb_tune_get.argtypes = [P, I, V, V, V]
         L.orb_debug_pyramid.argtypes = [P, I, I, V, ctypes.c_size_t, V]
+        L.orb_capture_begin.argtypes = [P]
+        L.orb_capture_end.argtypes = [P, ctypes.POINTER(P)]
+        L.orb_graph_launch.argtypes = [P]
+        L.orb_graph_destroy.argtypes = [P]
         L.orb_lindex_create.argtypes = [P, ctypes.POINTER(LindexParams), ctypes.POINTER(P)]
         L.orb_lindex_destroy.argtypes = [P]
         L.orb_lindex_build.argtypes = [P, V, I, I]
@@ -259,6 +264,17 @@ class Context:
         self._check(lib().orb_profile_read(self.h, _np_ptr(ms), _np_ptr(cnt)), "orb_profile_read")
         return {STAGES[i]: (float(ms[i]), int(cnt[i])) for i in range(NUM_STAGES)}
 
+    def capture_begin(self):
+        """Record the following calls into a HIP graph (orb_capture_begin)."""
+        self._bind_stream()
+        self._check(lib().orb_capture_begin(self.h), "orb_capture_begin")
+
+    def capture_end(self):
+        """End the recording; returns a Graph whose launch() replays it on the context's stream."""
+        g = ctypes.c_void_p()
+        self._check(lib().orb_capture_end(self.h, ctypes.byref(g)), "orb_capture_end")
+        return Graph(self, g)
+
     def tune_set(self, stage, variant):
         """Force tile-shape variant `variant` of a tuned stage (-1: autotune)."""
         self._check(lib().orb_tune_set(self.h, stage, variant), "orb_tune_set")
@@ -298,6 +314,29 @@ class Context:
         out = self._torch.empty(yx.shape[0], dtype=self._torch.float32, device=yx.device)
         self._check(lib().orb_debug_atan2(self.h, _dptr(yx), yx.shape[0], _dptr(out)), "orb_debug_atan2")
         return out
+
+
+class Graph:
+    """A recorded sequence of context calls (orb_graph)."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    def launch(self):
+        self.ctx._bind_stream()
+        self.ctx._check(lib().orb_graph_launch(self.h), "orb_graph_launch")
+
+    def close(self):
+        if self.h:
+            lib().orb_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LearnedIndex:

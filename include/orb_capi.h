@@ -210,6 +210,20 @@ int orb_lindex_load(orb_lindex* lx, const char* path);
 int orb_profile_enable(orb_ctx* ctx, unsigned stage_mask);
 int orb_profile_read(orb_ctx* ctx, double* ms_sum, int* counts);
 
+/* HIP graph capture (no counterpart in the reference, which synchronises
+   after every launch, orbd.cu:1598-1791).  Between orb_capture_begin and
+   orb_capture_end the context's calls are recorded into a graph instead of
+   running; orb_graph_launch replays the recording on the context's stream.
+   Each recorded call must have been made once before with the same frame
+   sizes, parameters and buffer pointers (so no allocation, upload or tuning
+   is needed), else it returns ORB_E_STATE.  Replays read and write the
+   buffers the recorded calls named. */
+typedef struct orb_graph orb_graph;
+int orb_capture_begin(orb_ctx* ctx);
+int orb_capture_end(orb_ctx* ctx, orb_graph** graph);
+int orb_graph_launch(orb_graph* graph);
+int orb_graph_destroy(orb_graph* graph);
+
 /* Autotuner (no counterpart in the reference, whose launch shapes are fixed,
    orbd.cu:1564-1793).  Tile shapes of a stage are chosen per list of frame
    sizes on its first call: every variant is timed on the context's stream

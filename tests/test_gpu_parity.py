@@ -181,6 +181,68 @@ def test_batch_matches_per_frame_oracle(torch, ctx, oracle):
             assert np.array_equal(desc[i][:n].cpu().numpy(), want_d)
 
 
+def test_graph_replay_matches_eager_step(torch, ctx, oracle):
+    """A recorded detect(2 frames) + match step replays to the eager results, several times, with
+    eager work of other sizes and host synchronisations between the replays; recording a call that
+    was not made before fails cleanly."""
+    import orb_amd
+    cap = 2000
+    ctx.init(noctaves=8, max_pts=cap, scale_factor=1.2)
+    a = orb_amd.synth_image(31, 640, 480, 40)
+    b, _ = orb_amd.synth_warp(31, a, 640, 480)
+    imgs = [torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()]
+    pts = ctx.alloc_points(cap, frames=2)
+    desc = ctx.alloc_desc(cap, frames=2)
+    cnt = torch.zeros(2, dtype=torch.int32, device="cuda")
+
+    def step():
+        ctx.detect_and_compute_batch(imgs, [640, 640], [480, 480], [pts[0], pts[1]], [desc[0], desc[1]], cnt)
+        ctx.match_dcount(pts[0], desc[0], cnt[0:1], cap, desc[1], cnt[1:2], cap)
+
+    step()
+    torch.cuda.synchronize()
+    n0, n1 = (int(v) for v in cnt.cpu())
+    want = [pts[0][:n0].clone(), pts[1][:n1].clone(), desc[0][:n0].clone(), desc[1][:n1].clone()]
+    wp, wd, _ = oracle.detect_and_compute(a[:, :640].copy(), noctaves=8, scale=1.2, max_pts=cap)
+    assert_same_points(orb_amd.points_to_numpy(pts[0], n0), wp)
+    other = torch.from_numpy(orb_amd.synth_image(32, 333, 257, 8)).cuda()
+    opts, odesc = ctx.alloc_points(cap), ctx.alloc_desc(cap)
+    ocnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    ctx.capture_begin()
+    step()
+    graph = ctx.capture_end()
+    for rep in range(4):
+        print(f"graph replay {rep}", flush=True)
+        for t in (pts, desc, cnt):
+            t.zero_()
+        torch.cuda.synchronize()
+        graph.launch()
+        torch.cuda.synchronize()
+        assert [int(v) for v in cnt.cpu()] == [n0, n1]
+        got = [pts[0][:n0], pts[1][:n1], desc[0][:n0], desc[1][:n1]]
+        for g, w_ in zip(got, want):
+            assert torch.equal(g, w_)
+        # eager work of another size on the same context between replays
+        ctx.detect_and_compute(other, 333, 257, opts, odesc, ocnt)
+        torch.cuda.synchronize()
+    graph.close()
+    # a call that would need an allocation cannot be recorded
+    new_size = torch.from_numpy(orb_amd.synth_image(33, 320, 240, 8)).cuda()
+    torch.cuda.synchronize()
+    ctx.capture_begin()
+    with pytest.raises(orb_amd.OrbError):
+        ctx.detect_and_compute(new_size, 320, 240, opts, odesc, ocnt)
+    try:
+        ctx.capture_end().close()   # an empty recording; instantiating it may be refused
+    except orb_amd.OrbError:
+        pass
+    # the context works normally afterwards
+    ctx.detect_and_compute(new_size, 320, 240, opts, odesc, ocnt)
+    torch.cuda.synchronize()
+
+
 def test_match_reference_rule_and_device_counts(torch, ctx, oracle):
     import orb_amd
     w, h = 640, 480
