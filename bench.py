@@ -210,8 +210,9 @@ def extract(ctx, torch, dev, seeds, w, h, nshapes, cap, warp_seed=None, limit=No
 def learned_match(ctx, torch, dev, log=None):
     """Learned-bucket matching: a 1,048,576-descriptor database extracted by the pipeline from 2000
     seeded 1280x720 frames, 65536 queries from warped copies; MLP 256->512->512->1024 buckets (BF16,
-    FP32 accumulate, GELU) trained on exact matches of other warped copies; top-8 bucket probe, exact
-    fallback when max softmax < 0.3; recall@1 against exact brute force."""
+    FP32 accumulate, GELU, plus the centroid term at scale 0.125) trained on exact matches of other
+    warped copies; top-8 bucket probe, exact fallback when max softmax < 0.3; recall@1 against exact
+    brute force, and the trained network against nearest-centroid probing at equal probe counts."""
     import orb_amd
     w, h, nshapes, per = 1280, 720, 120, 525
     ndb, nq = 1 << 20, 65536
@@ -239,18 +240,13 @@ def learned_match(ctx, torch, dev, log=None):
     # exact labels of the training queries
     labels = ctx.match_top2(train_q, db, method=orb_amd.ORB_MATCH_MFMA)[0]
     torch.cuda.synchronize()
-    lx = orb_amd.LearnedIndex(ctx, nbuckets=1024, hidden=512, nprobe=8, fallback=0.3, seed=4)
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=1024, hidden=512, nprobe=8, fallback=0.3, seed=4, centroid_scale=0.125)
     t0 = time.perf_counter()
     lx.build(db, iters=8)
     t_build = time.perf_counter() - t0
     off, _, _ = lx.bucket_info()
     sizes = np.diff(off)
     note("buckets built")
-    t0 = time.perf_counter()
-    losses = lx.train(train_q, labels, epochs=3, batch=4096, lr=1e-3)
-    note(f"trained, losses {losses}")
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
 
     def timed(fn, reps=3):
         fn()
@@ -264,44 +260,53 @@ def learned_match(ctx, torch, dev, log=None):
         return r, best * 1e3
 
     exact, ms_exact = timed(lambda: ctx.match_top2(test_q, db, method=orb_amd.ORB_MATCH_MFMA))
-    (lq, nfb), ms_learned = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
-    (cq, _), ms_cent = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS))
     d_ex = exact[1].cpu().numpy()
     rec = lambda o: float((o[1].cpu().numpy() == d_ex).mean())
     rec_idx = lambda o: float((o[0].cpu().numpy() == exact[0].cpu().numpy()).mean())
-    # the network alone (no fallback), and a sweep of probe settings
+    # untrained, the network (last layer zero + centroid term) ranks buckets as the centroids do
     lx.set_probe(8, 0.0)
-    (nq0, _), ms_net = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
+    (u0, _), _ = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=1)
+    (c0, _), _ = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS), reps=1)
+    untrained_same = all(bool(torch.equal(a_, b_)) for a_, b_ in zip(u0, c0))
+    t0 = time.perf_counter()
+    losses = lx.train(train_q, labels, epochs=3, batch=4096, lr=1e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    note(f"trained, losses {losses}")
+    # the trained network against nearest-centroid probing, same probe counts, no fallback
+    versus = []
+    for npb in (4, 8, 16, 32):
+        lx.set_probe(npb, 0.0)
+        (o, _), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
+        (oc, _), msc = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS), reps=2)
+        versus.append({"nprobe": npb, "network_recall_at_1": round(rec(o), 5), "network_ms": round(ms, 3),
+                       "centroid_recall_at_1": round(rec(oc), 5), "centroid_ms": round(msc, 3)})
+    # the learned matcher as specified: top-8 probe, exact fallback below max softmax 0.3
+    lx.set_probe(8, 0.3)
+    (lq, nfb), ms_learned = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
     sweep = []
     for npb in (8, 16, 32):
-        for fb in (0.3, 0.5, 0.7):
+        for fb in (0.0, 0.3, 0.5):
             lx.set_probe(npb, fb)
             (o, nf), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
             sweep.append({"nprobe": npb, "fallback": fb, "ms": round(ms, 3), "recall_at_1": round(rec(o), 5),
                           "fallback_queries": int(nf), "speedup": round(ms_exact / ms, 3)})
     ok = [r for r in sweep if r["recall_at_1"] >= 0.99]
     best = min(ok, key=lambda r: r["ms"]) if ok else None
-    # the same bucket search probed by nearest centroids (no network, no fallback)
-    cent = []
-    for npb in (8, 16, 32, 64, 128):
-        lx.set_probe(npb, 0.3)
-        (o, _), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS), reps=2)
-        cent.append({"nprobe": npb, "ms": round(ms, 3), "recall_at_1": round(rec(o), 5),
-                     "speedup": round(ms_exact / ms, 3)})
-    ok_c = [r for r in cent if r["recall_at_1"] >= 0.99]
+    ok_c = [{"nprobe": r["nprobe"], "ms": r["centroid_ms"], "recall_at_1": r["centroid_recall_at_1"],
+             "speedup": round(ms_exact / r["centroid_ms"], 3)} for r in versus if r["centroid_recall_at_1"] >= 0.99]
     best_c = min(ok_c, key=lambda r: r["ms"]) if ok_c else None
     res = {
         "database": int(db.shape[0]), "queries": int(test_q.shape[0]), "train_queries": int(train_q.shape[0]),
-        "buckets": 1024, "nprobe": 8, "fallback_threshold": 0.3,
+        "buckets": 1024, "centroid_scale": 0.125,
         "bucket_size": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
         "exact_mfma_ms": round(ms_exact, 3),
-        "learned_ms": round(ms_learned, 3), "learned_recall_at_1": round(rec(lq), 5),
-        "learned_recall_idx": round(rec_idx(lq), 5), "fallback_queries": int(nfb),
-        "speedup_vs_exact": round(ms_exact / ms_learned, 3),
-        "centroid_probe_ms": round(ms_cent, 3), "centroid_probe_recall_at_1": round(rec(cq), 5),
-        "network_only_ms": round(ms_net, 3), "network_only_recall_at_1": round(rec(nq0), 5),
-        "sweep": sweep, "fastest_at_99_recall": best,
-        "centroid_sweep": cent, "centroid_fastest_at_99_recall": best_c,
+        "top8_fallback_0.3": {"ms": round(ms_learned, 3), "recall_at_1": round(rec(lq), 5),
+                              "recall_idx": round(rec_idx(lq), 5), "fallback_queries": int(nfb),
+                              "speedup_vs_exact": round(ms_exact / ms_learned, 3)},
+        "untrained_equals_centroid_probe": untrained_same,
+        "network_vs_centroids": versus,
+        "sweep": sweep, "fastest_at_99_recall": best, "centroid_fastest_at_99_recall": best_c,
         "train_loss_per_epoch": [round(float(x), 4) for x in losses],
         "build_s": round(t_build, 2), "train_s": round(t_train, 2),
         "data_s": round(t_q - t_all, 1),

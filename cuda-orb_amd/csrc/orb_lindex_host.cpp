@@ -86,6 +86,8 @@ struct orb_lindex {
     /* database */
     int ndb = 0;
     DBuf<uint8_t> db, db_sorted, centroids;
+    DBuf<uint16_t> cpm;      /* centroids as +/-centroid_scale bf16, nbuckets x 256 */
+    uint16_t cscale16 = 0;   /* bf16 bits of centroid_scale (0: no centroid term) */
     DBuf<int> assign, hist, offsets, cursor, perm;
     /* query workspace */
     DBuf<int> probes, list_count, list_offsets, list_cursor, lists, unit_offsets, fb_count, fb_index, fb_qidx;
@@ -120,6 +122,14 @@ int lfail(orb_lindex* L, int code, const char* msg) { return orb__ctx_fail(L ? L
     do { if (!(buf).ensure(n)) return lfail(L, ORB_E_NOMEM, "device allocation failed"); } while (0)
 
 size_t nparams(const orb_lindex* L) { return L->off[6]; }
+
+/* float -> bf16 bits, round to nearest even */
+uint16_t bf16_bits(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
 
 int ensure_batch(orb_lindex* L, int B)
 {
@@ -176,10 +186,27 @@ int forward(orb_lindex* L, int B, hipStream_t s)
     g.out32 = L->Z2.p;
     g.out16 = L->A2.p;
     LTRY(L, launch_gemm(g, s));
+    if (L->cscale16) {
+        /* centroid term: Z3 = <x, c_b> * centroid_scale, exact in fp32 */
+        if (!L->cpm.p) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
+        g = gemm(L->X.p, kIn, 1, L->cpm.p, 1, kIn, B, C, kIn);
+        g.out32 = L->Z3.p;
+        LTRY(L, launch_gemm(g, s));
+    }
     g = gemm(L->A2.p, H, 1, w16 + L->off[4], C, 1, B, C, H);
+    g.addin = L->cscale16 ? L->Z3.p : nullptr;
     g.bias = w + L->off[5];
     g.out32 = L->Z3.p;
     LTRY(L, launch_gemm(g, s));
+    return ORB_OK;
+}
+
+/* +/-centroid_scale expansion of the final centroids for the centroid term */
+int make_cpm(orb_lindex* L, hipStream_t s)
+{
+    if (!L->cscale16) return ORB_OK;
+    LALLOC(L, L->cpm, (size_t)L->C * kIn);
+    LTRY(L, launch_bits_pm(L->centroids.p, L->C, L->cscale16, (uint16_t)(L->cscale16 ^ 0x8000u), L->cpm.p, s));
     return ORB_OK;
 }
 
@@ -244,10 +271,15 @@ int orb_lindex_create(orb_ctx* ctx, const orb_lindex_params* p, orb_lindex** out
     std::vector<float> h(np, 0.f);
     Rng rng{p->seed * 0x2545F4914F6CDD1Dull + 11};
     const int fan_in[3] = {kIn, L->H, L->H};
+    L->cscale16 = p->centroid_scale != 0.f ? bf16_bits(p->centroid_scale) : 0;
     for (int t = 0; t < 3; t++) {
         const double a = 1.0 / std::sqrt((double)fan_in[t]);
         for (size_t i = L->off[2 * t]; i < L->off[2 * t + 1]; i++) h[i] = (float)((2.0 * rng.uniform() - 1.0) * a);
     }
+    /* with the centroid term the last layer starts at zero: the untrained
+       network ranks buckets as nearest-centroid probing does */
+    if (L->cscale16)
+        for (size_t i = L->off[4]; i < L->off[5]; i++) h[i] = 0.f;
     int rc = orb_lindex_set_weights(L, h.data());
     if (rc != ORB_OK) {
         delete L;
@@ -333,6 +365,8 @@ int orb_lindex_build(orb_lindex* L, const uint8_t* d_db, int ndb, int iters)
         LTRY(L, launch_bucket_scatter(L->assign.p, ndb, L->db.p, L->cursor.p, L->perm.p, L->db_sorted.p, s));
         if (it < iters) LTRY(L, launch_kmodes_update(L->db_sorted.p, L->offsets.p, nb, L->centroids.p, s));
     }
+    int rc = make_cpm(L, s);
+    if (rc != ORB_OK) return rc;
     LTRY(L, hipStreamSynchronize(s));
     return ORB_OK;
 }
@@ -404,6 +438,9 @@ int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int
     Rng rng{L->prm.seed * 0xD1B54A32D192ED03ull + 3};
     std::vector<int> ord(n);
     const int nbatch = n / batch;
+    /* cosine decay of the step size from lr to lr / 20 over the whole run */
+    const double total_steps = (double)epochs * nbatch;
+    long long t_step = 0;
     std::vector<float> lv(batch);
     for (int e = 0; e < epochs; e++) {
         for (int i = 0; i < n; i++) ord[i] = i;
@@ -424,7 +461,9 @@ int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int
             rc = backward(L, batch, s);
             if (rc != ORB_OK) return rc;
             L->step++;
-            LTRY(L, launch_adam(L->w.p, L->w16.p, L->g.p, L->m.p, L->v.p, (long long)nparams(L), lr, 0.9f, 0.999f,
+            const double frac = (double)(t_step++) / total_steps;
+            const float lr_t = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(3.14159265358979323846 * frac))));
+            LTRY(L, launch_adam(L->w.p, L->w16.p, L->g.p, L->m.p, L->v.p, (long long)nparams(L), lr_t, 0.9f, 0.999f,
                                 1e-8f, L->step, s));
             if (h_epoch_loss && (bi == nbatch - 1)) {
                 LTRY(L, hipMemcpyAsync(lv.data(), L->loss.p, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
@@ -539,9 +578,11 @@ int orb_lindex_save(orb_lindex* L, const char* path)
     if (rc != ORB_OK) return rc;
     FILE* f = fopen(path, "wb");
     if (!f) return lfail(L, ORB_E_ARG, "cannot open the file for writing");
-    const int32_t hdr[4] = {1, L->C, L->H, L->prm.nprobe};
+    /* version 2: after the fallback threshold, the bf16 bits of centroid_scale */
+    const int32_t hdr[4] = {2, L->C, L->H, L->prm.nprobe};
+    const uint32_t cs = L->cscale16;
     const bool ok = fwrite("ORBLIDX1", 1, 8, f) == 8 && fwrite(hdr, sizeof(hdr), 1, f) == 1 &&
-                    fwrite(&L->prm.fallback, sizeof(float), 1, f) == 1 &&
+                    fwrite(&L->prm.fallback, sizeof(float), 1, f) == 1 && fwrite(&cs, sizeof(cs), 1, f) == 1 &&
                     fwrite(w.data(), sizeof(float), w.size(), f) == w.size() &&
                     fwrite(cen.data(), 1, cen.size(), f) == cen.size();
     fclose(f);
@@ -556,8 +597,10 @@ int orb_lindex_load(orb_lindex* L, const char* path)
     char magic[8];
     int32_t hdr[4];
     float fb;
+    uint32_t cs = 0;
     bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ORBLIDX1", 8) == 0 && fread(hdr, sizeof(hdr), 1, f) == 1 &&
-              hdr[0] == 1 && hdr[1] == L->C && hdr[2] == L->H && fread(&fb, sizeof(float), 1, f) == 1;
+              (hdr[0] == 1 || hdr[0] == 2) && hdr[1] == L->C && hdr[2] == L->H && fread(&fb, sizeof(float), 1, f) == 1;
+    if (ok && hdr[0] == 2) ok = fread(&cs, sizeof(cs), 1, f) == 1 && cs <= 0xFFFFu;
     std::vector<float> w(nparams(L));
     std::vector<uint8_t> cen((size_t)L->C * 32);
     ok = ok && fread(w.data(), sizeof(float), w.size(), f) == w.size() && fread(cen.data(), 1, cen.size(), f) == cen.size();
@@ -571,6 +614,7 @@ int orb_lindex_load(orb_lindex* L, const char* path)
     LTRY(L, hipStreamSynchronize(s));
     L->prm.nprobe = hdr[3];
     L->prm.fallback = fb;
+    L->cscale16 = (uint16_t)cs;
     L->ndb = -1; /* centroids loaded: orb_lindex_build(db, n, 0) assigns the database to them */
     return ORB_OK;
 }

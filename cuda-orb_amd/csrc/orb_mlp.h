@@ -12,14 +12,16 @@ namespace orbk
 
 /* C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak],
    B(k, n) = B[k*sbk + n*sbn] (bf16 bits); M, N, K multiples of 64.
-   Epilogue, in order: z += bias[n]; z *= gelu'(zmul[m*ldc + n]) if zmul;
-   out32[m*ldc + n] = z; out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
+   Epilogue, in order: z += addin[m*ldc + n] if addin; z += bias[n];
+   z *= gelu'(zmul[m*ldc + n]) if zmul; out32[m*ldc + n] = z;
+   out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
 struct GemmArgs {
     const uint16_t* A;
     long long sam, sak;
     const uint16_t* B;
     long long sbk, sbn;
     int M, N, K;
+    const float* addin;
     const float* bias;
     int gelu;
     const float* zmul;
@@ -30,6 +32,8 @@ struct GemmArgs {
 
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s);
+/* as launch_bits_pm1 with the two values given as bf16 bits (set bit -> pos, clear bit -> neg) */
+hipError_t launch_bits_pm(const uint8_t* desc, int n, uint16_t pos, uint16_t neg, uint16_t* X, hipStream_t s);
 hipError_t launch_softmax_ce(const float* Z, const int* labels, int n, int C, uint16_t* dZ, float* loss,
                              hipStream_t s);
 hipError_t launch_colsum(const uint16_t* X, int n, int C, float* out, hipStream_t s);

@@ -121,7 +121,9 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + 16 * a + 4 * grp + r;
                 const long long o = (long long)m * g.ldc + n;
-                float z = acc[a][b][r] + bias;
+                float z = acc[a][b][r];
+                if (g.addin) z = z + g.addin[o];
+                z = z + bias;
                 if (g.zmul) z = z * gelu_grad(g.zmul[o]);
                 if (g.out32) g.out32[o] = z;
                 if (g.out16) g.out16[o] = f2bf(g.gelu ? gelu(z) : z);
@@ -133,14 +135,15 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
 /* ------------------------------------------------------------ elementwise */
 
 /* descriptor bits -> +/-1 bf16 (bit k of the descriptor -> column k) */
-__global__ void k_bits_pm1(const uint8_t* __restrict__ desc, int n, uint16_t* __restrict__ X)
+__global__ void k_bits_pm1(const uint8_t* __restrict__ desc, int n, uint16_t pos, uint16_t neg,
+                           uint16_t* __restrict__ X)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)n * 256) return;
     const long long r = i >> 8;
     const int k = (int)(i & 255);
     const int bit = (desc[r * 32 + (k >> 3)] >> (k & 7)) & 1;
-    X[i] = bit ? (uint16_t)0x3F80 : (uint16_t)0xBF80;
+    X[i] = bit ? pos : neg;
 }
 
 /* one wavefront per row of C logits: loss and bf16 gradient (softmax - onehot) / n */
@@ -255,11 +258,17 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s)
+hipError_t launch_bits_pm(const uint8_t* desc, int n, uint16_t pos, uint16_t neg, uint16_t* X, hipStream_t s)
 {
     const long long tot = (long long)n * 256;
-    if (tot > 0) hipLaunchKernelGGL(k_bits_pm1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, desc, n, X);
+    if (tot > 0)
+        hipLaunchKernelGGL(k_bits_pm1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, desc, n, pos, neg, X);
     return hipGetLastError();
+}
+
+hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s)
+{
+    return launch_bits_pm(desc, n, (uint16_t)0x3F80, (uint16_t)0xBF80, X, s);
 }
 
 hipError_t launch_softmax_ce(const float* Z, const int* labels, int n, int C, uint16_t* dZ, float* loss, hipStream_t s)

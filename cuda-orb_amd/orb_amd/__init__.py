@@ -54,7 +54,7 @@ class OrbError(RuntimeError):
 
 class LindexParams(ctypes.Structure):
     _fields_ = [("nbuckets", ctypes.c_int), ("hidden", ctypes.c_int), ("nprobe", ctypes.c_int),
-                ("fallback", ctypes.c_float), ("seed", ctypes.c_uint64)]
+                ("fallback", ctypes.c_float), ("seed", ctypes.c_uint64), ("centroid_scale", ctypes.c_float)]
 
 
 class Params(ctypes.Structure):
@@ -343,11 +343,11 @@ class LearnedIndex:
     """Learned bucket matcher (orb_lindex_*): k-modes buckets over a descriptor database, a BF16 MLP
     that routes queries to buckets, bucketed exact search with exact fallback."""
 
-    def __init__(self, ctx, nbuckets=1024, hidden=512, nprobe=8, fallback=0.3, seed=4):
+    def __init__(self, ctx, nbuckets=1024, hidden=512, nprobe=8, fallback=0.3, seed=4, centroid_scale=0.0):
         self.ctx = ctx
         torch = ctx._torch
         self._torch = torch
-        self.prm = LindexParams(nbuckets, hidden, nprobe, fallback, seed)
+        self.prm = LindexParams(nbuckets, hidden, nprobe, fallback, seed, centroid_scale)
         h = ctypes.c_void_p()
         ctx._bind_stream()
         ctx._check(lib().orb_lindex_create(ctx.h, ctypes.byref(self.prm), ctypes.byref(h)), "orb_lindex_create")

@@ -141,14 +141,20 @@ int orb_synth_warp(uint64_t seed, const uint8_t* src, int width, int height, int
    accumulate) trained on (query, exact nearest neighbour) pairs maps a query
    to buckets; a query searches its nprobe most likely buckets exactly, and
    falls back to the exact search of the whole database when the network's
-   largest softmax probability is below `fallback`.  Results follow
-   orb_match_top2's conventions (ties to the lowest database index). */
+   largest softmax probability is below `fallback`.  With centroid_scale
+   g != 0 the logits also get g * <x, c_b> for every bucket b, with x and c_b
+   the +/-1 expansions of the query and of the bucket's centroid (equal to
+   g * (256 - 2 * Hamming distance)), so the untrained network (last layer
+   zero) ranks buckets exactly as nearest-centroid probing does and training
+   learns a correction to that ranking.  Results follow orb_match_top2's
+   conventions (ties to the lowest database index). */
 typedef struct {
-    int nbuckets;    /* multiple of 64, <= 1024 */
-    int hidden;      /* multiple of 64 */
-    int nprobe;      /* buckets searched per query */
-    float fallback;  /* exact search when max softmax < fallback */
-    uint64_t seed;   /* weight initialisation, centroid choice, shuffling */
+    int nbuckets;          /* multiple of 64, <= 1024 */
+    int hidden;            /* multiple of 64 */
+    int nprobe;            /* buckets searched per query */
+    float fallback;        /* exact search when max softmax < fallback */
+    uint64_t seed;         /* weight initialisation, centroid choice, shuffling */
+    float centroid_scale;  /* g above (rounded to bf16); 0: the plain MLP */
 } orb_lindex_params;
 
 typedef struct orb_lindex orb_lindex;
@@ -168,7 +174,8 @@ int orb_lindex_build(orb_lindex* lx, const uint8_t* d_db, int ndb, int iters);
 int orb_lindex_bucket_info(orb_lindex* lx, int* h_offsets, uint8_t* h_centroids, int* d_bucket_of);
 /* Train on n queries whose exact nearest database entries are d_target
    (labels = their buckets): `epochs` shuffled passes of Adam steps with the
-   given batch (multiple of 64) and learning rate; h_epoch_loss (may be NULL)
+   given batch (multiple of 64); the step size decays from lr to lr/20 along a
+   cosine over the whole run; h_epoch_loss (may be NULL)
    receives the mean loss of each epoch's last batch.  Synchronous. */
 int orb_lindex_train(orb_lindex* lx, const uint8_t* d_q, const int* d_target, int n, int epochs, int batch,
                      float lr, float* h_epoch_loss);

@@ -9,7 +9,8 @@ exact bucketed search used to check the probe results.
 Network (no counterpart in the reference; it is the learned part the
 workload adds): X = +/-1 expansion of the 256 descriptor bits,
 Z1 = X W1 + b1, A1 = bf16(gelu(Z1)), Z2 = A1 W2 + b2, A2 = bf16(gelu(Z2)),
-Z3 = A2 W3 + b3 (bucket logits), loss = mean cross-entropy.
+Z3 = A2 W3 + b3 (bucket logits), plus with a centroid scale g the term
+X Cg^T (Cg = the bucket centroids' bits as +/-g), loss = mean cross-entropy.
 Backward: dZ3 = bf16((softmax(Z3) - onehot) / n), dW3 = A2^T dZ3,
 db3 = colsum(dZ3), dZ2 = bf16((dZ3 W3^T) * gelu'(Z2)), ... down to dW1.
 gelu(x) = x * Phi(x) with Phi the standard normal CDF (erf form).
@@ -45,19 +46,28 @@ def gelu_grad(z):
     return 0.5 * (1.0 + erf(z * SQRT1_2)) + z * INV_SQRT_2PI * np.exp(-0.5 * z * z)
 
 
-def forward(params, desc):
-    """params: dict of float64 arrays W1 (256,H), b1, W2 (H,H), b2, W3 (H,C), b3 (bf16-valued weights)."""
+def centroid_term(centroids, scale):
+    """(C, 32) uint8 centroids -> (C, 256) float64 of +/-bf16(scale)."""
+    return bits_pm1(centroids) * bf16(np.array([scale]))[0]
+
+
+def forward(params, desc, cg=None):
+    """params: dict of float64 arrays W1 (256,H), b1, W2 (H,H), b2, W3 (H,C), b3 (bf16-valued weights);
+    cg: optional centroid term matrix from centroid_term()."""
     X = bits_pm1(desc)
     Z1 = X @ params["W1"] + params["b1"]
     A1 = bf16(gelu(Z1))
     Z2 = A1 @ params["W2"] + params["b2"]
     A2 = bf16(gelu(Z2))
-    Z3 = A2 @ params["W3"] + params["b3"]
+    Z3 = A2 @ params["W3"]
+    if cg is not None:
+        Z3 = Z3 + X @ cg.T
+    Z3 = Z3 + params["b3"]
     return {"X": X, "Z1": Z1, "A1": A1, "Z2": Z2, "A2": A2, "Z3": Z3}
 
 
-def loss_and_grads(params, desc, labels):
-    f = forward(params, desc)
+def loss_and_grads(params, desc, labels, cg=None):
+    f = forward(params, desc, cg)
     n = len(labels)
     Z3 = f["Z3"]
     m = Z3.max(1, keepdims=True)
