@@ -407,9 +407,9 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
         key_bytes += round256(sizeof(unsigned long long) * (size_t)(g.h[0] + 2 * key_guard_rows[f]) * g.p[0]);
         const int rows = g.h[0] - 2 * border, span = g.w[0] - 2 * border;
         F.nms_first = nms_blocks;
-        F.nms_x0 = border & ~3; /* segments start dword-aligned; pixels left of the border are masked */
+        F.nms_x0 = 0; /* segments start at x = 0; pixels outside the border are masked */
         F.nms_rows = rows > 0 && span > 0 ? rows : 0;
-        F.nms_segs = rows > 0 && span > 0 ? (g.w[0] - border - F.nms_x0 + kNmsSeg - 1) / kNmsSeg : 0;
+        F.nms_segs = rows > 0 && span > 0 ? (g.w[0] - border + kNmsSeg - 1) / kNmsSeg : 0;
         for (int b = 0; b < F.nms_rows * F.nms_segs; b++) nms_frame.push_back(f);
         nms_blocks += F.nms_rows * F.nms_segs;
     }
@@ -427,6 +427,8 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     const size_t o_tabs = carve(sizeof(int) * (size_t)tab_ints);
     const size_t o_pyr = carve(2 * pyr_bytes);
     const size_t o_keys = carve(key_bytes);
+    std::vector<size_t> o_bits(nf);
+    for (int f = 0; f < nf; f++) o_bits[f] = carve(sizeof(uint32_t) * (size_t)geo[f].h[0] * (geo[f].p[0] / 32));
     const size_t total = off;
     void* arena = nullptr;
     hipError_t e = hipMalloc(&arena, total);
@@ -453,6 +455,7 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
         F.raw = pyr + pyr_off[f] + guard[f];
         F.blur = pyr + pyr_bytes + pyr_off[f] + guard[f];
         F.keys = (unsigned long long*)((char*)keys + key_off[f]) + (size_t)key_guard_rows[f] * g.p[0];
+        F.cbits = (uint32_t*)(base + o_bits[f]);
         if (g.ref) continue;
         for (int l = 1; l < g.L; l++) {
             int* t = tabs.data() + F.tab_off[l];

@@ -40,9 +40,10 @@ constexpr int kPyrVariants = 5;
 constexpr int kPyrTHs[kPyrVariants] = {15, 22, 29, 36, 50};
 /* detection tile: 128 x 32 pixels, 256 threads */
 constexpr int kDetTW = 128, kDetTH = 32;
-/* NMS block: one level-0 row segment of 1024 pixels, 4 per thread; at most
-   every other pixel of a row survives, so 512 candidates fit a segment */
-constexpr int kNmsSeg = 1024;
+/* NMS unit: one wavefront per level-0 row segment of 2048 pixels (64 words
+   of the corner bitmap); at most every other pixel of a row survives, so
+   kNmsCap candidates fit a segment */
+constexpr int kNmsSeg = 2048;
 constexpr int kNmsCap = kNmsSeg / 2;
 /* describe: one wavefront per keypoint, 4 per workgroup, at most
    kDescMaxBlocks workgroups per frame looping over the points */
@@ -68,7 +69,10 @@ struct FrameDev {
     uint8_t* raw;              /* level-0 start of the raw flat pyramid */
     uint8_t* blur;             /* level-0 start of the blurred flat pyramid */
     unsigned long long* keys;  /* row 0 of the key map */
-    /* NMS blocks: rows [border, h0-border), segments of kNmsSeg pixels from nms_x0 */
+    uint32_t* cbits;           /* corner bitmap: bit x of row y set where a scored corner landed
+                                  (p0/32 words per row; the NMS clears the words it reads) */
+    /* NMS units: rows [border, h0-border), nms_segs segments of kNmsSeg pixels per
+       row from x = 0, from unit nms_first (also the index of the unit's segment) */
     int nms_first, nms_rows, nms_segs, nms_x0;
     int rmax;                  /* largest NMS radius of the frame's levels */
 };
@@ -99,10 +103,10 @@ struct DetectLaunch {
     int nframes;
     const Job* pyr_jobs; int pyr_njobs; int pyr_blocks; int pyr_units; int pyr_variant;
     const Job* det_jobs; int det_njobs; int det_blocks;
-    int nms_blocks;
-    const int* nms_block_frame; /* frame of each NMS block */
-    Candidate* cand;            /* kNmsSeg per NMS block */
-    int* blk_count;             /* survivors per NMS block */
+    int nms_blocks;             /* NMS units (row segments) */
+    const int* nms_block_frame; /* frame of each NMS unit */
+    Candidate* cand;            /* kNmsCap per NMS unit */
+    int* blk_count;             /* survivors per NMS unit */
     uint32_t* epochs;           /* per frame */
     int max_blocks_per_frame;
     int desc_blocks_per_frame;

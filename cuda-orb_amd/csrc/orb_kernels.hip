@@ -6,8 +6,9 @@
  *               (orbd.cu:514-535 pyramid, orbd.cu:1273-1338 blur)
  *   k_detect    FAST-9 segment test + Harris / FAST score on every level,
  *               64-bit atomic max into the level-0 key map (orbd.cu:672-756)
- *   k_nms       square cross-level NMS, scan compaction of the survivors of
- *               each 1024-pixel row segment (orbd.cu:919-983)
+ *   k_nms       square cross-level NMS of the pixels detection marked, the
+ *               survivors of each 2048-pixel row segment in x order
+ *               (orbd.cu:919-983)
  *   k_select    per-frame scan of the segment counts, keypoints out in
  *               raster order; with more than max_pts survivors, an exact
  *               radix select of the score cut
@@ -125,6 +126,14 @@ __device__ __forceinline__ int from_l0(const FrameDev& F, const int* __restrict_
     return v;
 }
 
+/* LDS written by lanes of this wavefront is visible to its other lanes */
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int wave_sum(int v)
 {
 #pragma unroll
@@ -182,7 +191,7 @@ struct PyrLane {
 template <int MODE>
 __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRow<MODE>& R)
 {
-    if (MODE == 2) {
+    if constexpr (MODE == 2) {
         const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
         const int sa = ya * P.ip, sb = yb * P.ip;
 #pragma unroll
@@ -193,7 +202,7 @@ __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRo
             R.v[4 * k + 3] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], sb, 0);
         }
         R.fy = ty & 255;
-    } else if (MODE == 1) {
+    } else if constexpr (MODE == 1) {
         const int so = (yr << P.l) * P.ip;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -215,7 +224,7 @@ __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRo
 template <int MODE>
 __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>& R, uint32_t& hin, uint32_t& out)
 {
-    if (MODE == 2) {
+    if constexpr (MODE == 2) {
         hin = 0;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -224,7 +233,7 @@ __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>&
             hin |= (uint32_t)(((256 - R.fy) * top + R.fy * bot + 32768) >> 16) << (8 * k);
         }
         out = hin & ~P.padmask;
-    } else if (MODE == 1) {
+    } else if constexpr (MODE == 1) {
         hin = R.v[0] | (R.v[1] << 8) | (R.v[2] << 16) | (R.v[3] << 24);
         const uint32_t pad = R.v[4] | (R.v[5] << 8) | (R.v[6] << 16) | (R.v[7] << 24);
         out = (hin & ~P.padmask) | pad;
@@ -613,6 +622,8 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
                                        (unsigned long long)(7 - l);
         __hip_atomic_fetch_max(as_global(F.keys) + (long long)Y0 * F.p0 + X0, key, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(as_global(F.cbits) + (long long)Y0 * (F.p0 >> 5) + (X0 >> 5), 1u << (X0 & 31),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -648,139 +659,120 @@ __device__ __forceinline__ int block_scan(int v, int* s_tmp, int& total)
     return res;
 }
 
-/* NMS of orbd.cu:945-966: a key survives unless a positive neighbour in its
-   (2r+1)^2 square is greater, or equal and at dx <= 0 && dy <= 0 (flat
-   indexing with pitch p0); r is the radius of the key's own level.
-   A thread owns 4 adjacent level-0 pixels of one row. */
-
-/* radius <= 4 (every bilinear pyramid up to 8 levels at scale <= 1.25): per
-   neighbour row, the 12 keys x-4 .. x+7 around the 4 pixels come in three
-   aligned 32-byte loads and all comparisons run from registers */
-__device__ __forceinline__ uint32_t nms_rows4(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0, int xs,
-                                              const unsigned long long* k4, uint32_t valid, uint32_t epoch,
-                                              const int* rk, int R)
+/* NMS of orbd.cu:945-966: a key survives unless a live neighbour in its
+   (2r+1)^2 square (r the radius of the key's own level) is greater, or equal
+   at dx <= 0 && dy <= 0, with the reference's flat indexing (pitch p0).
+   Detection marks every scored corner in a bitmap of the level-0 pixels, so
+   the NMS only visits marked pixels: one wavefront per 2048-pixel row
+   segment reads its 64 bitmap words (and clears them for the next call),
+   compacts the marked pixels in x order, checks one per lane and writes the
+   survivors of the segment in x order. */
+template <int R>
+__device__ __forceinline__ bool nms_suppressed(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0,
+                                               int x, uint32_t s, uint32_t epoch)
 {
-    uint32_t supp = 0;
-    uint32_t sk[4];
+    bool sup = false;
 #pragma unroll
-    for (int k = 0; k < 4; k++) sk[k] = (uint32_t)(k4[k] >> 3) & 0x7fffffffu;
     for (int dy = -R; dy <= R; dy++) {
-        const ORB_GLOBAL ulonglong4* rp = (const ORB_GLOBAL ulonglong4*)(rowp + (long long)dy * p0 + xs - 4);
-        unsigned long long nb[12];
+        unsigned long long nk[2 * R + 1];
 #pragma unroll
-        for (int t = 0; t < 3; t++) {
-            const ulonglong4 v = rp[t];
-            nb[4 * t] = v.x;
-            nb[4 * t + 1] = v.y;
-            nb[4 * t + 2] = v.z;
-            nb[4 * t + 3] = v.w;
-        }
+        for (int dx = -R; dx <= R; dx++) nk[dx + R] = rowp[(long long)dy * p0 + x + dx];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int r = rk[k];
-            if (!((valid >> k) & 1u) || abs(dy) > r) continue;
-#pragma unroll
-            for (int dx = -4; dx <= 4; dx++) {
-                if (abs(dx) > r || (dx == 0 && dy == 0)) continue;
-                const unsigned long long nk = nb[4 + k + dx];
-                const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
-                const bool live = (uint32_t)(nk >> kEpochShift) == epoch;
-                if (live && (ns > sk[k] || (ns == sk[k] && dx <= 0 && dy <= 0))) supp |= 1u << k;
-            }
+        for (int dx = -R; dx <= R; dx++) {
+            if (dx == 0 && dy == 0) continue;
+            const uint32_t ns = (uint32_t)(nk[dx + R] >> 3) & 0x7fffffffu;
+            const bool live = (uint32_t)(nk[dx + R] >> kEpochShift) == epoch;
+            sup |= live && (ns > s || (ns == s && dx <= 0 && dy <= 0));
         }
     }
-    return valid & ~supp;
+    return sup;
 }
 
-/* any radius: the wavefront takes its candidates one at a time and spreads
-   each neighbourhood over its 64 lanes */
-__device__ __forceinline__ uint32_t nms_wave(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0, int xs,
-                                             const unsigned long long* k4, uint32_t valid, uint32_t epoch,
-                                             const int* __restrict__ radius)
+__global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames, const int* __restrict__ unit_frame,
+                                             int nunits, const uint32_t* __restrict__ epochs,
+                                             Candidate* __restrict__ cand, int* __restrict__ seg_count)
 {
-    const int lane = threadIdx.x & 63;
-    uint32_t surv = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        unsigned long long m = __ballot((valid >> k) & 1u);
-        while (m) {
-            const int src = __ffsll((long long)m) - 1;
-            m &= m - 1;
-            const uint32_t klo = (uint32_t)__shfl((int)(uint32_t)k4[k], src, 64);
-            const uint32_t khi = (uint32_t)__shfl((int)(uint32_t)(k4[k] >> 32), src, 64);
-            const unsigned long long key = ((unsigned long long)khi << 32) | klo;
-            const int x = xs + 4 * (src - lane) + k;
-            const uint32_t sbits = (uint32_t)(key >> 3) & 0x7fffffffu;
-            const int r = radius[7u - (uint32_t)(key & 7u)];
-            const int side = 2 * r + 1;
-            bool sup = false;
-            for (int i = lane; i < side * side; i += 64) {
-                const int dy = i / side - r, dx = i - (i / side) * side - r;
-                if (dx == 0 && dy == 0) continue;
-                const unsigned long long nk = rowp[(long long)dy * p0 + x + dx];
-                const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
-                if ((uint32_t)(nk >> kEpochShift) == epoch && (ns > sbits || (ns == sbits && dx <= 0 && dy <= 0)))
-                    sup = true;
-            }
-            if (!__any(sup) && lane == src) surv |= 1u << k;
-        }
-    }
-    return surv;
-}
-
-__global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames, const int* __restrict__ block_frame,
-                                             const uint32_t* __restrict__ epochs, Candidate* __restrict__ cand,
-                                             int* __restrict__ blk_count)
-{
-    __shared__ int s_tmp[8];
+    __shared__ uint16_t s_x[4][kNmsSeg];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     /* consecutive rows on one XCD: their shared neighbour rows hit its L2 */
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int f = block_frame[bid];
+    const int u = xcd_remap(blockIdx.x, gridDim.x) * 4 + wid;
+    if (u >= nunits) return;
+    const int f = unit_frame[u];
     const FrameDev& F = frames[f];
-    const int b = bid - F.nms_first;
-    const int row = b / F.nms_segs, seg = b - row * F.nms_segs;
+    const int lu = u - F.nms_first;
+    const int row = lu / F.nms_segs, c = lu - row * F.nms_segs;
     const int y = F.border + row;
-    const int xs = F.nms_x0 + seg * kNmsSeg + 4 * threadIdx.x;
-    const int xlo = F.border, xhi = F.w0 - F.border;
+    const int wpr = F.p0 >> 5;
+    const int w = c * (kNmsSeg / 32) + lane;
+    uint32_t bits = 0;
+    if (w < wpr) {
+        ORB_GLOBAL uint32_t* bw = as_global(F.cbits) + (long long)y * wpr + w;
+        bits = *bw;
+        if (bits) *bw = 0u;
+    }
+    /* pixels inside [border, w0 - border) only */
+    const int x0 = w * 32;
+    const int lo = max(F.border - x0, 0), hi = min(F.w0 - F.border - x0, 32);
+    bits = hi <= lo ? 0u : bits & ((hi == 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u));
+    /* marked pixels of the segment, in x order, into LDS */
+    const int nl = __popc(bits);
+    int incl = nl;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, 63, 64);
+    int pos = incl - nl;
+    for (uint32_t b = bits; b; b &= b - 1) s_x[wid][pos++] = (uint16_t)(x0 + __ffs(b) - 1);
+    wave_sync_lds();
     const uint32_t epoch = epochs[f];
     const ORB_GLOBAL unsigned long long* rowp = as_global(F.keys) + (long long)y * F.p0;
-    unsigned long long k4[4] = {0ull, 0ull, 0ull, 0ull};
-    if (xs < xhi) {
-        /* xs is a multiple of 4 below w0 <= p0 (a multiple of 128): 32-byte aligned, inside the row */
-        const ulonglong4 a = *(const ORB_GLOBAL ulonglong4*)(rowp + xs);
-        k4[0] = a.x;
-        k4[1] = a.y;
-        k4[2] = a.z;
-        k4[3] = a.w;
-    }
-    uint32_t valid = 0;
-    int rk[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int x = xs + k;
-        const bool c = x >= xlo && x < xhi && (uint32_t)(k4[k] >> kEpochShift) == epoch;
-        valid |= (uint32_t)c << k;
-        rk[k] = c ? F.radius[7u - (uint32_t)(k4[k] & 7u)] : 0;
-    }
-    uint32_t surv = 0;
-    if (__any(valid != 0)) {
-        if (F.rmax <= 4) surv = nms_rows4(rowp, F.p0, xs, k4, valid, epoch, rk, F.rmax);
-        else surv = nms_wave(rowp, F.p0, xs, k4, valid, epoch, F.radius);
-    }
-    int total;
-    int pos = block_scan<256>(__popc(surv), s_tmp, total);
-    ORB_GLOBAL Candidate* out = as_global(cand) + (long long)bid * kNmsCap;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (surv & (1u << k)) {
-            const unsigned long long key = k4[k];
-            Candidate c;
-            c.pos = ((7u - (uint32_t)(key & 7u)) << 28) | ((uint32_t)y << 14) | (uint32_t)(xs + k);
-            c.score = (uint32_t)(key >> 3) & 0x7fffffffu;
-            out[pos++] = c;
+    ORB_GLOBAL Candidate* out = as_global(cand) + (long long)u * kNmsCap;
+    int nout = 0;
+    for (int j0 = 0; j0 < total; j0 += 64) {
+        const int j = j0 + lane;
+        bool surv = false;
+        Candidate cd;
+        cd.pos = 0;
+        cd.score = 0;
+        if (j < total) {
+            const int x = s_x[wid][j];
+            const unsigned long long key = rowp[x];
+            if ((uint32_t)(key >> kEpochShift) == epoch) {
+                const uint32_t s = (uint32_t)(key >> 3) & 0x7fffffffu;
+                const uint32_t lv = 7u - (uint32_t)(key & 7u);
+                const int r = F.radius[lv];
+                bool sup;
+                switch (r) {
+                case 1: sup = nms_suppressed<1>(rowp, F.p0, x, s, epoch); break;
+                case 2: sup = nms_suppressed<2>(rowp, F.p0, x, s, epoch); break;
+                case 3: sup = nms_suppressed<3>(rowp, F.p0, x, s, epoch); break;
+                case 4: sup = nms_suppressed<4>(rowp, F.p0, x, s, epoch); break;
+                default:
+                    sup = false;
+                    for (int dy = -r; dy <= r; dy++)
+                        for (int dx = -r; dx <= r; dx++) {
+                            if (dx == 0 && dy == 0) continue;
+                            const unsigned long long nk = rowp[(long long)dy * F.p0 + x + dx];
+                            const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
+                            if ((uint32_t)(nk >> kEpochShift) == epoch && (ns > s || (ns == s && dx <= 0 && dy <= 0)))
+                                sup = true;
+                        }
+                }
+                if (!sup) {
+                    surv = true;
+                    cd.pos = (lv << 28) | ((uint32_t)y << 14) | (uint32_t)x;
+                    cd.score = s;
+                }
+            }
         }
+        const unsigned long long m = __ballot(surv);
+        if (surv)
+            out[nout + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0))] = cd;
+        nout += __popcll(m);
     }
-    if (threadIdx.x == 0) blk_count[bid] = total;
+    if (lane == 0) seg_count[u] = nout;
 }
 
 /* ---------------------------------------------------------- select + emit */
@@ -894,13 +886,6 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
 }
 
 /* --------------------------------------------------------- angle + describe */
-
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 /* staged patches per wavefront: the raw 31-row disc region (36-byte rows) and
    the blurred 37x37 region of the rotated pattern (40-byte rows), each row the
@@ -1099,8 +1084,8 @@ hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s,
     }
     if (d.nms_blocks > 0) {
         ev_start(ev, kStNms, s);
-        hipLaunchKernelGGL(k_nms, dim3(d.nms_blocks), dim3(256), 0, s, d.frames, d.nms_block_frame, d.epochs,
-                           d.cand, d.blk_count);
+        hipLaunchKernelGGL(k_nms, dim3((d.nms_blocks + 3) / 4), dim3(256), 0, s, d.frames, d.nms_block_frame,
+                           d.nms_blocks, d.epochs, d.cand, d.blk_count);
         ev_stop(ev, kStNms, s);
     }
     ev_start(ev, kStSelect, s);
