@@ -479,6 +479,36 @@ __device__ float harris_lds(const uint8_t* c, float scale)
     return __fmul_rn(__fmaf_rn(-0.04f, __int2float_rn(B), __int2float_rn(A)), scale);
 }
 
+/* exclusive scan of one int per thread over a workgroup of NT threads */
+template <int NT>
+__device__ __forceinline__ int block_scan(int v, int* s_tmp, int& total)
+{
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int q = lane < NW ? s_tmp[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < NW; o <<= 1) {
+            const int y = __shfl_up(q, o, 64);
+            if (lane >= o) q += y;
+        }
+        if (lane < NW) s_tmp[NW + lane] = q;
+    }
+    __syncthreads();
+    total = s_tmp[2 * NW - 1];
+    const int res = x - v + (wid > 0 ? s_tmp[NW + wid - 1] : 0);
+    __syncthreads();
+    return res;
+}
+
 /* Slot of this lane in a list that the wavefront appends to: one LDS atomic
    per wavefront, lanes ordered by lane id; -1 where pred is false. */
 __device__ __forceinline__ int wave_append(bool pred, int* counter)
@@ -512,6 +542,7 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     __shared__ uint16_t s_cand[kDetTW * kDetTH];
     __shared__ uint16_t s_corner[kDetTW * kDetTH];
     __shared__ int s_n[2];
+    __shared__ int s_scan[8];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const Job jb = find_job(jobs, njobs, b);
     const FrameDev& F = frames[jb.frame];
@@ -541,14 +572,19 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     }
     __syncthreads();
     const int thr = T.threshold;
-    /* pass 1: compass test on every pixel of the tile */
+    /* pass 1: compass test on every pixel of the tile (a bit per pixel of the
+       thread's column), then one workgroup-wide compaction */
+    int ncand;
     {
+        constexpr int kRows = kDetTH * kDetTW / 256, kStep = 256 / kDetTW;
+        static_assert(kRows <= 32, "a bit per row of the thread's column");
         const int lx = tid & (kDetTW - 1);
         const int ix = x0 + lx;
         const bool col_ok = ix >= 3 && ix < w - 3;
+        uint32_t m = 0;
 #pragma unroll
-        for (int rr = 0; rr < kDetTH * kDetTW / 256; rr++) {
-            const int ly = tid / kDetTW + (256 / kDetTW) * rr;
+        for (int rr = 0; rr < kRows; rr++) {
+            const int ly = tid / kDetTW + kStep * rr;
             const int iy = y0 + ly;
             const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx + 4];
             const int v = c[0];
@@ -559,14 +595,16 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
             const bool b0 = q0 > hi, b4 = q4 > hi, b8 = q8 > hi, b12 = q12 > hi;
             const bool dark = (d0 && d4) || (d4 && d8) || (d8 && d12) || (d12 && d0);
             const bool bright = (b0 && b4) || (b4 && b8) || (b8 && b12) || (b12 && b0);
-            const bool cand = col_ok && iy >= 3 && iy < h - 3 && (dark || bright);
-            const int slot = wave_append(cand, &s_n[0]);
-            if (slot >= 0) s_cand[slot] = (uint16_t)((ly << 8) | lx);
+            m |= (uint32_t)(col_ok && iy >= 3 && iy < h - 3 && (dark || bright)) << rr;
+        }
+        int off = block_scan<256>(__popc(m), s_scan, ncand);
+        for (; m; m &= m - 1) {
+            const int rr = __ffs(m) - 1;
+            s_cand[off++] = (uint16_t)(((tid / kDetTW + kStep * rr) << 8) | lx);
         }
     }
     __syncthreads();
     /* pass 2: exact segment test on the candidates */
-    const int ncand = s_n[0];
     for (int i0 = 0; i0 < ncand; i0 += 256) {
         const int i = i0 + tid;
         bool corner = false;
@@ -629,35 +667,6 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
 
 /* -------------------------------------------------------------------- NMS */
 
-/* exclusive scan of one int per thread over a workgroup of NT threads */
-template <int NT>
-__device__ __forceinline__ int block_scan(int v, int* s_tmp, int& total)
-{
-    constexpr int NW = NT / 64;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    int x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) s_tmp[wid] = x;
-    __syncthreads();
-    if (wid == 0) {
-        int q = lane < NW ? s_tmp[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < NW; o <<= 1) {
-            const int y = __shfl_up(q, o, 64);
-            if (lane >= o) q += y;
-        }
-        if (lane < NW) s_tmp[NW + lane] = q;
-    }
-    __syncthreads();
-    total = s_tmp[2 * NW - 1];
-    const int res = x - v + (wid > 0 ? s_tmp[NW + wid - 1] : 0);
-    __syncthreads();
-    return res;
-}
 
 /* NMS of orbd.cu:945-966: a key survives unless a live neighbour in its
    (2r+1)^2 square (r the radius of the key's own level) is greater, or equal
