@@ -368,6 +368,7 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
             F.loff[l] = g.off[l];
             F.radius[l] = g.radius[l];
             F.inv_sc[l] = (float)(1.0 / g.sc[l]);
+            F.sc[l] = g.sc[l];
             F.tab_off[l] = tab_ints;
             if (!g.ref && l > 0) tab_ints += 2 * (g.w[l] + g.h[l]);
             if (g.radius[l] > rmax) rmax = g.radius[l];

@@ -64,6 +64,7 @@ struct FrameDev {
     int lw[kMaxLevels], lh[kMaxLevels], lp[kMaxLevels], radius[kMaxLevels];
     long long loff[kMaxLevels];
     float inv_sc[kMaxLevels];  /* 1/s^l, level-0 -> level coordinate (scale != 2) */
+    double sc[kMaxLevels];     /* s^l, level -> level-0 coordinate (scale != 2) */
     int tab_off[kMaxLevels];   /* into the int table: xsrc[w], ysrc[h], x2l0[w], y2l0[h] */
     long long total;
     uint8_t* raw;              /* level-0 start of the raw flat pyramid */

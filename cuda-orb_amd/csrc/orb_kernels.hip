@@ -104,12 +104,15 @@ __device__ __forceinline__ const int* level_tab(const FrameDev& F, const int* ta
     return tabs + F.tab_off[l];
 }
 
+/* level coordinate -> level-0 coordinate: rn(v * s^l), evaluated as the
+   host tables are, floor(v * s^l + 0.5) in double without contraction */
 __device__ __forceinline__ int to_l0(const FrameDev& F, const int* __restrict__ tabs, int l, int v, bool is_y)
 {
+    (void)tabs;
+    (void)is_y;
     if (l == 0) return v;
     if (F.ref_mode) return v << l;
-    const int* t = level_tab(F, tabs, l);
-    return is_y ? t[2 * F.lw[l] + F.lh[l] + v] : t[F.lw[l] + F.lh[l] + v];
+    return (int)floor(__dadd_rn(__dmul_rn((double)v, F.sc[l]), 0.5));
 }
 
 /* level-0 coordinate -> level coordinate: the unique v with to_l0(v) == x0 */
@@ -926,12 +929,26 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
     if (threadIdx.x < 17) s_umax[threadIdx.x] = T.umax[threadIdx.x];
     __syncthreads();
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = *as_global(F.count);
     const int hk = T.half_patch, side = 2 * hk + 1;
-    for (int pi = lb * kDescPerBlock + wid; pi < n; pi += blocks_per_frame * kDescPerBlock) {
+    /* the count and the wavefront's first point load together */
+    int pi = lb * kDescPerBlock + wid;
+    int px = 0, py = 0, pl = 0;
+    if (pi < F.cap) {
+        const ORB_GLOBAL orb::OrbPoint* P = as_global(F.pts) + pi;
+        px = P->x;
+        py = P->y;
+        pl = P->octave;
+    }
+    const int n = *as_global(F.count);
+    for (; pi < n; pi += blocks_per_frame * kDescPerBlock) {
         ORB_GLOBAL orb::OrbPoint* P = as_global(F.pts) + pi;
-        const int l = P->octave;
-        const int cx = from_l0(F, T.tabs, l, P->x, false), cy = from_l0(F, T.tabs, l, P->y, true);
+        if (pi >= lb * kDescPerBlock + wid + blocks_per_frame * kDescPerBlock) {
+            px = P->x;
+            py = P->y;
+            pl = P->octave;
+        }
+        const int l = pl;
+        const int cx = from_l0(F, T.tabs, l, px, false), cy = from_l0(F, T.tabs, l, py, true);
         const int p = F.lp[l];
         const long long co = F.loff[l] + (long long)cy * p + cx;
 
