@@ -56,8 +56,8 @@ def stage_bytes(stage, frames, npts):
             tot += h * p0 + 2 * padded
         elif stage == "detect":         # read every raw level once
             tot += padded
-        elif stage == "nms":            # read the 8-byte level-0 key map inside the border
-            tot += 8 * (w - 62) * (h - 62)
+        elif stage == "nms":            # read the level-0 corner bitmap (1 bit per pixel) and clear it
+            tot += 2 * h * p0 // 8
     if stage == "describe":             # per keypoint: 749-px raw disc, 37x37 blurred patch, point, descriptor
         tot += npts * (749 + 37 * 37 + 28 + 32)
     return tot
