@@ -37,7 +37,7 @@ constexpr uint32_t kEpochLimit = 1u << 30;
 constexpr int kPyrStripW = 248;
 constexpr int kPyrBands = 8;         /* horizontal bands per frame in the unit order */
 constexpr int kPyrVariants = 5;
-constexpr int kPyrTHs[kPyrVariants] = {15, 22, 29, 36, 50};
+constexpr int kPyrTHs[kPyrVariants] = {18, 26, 34, 42, 58};
 /* detection tile: 128 x 32 pixels, 256 threads */
 constexpr int kDetTW = 128, kDetTH = 32;
 /* NMS unit: one wavefront per level-0 row segment of 2048 pixels (64 words

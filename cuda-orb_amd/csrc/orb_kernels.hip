@@ -169,9 +169,11 @@ __device__ __forceinline__ uint32_t lane_get(uint32_t v, int addr)
    decimation); MODE 2: bilinear. */
 template <int MODE>
 struct PyrRow {
-    uint32_t v[MODE == 2 ? 16 : (MODE == 1 ? 8 : 3)];
+    uint32_t v[MODE == 0 ? 3 : 8];
     int fy;
 };
+
+typedef unsigned short pyr_u16x2 __attribute__((ext_vector_type(2)));
 
 /* Per-lane constants of a strip.  All memory goes through buffer resources:
    the row base is a wave-uniform scalar offset and the column a 32-bit
@@ -184,29 +186,32 @@ constexpr uint32_t kOob = 0x40000000u;
 struct PyrLane {
     __amdgpu_buffer_rsrc_t src;  /* the input image, h0 * ip bytes */
     int ip, l, h0;
-    uint32_t xa[4], xb[4];       /* source byte offsets in a row (MODE 1: xb = the padding columns' own bytes) */
-    int fx[4];
+    /* MODE 1: xa = source byte offset in a row, xb = the padding columns' own bytes.
+       MODE 2: xa = offset of the source dword holding the column pair (x0, x1),
+       xb = v_perm selector giving x0 | x1 << 16 from it, wx = (256 - fx) | fx << 16 */
+    uint32_t xa[4], xb[4], wx[4];
     uint32_t base4, sel, direct; /* MODE 0: reflect window start, byte selector, the lane's own dword */
     uint32_t padmask;            /* bytes of the lane's dword that lie in the row padding */
     uint32_t ipmask;             /* level 0: padding bytes that exist in the input row */
 };
 
+/* `live` false (rows past the walk): every load reads the out-of-range offset,
+   which returns 0 without touching memory */
 template <int MODE>
-__device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRow<MODE>& R)
+__device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, bool live, PyrRow<MODE>& R)
 {
     if constexpr (MODE == 2) {
+        /* one (unaligned) dword per source row and column holds both taps */
         const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
-        const int sa = ya * P.ip, sb = yb * P.ip;
+        const uint32_t sa = live ? (uint32_t)(ya * P.ip) : kOob, sb = live ? (uint32_t)(yb * P.ip) : kOob;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            R.v[4 * k + 0] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xa[k], sa, 0);
-            R.v[4 * k + 1] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], sa, 0);
-            R.v[4 * k + 2] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xa[k], sb, 0);
-            R.v[4 * k + 3] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], sb, 0);
+            R.v[2 * k + 0] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.xa[k], sa, 0);
+            R.v[2 * k + 1] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.xa[k], sb, 0);
         }
         R.fy = ty & 255;
     } else if constexpr (MODE == 1) {
-        const int so = (yr << P.l) * P.ip;
+        const uint32_t so = live ? (uint32_t)((yr << P.l) * P.ip) : kOob;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             R.v[k] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xa[k], so, 0);
@@ -214,7 +219,7 @@ __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, PyrRo
         }
         R.fy = 0;
     } else {
-        const int so = yr * P.ip;
+        const uint32_t so = live ? (uint32_t)(yr * P.ip) : kOob;
         R.v[0] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.base4, so, 0);
         R.v[1] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.base4 + 4, so, 0);
         R.v[2] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.direct, so, 0);
@@ -228,13 +233,20 @@ template <int MODE>
 __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>& R, uint32_t& hin, uint32_t& out)
 {
     if constexpr (MODE == 2) {
-        hin = 0;
+        /* ((256-fy)((256-fx)a + fx b) + fy((256-fx)c + fx d) + 32768) >> 16 is
+           exact integer arithmetic, so the rows are combined first, both
+           columns at once in 16-bit lanes (each sum <= 255 * 256), then the
+           columns by a dot product; the result (< 2^24) is byte 2 */
+        const unsigned short wa = (unsigned short)(256 - R.fy), wb = (unsigned short)R.fy;
+        uint32_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const int top = (256 - P.fx[k]) * (int)R.v[4 * k + 0] + P.fx[k] * (int)R.v[4 * k + 1];
-            const int bot = (256 - P.fx[k]) * (int)R.v[4 * k + 2] + P.fx[k] * (int)R.v[4 * k + 3];
-            hin |= (uint32_t)(((256 - R.fy) * top + R.fy * bot + 32768) >> 16) << (8 * k);
+            const uint32_t ab = __builtin_amdgcn_perm(R.v[2 * k], R.v[2 * k], P.xb[k]);
+            const uint32_t cd = __builtin_amdgcn_perm(R.v[2 * k + 1], R.v[2 * k + 1], P.xb[k]);
+            const pyr_u16x2 col = __builtin_bit_cast(pyr_u16x2, ab) * wa + __builtin_bit_cast(pyr_u16x2, cd) * wb;
+            x[k] = __builtin_amdgcn_udot2(col, __builtin_bit_cast(pyr_u16x2, P.wx[k]), 32768u, false);
         }
+        hin = __builtin_amdgcn_perm(x[1], x[0], 0x0c0c0602u) | __builtin_amdgcn_perm(x[3], x[2], 0x06020c0cu);
         out = hin & ~P.padmask;
     } else if constexpr (MODE == 1) {
         hin = R.v[0] | (R.v[1] << 8) | (R.v[2] << 16) | (R.v[3] << 24);
@@ -252,9 +264,9 @@ __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>&
    (reflect-101 at the level's top and bottom): per row it forms the 4 level
    pixels (copy of the input, scale-2 decimation, or 8-bit bilinear from the
    input), takes its neighbours' pixels across lanes, runs the horizontal
-   7-tap pass and keeps the last 7 results in registers, from which the
+   7-tap pass and keeps the last 8 results in registers, from which the
    vertical pass emits the blurred row 3 above.  The loads of a row are issued
-   two rows ahead; stores outside the strip's rows or lanes go to kOob.
+   three rows ahead; stores outside the strip's rows or lanes go to kOob.
    Strips cover the whole pitch: row padding is written as the input's bytes
    at level 0 and as 0 elsewhere, in both the raw and the blurred level
    (orbd.cu:514-535, 1273-1338; blur out of place). */
@@ -265,24 +277,34 @@ __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int 
 {
     const int lane = threadIdx.x & 63;
     const int addr_up = ((lane + 63) & 63) << 2, addr_dn = ((lane + 1) & 63) << 2;
-    /* rows walked: the strip's rows plus 3 above and below, in steps of 7 */
+    /* rows walked: the strip's rows plus 3 above and below, in steps of 8 */
     const int nwalk = min(TH, h - ys) + 6;
     const int row_end = min(ys + TH, h);
     auto row_ty = [&](int i) { return __builtin_amdgcn_readlane(tyv, min(i, nwalk - 1)); };
     auto row_y = [&](int i) { return reflect101(ys - 3 + min(i, nwalk - 1), h); };
-    PyrRow<MODE> s0, s1;
-    pyr_load<MODE>(P, row_y(0), row_ty(0), s0);
-    pyr_load<MODE>(P, row_y(1), row_ty(1), s1);
-    float4 ring[7];
-    uint32_t outring[7];
-    for (int i0 = 0; i0 < nwalk; i0 += 7) {
+    /* walked row i is loaded into slot i % 4, three rows before it is used;
+       the loop is unrolled by 8, so every slot and ring entry is a fixed
+       register set, and at most 3 rows of loads and stores (38 vector memory
+       operations) are outstanding, inside what the wait counter can track */
+    PyrRow<MODE> st[4];
 #pragma unroll
-        for (int j = 0; j < 7; j++) {
+    for (int r = 0; r < 3; r++) {
+        /* in row order, as the waits in the loop assume */
+        __builtin_amdgcn_sched_barrier(0);
+        pyr_load<MODE>(P, row_y(r), row_ty(r), r < nwalk, st[r]);
+    }
+    float4 ring[8];
+    uint32_t outring[8];
+    for (int i0 = 0; i0 < nwalk; i0 += 8) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
             const int i = i0 + j;
-            PyrRow<MODE> sn;
-            pyr_load<MODE>(P, row_y(i + 2), row_ty(i + 2), sn);
+            /* keep the prefetch where it is written: the scheduler would sink it */
+            __builtin_amdgcn_sched_barrier(0);
+            pyr_load<MODE>(P, row_y(i + 3), row_ty(i + 3), i + 3 < nwalk, st[(j + 3) % 4]);
+            __builtin_amdgcn_sched_barrier(0);
             uint32_t hin, rawout;
-            pyr_pixels<MODE>(P, s0, hin, rawout);
+            pyr_pixels<MODE>(P, st[j % 4], hin, rawout);
             const uint32_t lf = lane_get(hin, addr_up), rt = lane_get(hin, addr_dn);
             const int rr = ys - 3 + i;
             outring[j] = rawout;
@@ -309,28 +331,29 @@ __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int 
             ring[j] = hv;
             /* vertical pass for row rr - 3 from ring rows rr-6 .. rr */
             const int o = rr - 3;
-            const float* c0 = (const float*)&ring[(j + 1) % 7];
-            const float* c1 = (const float*)&ring[(j + 2) % 7];
-            const float* c2 = (const float*)&ring[(j + 3) % 7];
-            const float* c3 = (const float*)&ring[(j + 4) % 7];
-            const float* c4 = (const float*)&ring[(j + 5) % 7];
-            const float* c5 = (const float*)&ring[(j + 6) % 7];
+            const float* c0 = (const float*)&ring[(j + 2) % 8];
+            const float* c1 = (const float*)&ring[(j + 3) % 8];
+            const float* c2 = (const float*)&ring[(j + 4) % 8];
+            const float* c3 = (const float*)&ring[(j + 5) % 8];
+            const float* c4 = (const float*)&ring[(j + 6) % 8];
+            const float* c5 = (const float*)&ring[(j + 7) % 8];
             const float* c6 = (const float*)&ring[j];
-            uint32_t out = 0;
+            /* 0 <= a < 255.5: a + 1.5 * 2^23 rounds a half to even into the
+               low mantissa byte, which is the blurred pixel */
+            uint32_t rb[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 float a = __fmul_rn(T.k0, c3[k]);
                 a = __fmaf_rn(T.k1, __fadd_rn(c2[k], c4[k]), a);
                 a = __fmaf_rn(T.k2, __fadd_rn(c1[k], c5[k]), a);
                 a = __fmaf_rn(T.k3, __fadd_rn(c0[k], c6[k]), a);
-                out |= (uint32_t)(uint8_t)__float2int_rn(a) << (8 * k);
+                rb[k] = __float_as_uint(__fadd_rn(a, 12582912.0f));
             }
+            uint32_t out = __builtin_amdgcn_perm(rb[1], rb[0], 0x0c0c0400u) | __builtin_amdgcn_perm(rb[3], rb[2], 0x04000c0cu);
             /* padding columns of the blurred row keep the raw row's bytes */
-            out = (out & ~P.padmask) | (outring[(j + 4) % 7] & P.padmask);
+            out = (out & ~P.padmask) | (outring[(j + 5) % 8] & P.padmask);
             const uint32_t boff = (o >= ys && o < row_end) ? (uint32_t)(o * p) : kOob;
             __builtin_amdgcn_raw_buffer_store_b32(out, blur, boff + cxw, 0, 0);
-            s0 = s1;
-            s1 = sn;
         }
     }
 }
@@ -340,6 +363,7 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
                                                  int njobs, int nunits, Tables T)
 {
     static_assert(TH + 6 <= 64, "one lane per walked row holds its bilinear row entry");
+    static_assert((TH + 6) % 8 == 0, "a full strip walks whole unrolled groups of 8 rows");
     const int lane = threadIdx.x & 63;
     const int u = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     if (u >= nunits) return;
@@ -368,15 +392,19 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
         if (x >= w) P.padmask |= 0xffu << (8 * k);
         if (l == 0 && x >= w && x < F.ip) P.ipmask |= 0xffu << (8 * k);
         if (mode == 2) {
+            /* the dword at min(x0, w0 - 4) ends inside the input row; at x0 = w0 - 1
+               both taps are that pixel, which fx = 0 gives exactly */
             const int v = tab[xr[k]];
-            P.xa[k] = (uint32_t)(v >> 8);
-            P.fx[k] = v & 255;
-            P.xb[k] = (uint32_t)min((v >> 8) + 1, F.w0 - 1);
+            const int x0 = v >> 8, fx = x0 == F.w0 - 1 ? 0 : (v & 255);
+            const int d = min(x0, F.w0 - 4), sh = x0 - d;
+            P.xa[k] = (uint32_t)d;
+            P.xb[k] = (uint32_t)sh | 0x0c00u | (uint32_t)(sh + 1) << 16 | 0x0c000000u;
+            P.wx[k] = (uint32_t)(256 - fx) | (uint32_t)fx << 16;
         } else {
             P.xa[k] = (uint32_t)(xr[k] << (mode == 1 ? l : 0));
             /* MODE 1 padding bytes (level 0 only): the input byte at the column itself */
             P.xb[k] = (l == 0 && x >= w && x < F.ip) ? (uint32_t)x : kOob;
-            P.fx[k] = 0;
+            P.wx[k] = 0;
         }
     }
     if (mode == 0) {

@@ -43,7 +43,7 @@ ORB_LINDEX_MLP = 0
 ORB_LINDEX_CENTROIDS = 1
 
 TUNE_PYRAMID = 0
-PYRAMID_STRIP_HEIGHTS = (15, 22, 29, 36, 50)
+PYRAMID_STRIP_HEIGHTS = (18, 26, 34, 42, 58)
 NUM_STAGES = 8
 STAGES = ["pyramid", "detect", "nms", "select", "emit", "describe", "match", "match_merge"]
 
