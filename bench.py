@@ -69,7 +69,7 @@ COUNTERS_FILE = ROOT / "profiles" / "orb_1080p_pair_hbm_counters.md"
 def stage_kernel(stage, strip_height):
     """rocprofv3 kernel name of a stage's launch."""
     return {"pyramid": f"void k_pyramid<{strip_height}>", "detect": "k_detect", "nms": "k_nms", "select": "k_select",
-            "describe": "k_describe", "match": "k_match_pop", "match_merge": "k_match_merge"}.get(stage)
+            "describe": "k_describe", "match": "void k_match_pop<false>", "match_merge": "k_match_merge"}.get(stage)
 
 
 def measured_traffic(kernel):
@@ -375,13 +375,20 @@ def main():
         ctx.detect_and_compute_batch(imgs, [W, W], [H, H], [pts[0], pts[1]], [desc[0], desc[1]], counts)
         ctx.match_dcount(pts[0], desc[0], counts[0:1], CAP, desc[1], counts[1:2], CAP)
 
-    # warm-up with every stage timed, to find the dominant kernel
-    ctx.profile_enable((1 << orb_amd.NUM_STAGES) - 1)
+    # warm-up, then every stage timed on its own (one event pair per step, so the host keeps
+    # ahead of the GPU and no launch gap falls inside a pair), to find the dominant kernel
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    prof = ctx.profile_read()
-    stage_ms = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in prof.items()}
+    stage_ms = {}
+    for si, name in enumerate(orb_amd.STAGES):
+        ctx.profile_enable(1 << si)
+        for _ in range(max(args.warmup, 10)):
+            step()
+        torch.cuda.synchronize()
+        v = ctx.profile_read()[name]
+        stage_ms[name] = v[0] / v[1] if v[1] else 0.0
+    ctx.profile_enable(0)
     dominant = max(stage_ms, key=stage_ms.get)
     n1, n2 = (int(v) for v in counts.cpu())
 

@@ -681,7 +681,8 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
     if (ndb >= (1 << 23)) return fail(c, ORB_E_CAPACITY, "database of %d entries exceeds 2^23-1", ndb);
     if (nq == 0) return ORB_OK;
     if (!q || (ndb > 0 && !db)) return fail(c, ORB_E_ARG, "null descriptor pointer");
-    if (method != ORB_MATCH_POPCOUNT && method != ORB_MATCH_MFMA) return fail(c, ORB_E_ARG, "unknown method %d", method);
+    if (method != ORB_MATCH_POPCOUNT && method != ORB_MATCH_MFMA && method != ORB_MATCH_HAMMING2)
+        return fail(c, ORB_E_ARG, "unknown method %d", method);
     MatchLaunch m;
     memset(&m, 0, sizeof(m));
     const size_t words = match_partial_words(nq, ndb, method, &m.nchunks, &m.chunk);
@@ -699,6 +700,7 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
     m.idx2 = idx2;
     m.d2 = d2;
     m.pts = pts;
+    m.hamming2 = method == ORB_MATCH_HAMMING2;
     StageEvents ev;
     memset(&ev, 0, sizeof(ev));
     const StageEvents* pev = take_events(c, ev, 0xc0u);
@@ -937,6 +939,31 @@ int orb_match_top2(orb_ctx* c, const uint8_t* q, int nq, const uint8_t* db, int 
 {
     if (!c) return fail(nullptr, ORB_E_ARG, "null context");
     return run_match(c, q, nq, nullptr, db, ndb, nullptr, method, idx1, d1, idx2, d2, nullptr);
+}
+
+int orb_export_keypoints(orb_ctx* c, const orb::OrbPoint* pts, const uint8_t* desc, const int* d_n, int cap,
+                         orb_cv_keypoint* kp, uint8_t* desc_out, int* d_nout)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (!c->inited) return fail(c, ORB_E_STATE, "orb_init has not been called");
+    if (cap < 0) return fail(c, ORB_E_ARG, "negative capacity");
+    if (cap == 0) return ORB_OK;
+    if (!pts || !kp) return fail(c, ORB_E_ARG, "null point pointer");
+    ExportArgs A;
+    memset(&A, 0, sizeof(A));
+    const double sf = c->prm.scale_factor;
+    A.ref_mode = sf == 2.0;
+    A.levels = c->prm.noctaves < kMaxLevels ? c->prm.noctaves : kMaxLevels;
+    A.patch = (float)c->prm.patch_size;
+    double sc = 1.0;
+    for (int l = 0; l < kMaxLevels; l++) {
+        if (l > 0) sc *= sf;   /* the product detection uses for s^l */
+        A.sc[l] = A.ref_mode ? (double)(1 << l) : sc;
+        A.lsc[l] = (float)pow(sf, (double)l);
+    }
+    hipError_t e = launch_export(pts, desc, d_n, cap, c->prm.retain_topn, A, kp, desc_out, d_nout, c->stream);
+    if (e != hipSuccess) return hip_fail(c, e, "export launch");
+    return ORB_OK;
 }
 
 int orb_sync(orb_ctx* c)

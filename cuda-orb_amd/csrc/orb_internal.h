@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "orb_capi.h"
 #include "orb_structures.h"
 
 namespace orbk
@@ -129,6 +130,15 @@ struct StageEvents {
 hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s, const StageEvents* ev);
 /* the pyramid stage alone with strip-height variant `variant` (autotuner) */
 hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s);
+/* OpenCV-layout export: per-level scales of the context's geometry */
+struct ExportArgs {
+    double sc[kMaxLevels];  /* s^l: level -> level-0 coordinate, as detection rounds it */
+    float lsc[kMaxLevels];  /* OpenCV layerScale: (float)pow(scale, l) */
+    float patch;            /* patch size */
+    int ref_mode, levels;
+};
+hipError_t launch_export(const orb::OrbPoint* pts, const uint8_t* desc, const int* d_n, int cap, int retain,
+                         const ExportArgs& A, orb_cv_keypoint* kp, uint8_t* desc_out, int* d_nout, hipStream_t s);
 hipError_t launch_debug_sincos(const float* in, int n, float* s, float* c, hipStream_t st);
 hipError_t launch_debug_atan2(const float* yx, int n, float* out, hipStream_t st);
 
@@ -139,6 +149,7 @@ struct MatchLaunch {
     uint32_t* partial; int nchunks; int chunk;
     int* idx1; int* d1; int* idx2; int* d2;   /* top-2 outputs (may be null) */
     orb::OrbPoint* pts;                       /* reference-rule output (may be null) */
+    bool hamming2;                            /* popcount matcher: count differing bit pairs */
 };
 hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);

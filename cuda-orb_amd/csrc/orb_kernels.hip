@@ -603,35 +603,63 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     }
     __syncthreads();
     const int thr = T.threshold;
-    /* pass 1: compass test on every pixel of the tile (a bit per pixel of the
-       thread's column), then one workgroup-wide compaction */
+    /* pass 1: compass test on every pixel of the tile, 4 pixels (one LDS
+       dword) at a time, then one workgroup-wide compaction.  With q0, q4, q8,
+       q12 the compass points (ring (0,3), (3,0), (0,-3), (-3,0)), "two
+       neighbouring compass points both darker than v-t" is
+       max(min(q0,q8), min(q4,q12)) < v-t and "both brighter than v+t" is
+       min(max(q0,q8), max(q4,q12)) > v+t; the even and the odd bytes are
+       tested in the two 16-bit lanes of packed integer operations, where a
+       difference's sign bit is the comparison. */
     int ncand;
     {
-        constexpr int kRows = kDetTH * kDetTW / 256, kStep = 256 / kDetTW;
-        static_assert(kRows <= 32, "a bit per row of the thread's column");
-        const int lx = tid & (kDetTW - 1);
-        const int ix = x0 + lx;
-        const bool col_ok = ix >= 3 && ix < w - 3;
+        constexpr int kG = kDetTW / 4, kStep = 256 / kG, kRows = kDetTH / kStep;
+        static_assert(kRows * 4 <= 32, "a bit per pixel of the thread's rows");
+        const int lx = 4 * (tid & (kG - 1));
+        uint32_t colm = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) colm |= (uint32_t)(x0 + lx + j >= 3 && x0 + lx + j < w - 3) << j;
+        const pyr_u16x2 t2 = {(unsigned short)thr, (unsigned short)thr};
         uint32_t m = 0;
 #pragma unroll
         for (int rr = 0; rr < kRows; rr++) {
-            const int ly = tid / kDetTW + kStep * rr;
+            const int ly = tid / kG + kStep * rr;
             const int iy = y0 + ly;
             const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx + 4];
-            const int v = c[0];
-            const int q0 = c[RING_OFF(0, 3)], q4 = c[RING_OFF(3, 0)], q8 = c[RING_OFF(0, -3)],
-                      q12 = c[RING_OFF(-3, 0)];
-            const int lo = v - thr, hi = v + thr;
-            const bool d0 = q0 < lo, d4 = q4 < lo, d8 = q8 < lo, d12 = q12 < lo;
-            const bool b0 = q0 > hi, b4 = q4 > hi, b8 = q8 > hi, b12 = q12 > hi;
-            const bool dark = (d0 && d4) || (d4 && d8) || (d8 && d12) || (d12 && d0);
-            const bool bright = (b0 && b4) || (b4 && b8) || (b8 && b12) || (b12 && b0);
-            m |= (uint32_t)(col_ok && iy >= 3 && iy < h - 3 && (dark || bright)) << rr;
+            const uint32_t C = *(const uint32_t*)c;
+            const uint32_t Lw = *(const uint32_t*)(c - 4), Rw = *(const uint32_t*)(c + 4);
+            const uint32_t U = *(const uint32_t*)(c - 3 * kDetSW), D = *(const uint32_t*)(c + 3 * kDetSW);
+            const uint32_t L = __builtin_amdgcn_alignbyte(C, Lw, 1);  /* bytes at x - 3 */
+            const uint32_t R = __builtin_amdgcn_alignbyte(Rw, C, 3);  /* bytes at x + 3 */
+            uint32_t hit[2];
+#pragma unroll
+            for (int half = 0; half < 2; half++) {
+                /* bytes 0,2 (half 0) or 1,3 (half 1) as two 16-bit lanes */
+                const uint32_t sel = half ? 0x0c030c01u : 0x0c020c00u;
+                const pyr_u16x2 v = __builtin_bit_cast(pyr_u16x2, __builtin_amdgcn_perm(C, C, sel));
+                const pyr_u16x2 q0 = __builtin_bit_cast(pyr_u16x2, __builtin_amdgcn_perm(D, D, sel));
+                const pyr_u16x2 q4 = __builtin_bit_cast(pyr_u16x2, __builtin_amdgcn_perm(R, R, sel));
+                const pyr_u16x2 q8 = __builtin_bit_cast(pyr_u16x2, __builtin_amdgcn_perm(U, U, sel));
+                const pyr_u16x2 q12 = __builtin_bit_cast(pyr_u16x2, __builtin_amdgcn_perm(L, L, sel));
+                const pyr_u16x2 mn = __builtin_elementwise_max(__builtin_elementwise_min(q0, q8),
+                                                               __builtin_elementwise_min(q4, q12));
+                const pyr_u16x2 mx = __builtin_elementwise_min(__builtin_elementwise_max(q0, q8),
+                                                               __builtin_elementwise_max(q4, q12));
+                /* mn + t - v < 0 (darker), v + t - mx < 0 (brighter): 16-bit
+                   differences in [-255, 510], sign bits 15 and 31 */
+                const uint32_t dk = __builtin_bit_cast(uint32_t, (pyr_u16x2)(mn + t2 - v));
+                const uint32_t br = __builtin_bit_cast(uint32_t, (pyr_u16x2)(v + t2 - mx));
+                hit[half] = (dk | br) & 0x80008000u;
+            }
+            /* pixel j of the dword -> bit j */
+            const uint32_t tb = (hit[0] >> 15) | (hit[1] >> 14);
+            const uint32_t nib = (tb | (tb >> 14)) & 15u;
+            m |= (iy >= 3 && iy < h - 3 ? nib & colm : 0u) << (4 * rr);
         }
         int off = block_scan<256>(__popc(m), s_scan, ncand);
         for (; m; m &= m - 1) {
-            const int rr = __ffs(m) - 1;
-            s_cand[off++] = (uint16_t)(((tid / kDetTW + kStep * rr) << 8) | lx);
+            const int bit = __ffs(m) - 1;
+            s_cand[off++] = (uint16_t)(((tid / kG + kStep * (bit >> 2)) << 8) | (lx + (bit & 3)));
         }
     }
     __syncthreads();
@@ -925,6 +953,111 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
     }
 }
 
+/* ------------------------------------------------------ OpenCV-layout export */
+
+/* One workgroup converts a keypoint set (raster order, as detection writes it)
+   to OpenCV's cv::KeyPoint layout: pt = level coordinate x layerScale,
+   layerScale = (float)pow(scale, level), size = patch_size x layerScale, angle
+   in degrees [0, 360), response = score, octave = level, class_id = -1.  With
+   retain > 0 and more points than that, the `retain` best by score are kept
+   (ties in raster order, the rule of the max_pts cut) by the same exact radix
+   select as k_select; the kept points stay in raster order and their
+   descriptors follow them.  Scores are positive, so their bits order them. */
+__global__ __launch_bounds__(1024) void k_export(const orb::OrbPoint* __restrict__ pts,
+                                                 const uint8_t* __restrict__ desc, const int* __restrict__ d_n,
+                                                 int cap, int retain, ExportArgs A, orb_cv_keypoint* __restrict__ kp,
+                                                 uint8_t* __restrict__ desc_out, int* __restrict__ d_nout)
+{
+    __shared__ int s_tmp[32];
+    __shared__ int s_hist[256];
+    __shared__ int s_pick[2];
+    const int tid = threadIdx.x;
+    const int n = d_n ? min(max(*d_n, 0), cap) : cap;
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, tid * per), i1 = min(n, i0 + per);
+    const bool cut = retain > 0 && n > retain;
+    uint32_t thr = 0;
+    int need = 0;
+    if (cut) {
+        uint32_t prefix = 0, pmask = 0;
+        int k = retain;
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            for (int i = tid; i < 256; i += 1024) s_hist[i] = 0;
+            __syncthreads();
+            for (int i = i0; i < i1; i++) {
+                const uint32_t sv = __float_as_uint(pts[i].score);
+                if ((sv & pmask) == prefix) atomicAdd(&s_hist[(sv >> shift) & 255u], 1);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int acc = 0, d = 255;
+                for (; d > 0; d--) {
+                    if (acc + s_hist[d] >= k) break;
+                    acc += s_hist[d];
+                }
+                s_pick[0] = d;
+                s_pick[1] = k - acc;
+            }
+            __syncthreads();
+            prefix |= (uint32_t)s_pick[0] << shift;
+            pmask |= 255u << shift;
+            k = s_pick[1];
+            __syncthreads();
+        }
+        thr = prefix;
+        need = k;
+    }
+    auto kept = [&](uint32_t sv, int& eq_seen) {
+        if (!cut || sv > thr) return true;
+        return sv == thr && eq_seen++ < need;
+    };
+    /* equal-to-threshold points before this thread's range, then the output slots */
+    int eq = 0, tot;
+    if (cut)
+        for (int i = i0; i < i1; i++) eq += __float_as_uint(pts[i].score) == thr;
+    int eq_before = block_scan<1024>(eq, s_tmp, tot);
+    int mine = 0;
+    {
+        int e = eq_before;
+        for (int i = i0; i < i1; i++) mine += kept(__float_as_uint(pts[i].score), e);
+    }
+    int total;
+    int out = block_scan<1024>(mine, s_tmp, total);
+    for (int i = i0; i < i1; i++) {
+        const orb::OrbPoint P = pts[i];
+        if (!kept(__float_as_uint(P.score), eq_before)) continue;
+        const int l = min(max(P.octave, 0), A.levels - 1);
+        int lx, ly;
+        if (A.ref_mode) {
+            lx = P.x >> l;
+            ly = P.y >> l;
+        } else {
+            lx = (int)floor(__dadd_rn(__ddiv_rn((double)P.x, A.sc[l]), 0.5));
+            ly = (int)floor(__dadd_rn(__ddiv_rn((double)P.y, A.sc[l]), 0.5));
+        }
+        orb_cv_keypoint o;
+        o.x = __fmul_rn((float)lx, A.lsc[l]);
+        o.y = __fmul_rn((float)ly, A.lsc[l]);
+        o.size = __fmul_rn(A.patch, A.lsc[l]);
+        float deg = __fmul_rn(P.angle, 57.2957795130823208768f);
+        if (deg < 0.f) deg = __fadd_rn(deg, 360.f);
+        if (deg >= 360.f) deg = __fsub_rn(deg, 360.f);
+        o.angle = deg;
+        o.response = P.score;
+        o.octave = l;
+        o.class_id = -1;
+        kp[out] = o;
+        if (desc && desc_out) {
+            const uint4* src = (const uint4*)(desc + (long long)i * 32);
+            uint4* dst = (uint4*)(desc_out + (long long)out * 32);
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
+        out++;
+    }
+    if (tid == 0 && d_nout) *d_nout = total;
+}
+
 /* --------------------------------------------------------- angle + describe */
 
 /* staged patches per wavefront: the raw 31-row disc region (36-byte rows) and
@@ -1151,6 +1284,14 @@ hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s,
                            d.desc_blocks_per_frame, d.t, compute_desc);
         ev_stop(ev, kStDescribe, s);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_export(const orb::OrbPoint* pts, const uint8_t* desc, const int* d_n, int cap, int retain,
+                         const ExportArgs& A, orb_cv_keypoint* kp, uint8_t* desc_out, int* d_nout, hipStream_t s)
+{
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(k_export, dim3(1), dim3(1024), 0, s, pts, desc, d_n, cap, retain, A, kp, desc_out, d_nout);
     return hipGetLastError();
 }
 

@@ -54,9 +54,19 @@ __host__ __device__ inline Plan make_plan(int nq, int ndb, int qtile, int slots)
 
 /* ----------------------------------------------------------- popcount */
 
+/* distance contribution of one 32-bit word of a ^ b: its set bits, or with
+   H2 its set bit pairs (bits 2i, 2i+1), OpenCV's NORM_HAMMING2 */
+template <bool H2>
+__device__ __forceinline__ uint32_t word_dist(uint32_t x)
+{
+    if constexpr (H2) x = (x | (x >> 1)) & 0x55555555u;
+    return __popc(x);
+}
+
 /* A workgroup unit is 64 queries x one chunk; its 4 wavefronts scan a
    quarter of the chunk each and merge through LDS, so small problems still
    spread over many wavefronts without multiplying the merge work. */
+template <bool H2>
 __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q, int nq_cap,
                                                    const int* __restrict__ d_nq, const uint4* __restrict__ db,
                                                    int ndb_cap, const int* __restrict__ d_ndb, int slots,
@@ -80,14 +90,14 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
         const int j0 = min(ndb, c * P.chunk + wid * sub), j1 = min(ndb, j0 + sub);
         uint32_t k1 = kNoKey, k2 = kNoKey;
         auto pair = [&](uint4 a, uint4 b, int j) {
-            uint32_t d = __popc(qa.x ^ a.x);
-            d = __popc(qa.y ^ a.y) + d;
-            d = __popc(qa.z ^ a.z) + d;
-            d = __popc(qa.w ^ a.w) + d;
-            d = __popc(qb.x ^ b.x) + d;
-            d = __popc(qb.y ^ b.y) + d;
-            d = __popc(qb.z ^ b.z) + d;
-            d = __popc(qb.w ^ b.w) + d;
+            uint32_t d = word_dist<H2>(qa.x ^ a.x);
+            d = word_dist<H2>(qa.y ^ a.y) + d;
+            d = word_dist<H2>(qa.z ^ a.z) + d;
+            d = word_dist<H2>(qa.w ^ a.w) + d;
+            d = word_dist<H2>(qb.x ^ b.x) + d;
+            d = word_dist<H2>(qb.y ^ b.y) + d;
+            d = word_dist<H2>(qb.z ^ b.z) + d;
+            d = word_dist<H2>(qb.w ^ b.w) + d;
             top2_push(k1, k2, (d << kIdxBits) | (uint32_t)j);
         };
         int j = j0;
@@ -299,8 +309,12 @@ hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const Stag
     int blocks = (int)(units < kPopBlocks ? units : kPopBlocks);
     if (blocks < 1) blocks = 1;
     ev_start(ev, kStMatch, s);
-    hipLaunchKernelGGL(k_match_pop, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db, m.ndb,
-                       m.d_ndb, slots, (uint2*)m.partial);
+    if (m.hamming2)
+        hipLaunchKernelGGL(k_match_pop<true>, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db,
+                           m.ndb, m.d_ndb, slots, (uint2*)m.partial);
+    else
+        hipLaunchKernelGGL(k_match_pop<false>, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db,
+                           m.ndb, m.d_ndb, slots, (uint2*)m.partial);
     ev_stop(ev, kStMatch, s);
     launch_merge(m, kPopQ, slots, s, ev);
     return hipGetLastError();

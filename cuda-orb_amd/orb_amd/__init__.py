@@ -19,6 +19,7 @@ LIB_PATH = HERE / "liborb_amd.so"
 ORB_OK = 0
 ORB_MATCH_POPCOUNT = 0
 ORB_MATCH_MFMA = 1
+ORB_MATCH_HAMMING2 = 2
 HARRIS_SCORE = 0
 FAST_SCORE = 1
 
@@ -27,10 +28,15 @@ POINT_DTYPE = np.dtype([("x", "<i4"), ("y", "<i4"), ("octave", "<i4"), ("score",
                         ("angle", "<f4"), ("match", "<i4"), ("distance", "<i4")])
 assert POINT_DTYPE.itemsize == 28
 
+# orb_cv_keypoint: the memory layout of OpenCV's cv::KeyPoint (28 bytes)
+CV_KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                              ("response", "<f4"), ("octave", "<i4"), ("class_id", "<i4")])
+assert CV_KEYPOINT_DTYPE.itemsize == 28
+
 EXPORTS = [
     "orb_ctx_create", "orb_ctx_destroy", "orb_set_stream", "orb_init", "orb_num_levels",
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
-    "orb_match_top2", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
+    "orb_match_top2", "orb_export_keypoints", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
     "orb_tune_set", "orb_tune_get", "orb_debug_pyramid", "orb_capture_begin", "orb_capture_end",
     "orb_graph_launch", "orb_graph_destroy",
@@ -86,6 +92,7 @@ def lib():
         L.orb_match.argtypes = [P, V, V, I, V, I]
         L.orb_match_dcount.argtypes = [P, V, V, V, I, V, V, I]
         L.orb_match_top2.argtypes = [P, V, I, V, I, I, V, V, V, V]
+        L.orb_export_keypoints.argtypes = [P, V, V, V, I, V, V, V]
         L.orb_sync.argtypes = [P]
         L.orb_last_error.argtypes = [P]
         L.orb_last_error.restype = ctypes.c_char_p
@@ -249,6 +256,25 @@ class Context:
         self._check(lib().orb_match_top2(self.h, _dptr(q), nq, _dptr(db), db.shape[0], method,
                                          *[_dptr(o) for o in out]), "orb_match_top2")
         return out
+
+    def export_keypoints(self, pts, desc=None, count=None, kp=None, desc_out=None, nout=None):
+        """OpenCV-layout export (orb_export_keypoints) of the points in pts ((cap, 7) int32, device),
+        count a device int32 tensor (None: all cap rows), honouring the retain_topn given to init.
+        Returns (kp (cap, 7) int32 tensor holding orb_cv_keypoint rows, desc_out or None, nout
+        device int32 tensor of 1 element).  Asynchronous."""
+        self._bind_stream()
+        torch = self._torch
+        cap = pts.shape[0]
+        if kp is None:
+            kp = torch.empty((cap, 7), dtype=torch.int32, device=pts.device)
+        if desc is not None and desc_out is None:
+            desc_out = torch.empty((cap, 32), dtype=torch.uint8, device=pts.device)
+        if nout is None:
+            nout = torch.zeros(1, dtype=torch.int32, device=pts.device)
+        self._check(lib().orb_export_keypoints(self.h, _dptr(pts), _dptr(desc), _dptr(count), cap, _dptr(kp),
+                                               _dptr(desc_out) if desc is not None else None, _dptr(nout)),
+                    "orb_export_keypoints")
+        return kp, desc_out, nout
 
     def sync(self):
         self._check(lib().orb_sync(self.h), "orb_sync")
@@ -449,6 +475,12 @@ def points_to_numpy(pts_tensor, n):
     """First n OrbPoints of an (N,7) int32 device tensor as a structured array."""
     a = pts_tensor[:n].contiguous().cpu().numpy()
     return a.view(POINT_DTYPE).reshape(-1)
+
+
+def as_cv_keypoints(kp):
+    """(n, 7) int32 tensor/array of orb_cv_keypoint rows -> numpy structured array (CV_KEYPOINT_DTYPE)."""
+    a = kp.cpu().numpy() if hasattr(kp, "cpu") else np.asarray(kp)
+    return np.ascontiguousarray(a, dtype=np.int32).view(CV_KEYPOINT_DTYPE).reshape(-1)
 
 
 class Orbor:

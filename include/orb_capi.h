@@ -43,7 +43,7 @@ typedef struct {
     int score_type;       /* HARRIS_SCORE or FAST_SCORE (orb.h:47) */
     int patch_size;       /* 31 (orb.h:49) */
     int fast_threshold;   /* FAST intensity threshold (orb.h:51) */
-    int retain_topn;      /* stored, unused, as in the reference (orb.cpp:41) */
+    int retain_topn;      /* unused by detection, as in the reference (orb.cpp:41); orb_export_keypoints honours it */
     int max_pts;          /* keypoints kept per image (orb.h:55) */
     double scale_factor;  /* 2.0: reference octaves; (1,2): bilinear levels */
 } orb_params;
@@ -104,15 +104,42 @@ int orb_match_dcount(orb_ctx* ctx, ORB_POINT* d_pts1, const uint8_t* d_desc1, co
 
 #define ORB_MATCH_POPCOUNT 0 /* VALU xor + popcount */
 #define ORB_MATCH_MFMA 1     /* +/-1 INT8 contraction on the matrix cores */
+#define ORB_MATCH_HAMMING2 2 /* VALU, OpenCV's NORM_HAMMING2: differing 2-bit pairs (WTA_K 3/4 codes) */
 
 /* Exact brute-force nearest and second-nearest neighbours of nq query
    descriptors in a database of ndb (both 32-byte rows).  Ties go to the
    lowest index: (d1, idx1) is the minimum of (distance, index), (d2, idx2)
    the minimum over the other entries; with no entry idx = -1, d = 257.
    Any output pointer may be NULL.  Generalises gHammingMatch (orbd.cu:1471)
-   to raw top-2 output. */
+   to raw top-2 output.  method: ORB_MATCH_POPCOUNT and ORB_MATCH_MFMA give
+   identical Hamming results; ORB_MATCH_HAMMING2 counts the 128 bit pairs
+   (bits 2i, 2i+1) that differ, OpenCV's norm for WTA_K 3/4 descriptors. */
 int orb_match_top2(orb_ctx* ctx, const uint8_t* d_query, int nq, const uint8_t* d_db, int ndb,
                    int method, int* d_idx1, int* d_d1, int* d_idx2, int* d_d2);
+
+/* A keypoint in the memory layout of OpenCV's cv::KeyPoint (28 bytes). */
+typedef struct {
+    float x, y;     /* pt: level coordinate x layerScale, layerScale = (float)pow(scale_factor, octave) */
+    float size;     /* patch_size x layerScale */
+    float angle;    /* degrees in [0, 360) */
+    float response; /* the detection score */
+    int octave;     /* pyramid level */
+    int class_id;   /* -1 */
+} orb_cv_keypoint;
+
+/* OpenCV-layout export of a keypoint set written by orb_detect_and_compute
+   with this context's current parameters: d_n (device int, or NULL for
+   cap points) counts the points at d_pts, at most cap.  The retain_topn of
+   orb_init, stored and unused by detection as in the reference (orb.cpp:41),
+   is honoured here: when it is > 0 and below the count, the retain_topn
+   highest scores are kept (ties in raster order).  Kept points stay in raster
+   order; d_kp[0..*d_nout) receives them and, when d_desc and d_desc_out are
+   both non-NULL, d_desc_out their descriptor rows.  d_kp and d_desc_out hold
+   cap entries.  No reference counterpart: the reference hands back OrbPoints
+   (orb_structures.h:8-35) that its demo draws field by field
+   (main.cpp:27-38), beside a separate OpenCV ORB run (main.cpp:309-329). */
+int orb_export_keypoints(orb_ctx* ctx, const ORB_POINT* d_pts, const uint8_t* d_desc, const int* d_n, int cap,
+                         orb_cv_keypoint* d_kp, uint8_t* d_desc_out, int* d_nout);
 
 /* Wait for all work on the context's stream (the reference synchronises after
    every launch, orbd.cu:1598-1791). */

@@ -70,6 +70,7 @@ def lib():
         L.orc_detect_and_compute.argtypes = [ctypes.POINTER(Params), P, I, I, P, P, P, P]
         L.orc_match.argtypes = [P, I, P, I, P, P]
         L.orc_match_top2.argtypes = [P, I, P, I, I, P, P, P, P]
+        L.orc_match_top2_norm.argtypes = [P, I, P, I, I, I, P, P, P, P]
         L.orc_pattern.argtypes = [I, P]
         L.orc_glibc_srand.argtypes = [ctypes.c_uint]
         L.orc_umax.argtypes = [I, P]
@@ -165,12 +166,62 @@ def match(desc1, desc2):
     return m, d
 
 
-def match_top2(q, db, nthreads=1):
+def match_top2(q, db, nthreads=1, norm=0):
+    """Exact top-2 (idx1, d1, idx2, d2); norm 0 Hamming, 1 NORM_HAMMING2 (orbd.cu:1471-1559 generalised)."""
     q = np.ascontiguousarray(q, np.uint8)
     db = np.ascontiguousarray(db, np.uint8)
     out = [np.zeros(len(q), np.int32) for _ in range(4)]
-    lib().orc_match_top2(ptr(q), len(q), ptr(db), len(db), nthreads, *[ptr(o) for o in out])
+    lib().orc_match_top2_norm(ptr(q), len(q), ptr(db), len(db), nthreads, norm, *[ptr(o) for o in out])
     return tuple(out)  # idx1, d1, idx2, d2
+
+
+CV_KEYPOINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("size", "<f4"), ("angle", "<f4"),
+                              ("response", "<f4"), ("octave", "<i4"), ("class_id", "<i4")])
+
+
+def export_cv(pts, desc, retain_topn, scale, patch_size=31):
+    """OpenCV-layout export of points in raster order (the library's orb_export_keypoints).
+
+    Kept: all points, or with 0 < retain_topn < n the retain_topn best by score (ties to the
+    earlier point, the rule of the max_pts cut), in their original order.  Per point of level l:
+    the level coordinate is recovered from the level-0 one (x >> l at scale 2, else
+    floor(x / s^l + 0.5) with s^l the running product detection uses), pt = coordinate *
+    float32(pow(scale, l)) in float32, size = patch_size * that, angle = radians * 180/pi in
+    float32 moved into [0, 360), response = score, octave = l, class_id = -1."""
+    n = len(pts)
+    keep = np.ones(n, bool)
+    if 0 < retain_topn < n:
+        order = np.lexsort((np.arange(n), -pts["score"].astype(np.float64)))
+        keep[:] = False
+        keep[order[:retain_topn]] = True
+    idx = np.nonzero(keep)[0]
+    out = np.zeros(len(idx), CV_KEYPOINT_DTYPE)
+    ref = scale == 2.0
+    for o, i in enumerate(idx):
+        l = int(pts["octave"][i])
+        x, y = int(pts["x"][i]), int(pts["y"][i])
+        if ref:
+            lx, ly = x >> l, y >> l
+        else:
+            sc = 1.0
+            for _ in range(l):
+                sc *= scale
+            lx, ly = int(np.floor(x / sc + 0.5)), int(np.floor(y / sc + 0.5))
+        lsc = np.float32(scale ** l)
+        out["x"][o] = np.float32(lx) * lsc
+        out["y"][o] = np.float32(ly) * lsc
+        out["size"][o] = np.float32(patch_size) * lsc
+        deg = np.float32(pts["angle"][i]) * np.float32(57.2957795130823208768)
+        if deg < 0:
+            deg = deg + np.float32(360.0)
+        if deg >= 360:
+            deg = deg - np.float32(360.0)
+        out["angle"][o] = deg
+        out["response"][o] = pts["score"][i]
+        out["octave"][o] = l
+        out["class_id"][o] = -1
+    d = None if desc is None else np.ascontiguousarray(desc[idx])
+    return out, d
 
 
 def fast_atan2(y, x):

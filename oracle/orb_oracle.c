@@ -708,11 +708,25 @@ void orc_match(const uint8_t* d1, int n1, const uint8_t* d2, int n2, int* match,
     }
 }
 
+/* OpenCV's NORM_HAMMING2 on 256-bit rows: the number of the 128 bit pairs
+   (bits 2i, 2i+1 of the little-endian row) in which a and b differ, counted
+   pair by pair */
+static inline int ham2(const uint8_t* a, const uint8_t* b)
+{
+    int d = 0;
+    for (int i = 0; i < 32; i++) {
+        const int x = a[i] ^ b[i];
+        for (int k = 0; k < 8; k += 2) d += ((x >> k) & 3) != 0;
+    }
+    return d;
+}
+
 /* exact nearest and second nearest, ties broken by the lowest index:
    (d1, idx1) is the lexicographic minimum of (distance, index), (d2, idx2) the
-   minimum over the other entries.  With no second entry idx2 = -1, d2 = 257. */
-void orc_match_top2(const uint8_t* q, int nq, const uint8_t* db, int ndb, int nthreads,
-                    int* idx1, int* d1, int* idx2, int* d2)
+   minimum over the other entries.  With no second entry idx2 = -1, d2 = 257.
+   norm 0: Hamming distance; norm 1: NORM_HAMMING2 (ham2). */
+void orc_match_top2_norm(const uint8_t* q, int nq, const uint8_t* db, int ndb, int nthreads, int norm,
+                         int* idx1, int* d1, int* idx2, int* d2)
 {
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
@@ -724,10 +738,16 @@ void orc_match_top2(const uint8_t* q, int nq, const uint8_t* db, int ndb, int nt
         int b1 = 257, b2 = 257, j1 = -1, j2 = -1;
         const uint8_t* qi = q + 32L * i;
         for (int j = 0; j < ndb; j++) {
-            int d = ham(qi, db + 32L * j);
+            int d = norm ? ham2(qi, db + 32L * j) : ham(qi, db + 32L * j);
             if (d < b1) { b2 = b1; j2 = j1; b1 = d; j1 = j; }
             else if (d < b2) { b2 = d; j2 = j; }
         }
         idx1[i] = j1; d1[i] = b1; idx2[i] = j2; d2[i] = b2;
     }
+}
+
+void orc_match_top2(const uint8_t* q, int nq, const uint8_t* db, int ndb, int nthreads,
+                    int* idx1, int* d1, int* idx2, int* d2)
+{
+    orc_match_top2_norm(q, nq, db, ndb, nthreads, 0, idx1, d1, idx2, d2);
 }

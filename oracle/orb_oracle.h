@@ -86,6 +86,9 @@ int orc_detect_and_compute(const orc_params* prm, const uint8_t* img, int w, int
 void orc_match(const uint8_t* d1, int n1, const uint8_t* d2, int n2, int* match, int* dist);
 void orc_match_top2(const uint8_t* q, int nq, const uint8_t* db, int ndb, int nthreads,
                     int* idx1, int* d1, int* idx2, int* d2);
+/* norm 0: Hamming; norm 1: OpenCV's NORM_HAMMING2 (differing bit pairs) */
+void orc_match_top2_norm(const uint8_t* q, int nq, const uint8_t* db, int ndb, int nthreads, int norm,
+                         int* idx1, int* d1, int* idx2, int* d2);
 
 #ifdef __cplusplus
 }

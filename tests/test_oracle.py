@@ -518,6 +518,82 @@ def test_hamming_metric_properties(oracle):
     assert all(D[i, k] <= D[i, j] + D[j, k] for i in range(10) for j in range(10) for k in range(10))
 
 
+def test_hamming2_closed_form_and_bounds(oracle):
+    """NORM_HAMMING2 counts the bit pairs (2i, 2i+1) that differ: hand cases, a brute-force
+    count over the unpacked bits, and ham2 <= ham <= 2 ham2."""
+    z = np.zeros((1, 32), np.uint8)
+    for byte, bits, want in ((0, 0b11, 1), (0, 0b110, 2), (5, 0b11000000, 1), (31, 0xff, 4)):
+        b = z.copy()
+        b[0, byte] = bits
+        assert oracle.match_top2(z, b, norm=1)[1][0] == want
+        assert oracle.match_top2(z, b, norm=0)[1][0] == bin(bits).count("1")
+    rng = np.random.default_rng(13)
+    q = rng.integers(0, 256, (30, 32), dtype=np.uint8)
+    db = rng.integers(0, 256, (200, 32), dtype=np.uint8)
+    db[77] = db[12]                       # tie: the lower index wins
+    q[0] = db[12]
+    bq = np.unpackbits(q, axis=1, bitorder="little").reshape(30, 128, 2)
+    bd = np.unpackbits(db, axis=1, bitorder="little").reshape(200, 128, 2)
+    D2 = (bq[:, None] != bd[None]).any(-1).sum(-1)
+    D1 = (bq[:, None] != bd[None]).sum((-1, -2))
+    assert ((D2 <= D1) & (D1 <= 2 * D2)).all()
+    i1, d1, i2, d2 = oracle.match_top2(q, db, norm=1)
+    for r in range(30):
+        order = np.lexsort((np.arange(200), D2[r]))
+        assert (i1[r], d1[r], i2[r], d2[r]) == (order[0], D2[r, order[0]], order[1], D2[r, order[1]])
+    assert (i1[0], d1[0], i2[0], d2[0]) == (12, 0, 77, 0)
+
+
+def export_points(oracle, scale, rng, n):
+    """Points at known level coordinates, mapped to level 0 by the detector's own rule."""
+    import ctypes
+    g = oracle.geometry(1920, 1080, 8 if scale != 2.0 else 5, scale)
+    pts = np.zeros(n, oracle.POINT_DTYPE)
+    lv = rng.integers(0, g.nlevels, n)
+    lxy = np.zeros((n, 2), int)
+    for i in range(n):
+        l = int(lv[i])
+        lx, ly = int(rng.integers(3, g.w[l] - 3)), int(rng.integers(3, g.h[l] - 3))
+        lxy[i] = (lx, ly)
+        pts["x"][i] = oracle.lib().orc_level_to_l0(ctypes.byref(g), l, lx)
+        pts["y"][i] = oracle.lib().orc_level_to_l0(ctypes.byref(g), l, ly)
+        pts["octave"][i] = l
+    pts["score"] = rng.integers(1, 40, n).astype(np.float32) * np.float32(1e-3)   # many ties
+    pts["angle"] = rng.uniform(-math.pi, math.pi, n).astype(np.float32)
+    return pts, lxy
+
+
+@pytest.mark.parametrize("scale", [1.2, 2.0])
+def test_export_cv_coordinates_angles_retain(oracle, scale):
+    """The export recovers each point's level coordinate (round trip through the detector's
+    level -> level-0 rule), scales it by float32(scale**l), keeps OpenCV's degree range, and
+    with retain_topn keeps exactly the top scores with ties to the earlier point, in order,
+    descriptors following."""
+    rng = np.random.default_rng(14)
+    pts, lxy = export_points(oracle, scale, rng, 400)
+    desc = rng.integers(0, 256, (400, 32), dtype=np.uint8)
+    kp, d = oracle.export_cv(pts, desc, -1, scale)
+    assert len(kp) == 400 and np.array_equal(d, desc)
+    lsc = np.array([np.float32(scale ** l) for l in pts["octave"]], np.float32)
+    assert np.array_equal(kp["x"], lxy[:, 0].astype(np.float32) * lsc)
+    assert np.array_equal(kp["y"], lxy[:, 1].astype(np.float32) * lsc)
+    if scale == 2.0:
+        assert np.array_equal(kp["x"], pts["x"].astype(np.float32))
+    assert np.array_equal(kp["size"], np.float32(31) * lsc)
+    assert ((kp["angle"] >= 0) & (kp["angle"] < 360)).all()
+    want = np.degrees(pts["angle"].astype(np.float64)) % 360.0
+    err = np.abs(kp["angle"] - want)
+    assert (np.minimum(err, 360 - err) < 1e-3).all()
+    assert (kp["octave"] == pts["octave"]).all() and (kp["class_id"] == -1).all()
+    for k in (1, 57, 399, 400, 1000):
+        kk, dk = oracle.export_cv(pts, desc, k, scale)
+        best = sorted(range(400), key=lambda i: (-float(pts["score"][i]), i))[:min(k, 400)]
+        keep = sorted(best)
+        assert np.array_equal(kk["x"], kp["x"][keep]) and np.array_equal(kk["y"], kp["y"][keep])
+        assert np.array_equal(kk["response"], pts["score"][keep])
+        assert np.array_equal(dk, desc[keep])
+
+
 # -------------------------------------------------------------- end to end
 
 def oracle_synth(w, h, nshapes, seed):
