@@ -79,15 +79,34 @@ __device__ __forceinline__ ORB_GLOBAL T* as_global(T* p)
     return (ORB_GLOBAL T*)p;
 }
 
+/* The job of unit b: jobs are sorted by first_block, so it is the last one
+   starting at or before b.  Each lane tests one job (a single coalesced load
+   per 64 jobs instead of a chain of dependent loads); called by whole
+   wavefronts with a wave-uniform b. */
 __device__ __forceinline__ Job find_job(const Job* __restrict__ jobs, int n, int b)
 {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].first_block <= b) lo = mid;
-        else hi = mid - 1;
+    const int lane = threadIdx.x & 63;
+    int count = 0;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        const unsigned long long m = __ballot(j < n && jobs[j].first_block <= b);
+        count += __popcll(m);
+        if (m != ~0ull) break;
     }
-    return jobs[lo];
+    return jobs[__builtin_amdgcn_readfirstlane(max(count - 1, 0))];
+}
+
+/* Fixed-point bilinear source position of level index i along an axis of S
+   input and D level pixels: floor(((2i+1)S - D) * 256 / (2D)) clamped to
+   [0, (S-1)*256] (centre-aligned, 8 fractional bits).  The numerator is an
+   integer below 2^53 and a non-integral quotient lies at least 1/(2D) from
+   the next integer, far beyond the division's rounding, so the floor of the
+   IEEE double quotient is the integer quotient. */
+__device__ __forceinline__ int resize_pos(int i, int S, int D)
+{
+    const double num = (double)((long long)(2 * i + 1) * S - D) * 256.0;
+    const double q = floor(__ddiv_rn(num, (double)(2 * D)));
+    return (int)fmin(fmax(q, 0.0), (double)(S - 1) * 256.0);
 }
 
 /* reflect-101 (cuda_utils.h:145-152), clamped so halo entries that no output
@@ -381,7 +400,6 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
     P.src = __builtin_amdgcn_make_buffer_rsrc((void*)F.img, 0, F.h0 * F.ip, 0x00020000);
     const bool aligned = (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
     const int mode = l == 0 ? (aligned ? 0 : 1) : (F.ref_mode ? 1 : 2);
-    const int* tab = T.tabs + F.tab_off[l];
     P.padmask = 0;
     P.ipmask = 0;
     int xr[4];
@@ -394,7 +412,7 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
         if (mode == 2) {
             /* the dword at min(x0, w0 - 4) ends inside the input row; at x0 = w0 - 1
                both taps are that pixel, which fx = 0 gives exactly */
-            const int v = tab[xr[k]];
+            const int v = resize_pos(xr[k], F.w0, w);
             const int x0 = v >> 8, fx = x0 == F.w0 - 1 ? 0 : (v & 255);
             const int d = min(x0, F.w0 - 4), sh = x0 - d;
             P.xa[k] = (uint32_t)d;
@@ -418,7 +436,7 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
     /* lanes 1..62 inside the pitch write; the others store to kOob */
     const uint32_t cxw = (lane >= 1 && lane <= 62 && cx < p) ? (uint32_t)cx : kOob;
     /* lane i holds the bilinear row entry of walked row i */
-    const int tyv = mode == 2 && lane < TH + 6 ? tab[w + reflect101(ys - 3 + lane, h)] : 0;
+    const int tyv = mode == 2 && lane < TH + 6 ? resize_pos(reflect101(ys - 3 + lane, h), F.h0, h) : 0;
     const int lbytes = h * p;
     __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc((void*)(F.raw + F.loff[l]), 0, lbytes, 0x00020000);
     __amdgpu_buffer_rsrc_t blur = __builtin_amdgcn_make_buffer_rsrc((void*)(F.blur + F.loff[l]), 0, lbytes, 0x00020000);
