@@ -157,8 +157,12 @@ struct orb_ctx {
     std::vector<FrameSet*> sets;
     FrameSet* fs = nullptr;
     unsigned long long use_clock = 0;
+    /* matcher workspace: [merge words: 2 per query | tile counters: 1 per 64
+       queries | MFMA per-chunk partials]; the first two regions start as "no
+       key" / 0 and every popcount match leaves them so */
     uint32_t* d_match_partial = nullptr;
-    size_t match_partial_cap = 0;
+    size_t match_partial_cap = 0;   /* words of the partial region */
+    size_t match_state_q = 0;       /* queries the merge region holds */
     int partial_pins = 0;                  /* graphs that recorded the matcher workspace */
     std::vector<uint32_t*> retired_partial;/* replaced workspaces still named by a graph */
     std::vector<FrameSet*> captured_sets;  /* during capture: sets the recorded calls use */
@@ -514,9 +518,23 @@ const StageEvents* take_events(orb_ctx* c, StageEvents& ev, unsigned stages)
             }
             (k == 0 ? ev.start[s] : ev.stop[s]) = e;
         }
-        if (ev.on[s]) c->recs.push_back({s, ev.start[s], ev.stop[s]});
     }
     return &ev;
+}
+
+/* Keep the event pairs of the stages the call launched; the others go back
+   to the pool unrecorded (they would otherwise report stale times). */
+void commit_events(orb_ctx* c, const StageEvents& ev)
+{
+    for (int s = 0; s < kNumStages; s++) {
+        if (!ev.on[s]) continue;
+        if (ev.rec[s]) {
+            c->recs.push_back({s, ev.start[s], ev.stop[s]});
+        } else {
+            c->ev_pool.push_back(ev.start[s]);
+            c->ev_pool.push_back(ev.stop[s]);
+        }
+    }
 }
 
 Tables make_tables(const orb_ctx* c)
@@ -651,14 +669,20 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     StageEvents ev;
     memset(&ev, 0, sizeof(ev));
     hipError_t e = launch_detect(d, compute_desc, c->stream, take_events(c, ev, 0x2fu));
+    commit_events(c, ev);
     if (e != hipSuccess) return hip_fail(c, e, "detect launch");
     return ORB_OK;
 }
 
-int ensure_partial(orb_ctx* c, size_t words)
+size_t state_words(size_t nq)
+{
+    return (2 * nq + (nq + 63) / 64 + 63) & ~(size_t)63;
+}
+
+int ensure_partial(orb_ctx* c, size_t words, int nq)
 {
     if (c->capturing) c->captured_partial = true;
-    if (words <= c->match_partial_cap) return ORB_OK;
+    if (words <= c->match_partial_cap && (size_t)nq <= c->match_state_q) return ORB_OK;
     if (c->capturing)
         return fail(c, ORB_E_STATE, "matcher workspace grows during graph capture: make the same call once before "
                                     "orb_capture_begin");
@@ -667,10 +691,16 @@ int ensure_partial(orb_ctx* c, size_t words)
         if (c->partial_pins > 0) c->retired_partial.push_back(c->d_match_partial); /* a graph still names it */
         else (void)hipFree(c->d_match_partial);
     }
+    const size_t pw = words > c->match_partial_cap ? words : c->match_partial_cap;
+    const size_t q = (size_t)nq > c->match_state_q ? (size_t)nq : c->match_state_q;
     c->d_match_partial = nullptr;
     c->match_partial_cap = 0;
-    HIP_TRY(c, hipMalloc((void**)&c->d_match_partial, words * sizeof(uint32_t)));
-    c->match_partial_cap = words;
+    c->match_state_q = 0;
+    HIP_TRY(c, hipMalloc((void**)&c->d_match_partial, (state_words(q) + pw) * sizeof(uint32_t)));
+    HIP_TRY(c, hipMemsetAsync(c->d_match_partial, 0xff, 8 * q, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->d_match_partial + 2 * q, 0, 4 * ((q + 63) / 64), c->stream));
+    c->match_partial_cap = pw;
+    c->match_state_q = q;
     return ORB_OK;
 }
 
@@ -686,7 +716,7 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
     MatchLaunch m;
     memset(&m, 0, sizeof(m));
     const size_t words = match_partial_words(nq, ndb, method, &m.nchunks, &m.chunk);
-    int rc = ensure_partial(c, words);
+    int rc = ensure_partial(c, words, nq);
     if (rc != ORB_OK) return rc;
     m.q = q;
     m.nq = nq;
@@ -694,7 +724,9 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
     m.db = db;
     m.ndb = ndb;
     m.d_ndb = d_ndb;
-    m.partial = c->d_match_partial;
+    m.state = (unsigned long long*)c->d_match_partial;
+    m.done = (int*)(c->d_match_partial + 2 * c->match_state_q);
+    m.partial = c->d_match_partial + state_words(c->match_state_q);
     m.idx1 = idx1;
     m.d1 = d1;
     m.idx2 = idx2;
@@ -706,6 +738,7 @@ int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8
     const StageEvents* pev = take_events(c, ev, 0xc0u);
     hipError_t e = method == ORB_MATCH_MFMA ? launch_match_mfma(m, c->stream, pev)
                                             : launch_match_popcount(m, c->stream, pev);
+    commit_events(c, ev);
     if (e != hipSuccess) return hip_fail(c, e, method == ORB_MATCH_MFMA ? "MFMA match" : "popcount match");
     return ORB_OK;
 }
@@ -981,7 +1014,7 @@ const char* orb_last_error(const orb_ctx* c)
 size_t orb_workspace_bytes(const orb_ctx* c)
 {
     if (!c) return 0;
-    size_t b = c->match_partial_cap * sizeof(uint32_t);
+    size_t b = c->d_match_partial ? (c->match_partial_cap + state_words(c->match_state_q)) * sizeof(uint32_t) : 0;
     for (const FrameSet* s : c->sets) b += s->arena_bytes;
     return b;
 }

@@ -18,6 +18,7 @@
  */
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -124,12 +125,13 @@ struct StageEvents {
     hipEvent_t start[kNumStages];
     hipEvent_t stop[kNumStages];
     bool on[kNumStages];
+    mutable bool rec[kNumStages];   /* set by launch_stage: the stage was launched */
 };
 
 /* orb_kernels.hip */
 hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s, const StageEvents* ev);
 /* the pyramid stage alone with strip-height variant `variant` (autotuner) */
-hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s);
+hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s, const StageEvents* ev = nullptr);
 /* OpenCV-layout export: per-level scales of the context's geometry */
 struct ExportArgs {
     double sc[kMaxLevels];  /* s^l: level -> level-0 coordinate, as detection rounds it */
@@ -146,7 +148,8 @@ hipError_t launch_debug_atan2(const float* yx, int n, float* out, hipStream_t st
 struct MatchLaunch {
     const uint8_t* q; int nq; const int* d_nq;
     const uint8_t* db; int ndb; const int* d_ndb;
-    uint32_t* partial; int nchunks; int chunk;
+    uint32_t* partial; int nchunks; int chunk;   /* MFMA: per-chunk top-2 of each query */
+    unsigned long long* state; int* done;       /* popcount: merge word per query, counter per 64-query tile */
     int* idx1; int* d1; int* idx2; int* d2;   /* top-2 outputs (may be null) */
     orb::OrbPoint* pts;                       /* reference-rule output (may be null) */
     bool hamming2;                            /* popcount matcher: count differing bit pairs */
@@ -154,13 +157,20 @@ struct MatchLaunch {
 hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
 
-inline void ev_start(const StageEvents* ev, int st, hipStream_t s)
+/* Launch one kernel of stage st.  When the stage is timed, its events go with
+   the dispatch itself (hipExtLaunchKernel marks the kernel's own start and
+   end), so timing adds no packets to the stream and no bubble between
+   kernels. */
+template <typename F, typename... Args>
+inline void launch_stage(const StageEvents* ev, int st, F kernel, dim3 grid, dim3 block, hipStream_t s,
+                         Args... args)
 {
-    if (ev && ev->on[st]) (void)hipEventRecord(ev->start[st], s);
-}
-inline void ev_stop(const StageEvents* ev, int st, hipStream_t s)
-{
-    if (ev && ev->on[st]) (void)hipEventRecord(ev->stop[st], s);
+    if (ev && ev->on[st]) {
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev->start[st], ev->stop[st], 0, args...);
+        ev->rec[st] = true;
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+    }
 }
 size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk);
 

@@ -1253,13 +1253,13 @@ __global__ void k_debug_atan2(const float* yx, int n, float* out)
 
 /* ------------------------------------------------------------- launchers */
 
-hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s)
+hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s, const StageEvents* ev)
 {
     switch (variant) {
-#define ORB_PYR_CASE(v)                                                                                     \
-    case v:                                                                                                 \
-        hipLaunchKernelGGL(k_pyramid<kPyrTHs[v]>, dim3(d.pyr_blocks), dim3(256), 0, s, d.frames, d.pyr_jobs, \
-                           d.pyr_njobs, d.pyr_units, d.t);                                                  \
+#define ORB_PYR_CASE(v)                                                                                         \
+    case v:                                                                                                     \
+        launch_stage(ev, kStPyramid, k_pyramid<kPyrTHs[v]>, dim3(d.pyr_blocks), dim3(256), s, d.frames, d.pyr_jobs, \
+                     d.pyr_njobs, d.pyr_units, d.t);                                                            \
         break;
         ORB_PYR_CASE(0)
         ORB_PYR_CASE(1)
@@ -1276,32 +1276,19 @@ hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s,
 {
     (void)hipGetLastError(); /* report only errors of these launches */
     if (d.pyr_blocks > 0) {
-        ev_start(ev, kStPyramid, s);
-        hipError_t e = launch_pyramid(d, d.pyr_variant, s);
+        hipError_t e = launch_pyramid(d, d.pyr_variant, s, ev);
         if (e != hipSuccess) return e;
-        ev_stop(ev, kStPyramid, s);
     }
-    if (d.det_blocks > 0) {
-        ev_start(ev, kStDetect, s);
-        hipLaunchKernelGGL(k_detect, dim3(d.det_blocks), dim3(256), 0, s, d.frames, d.det_jobs, d.det_njobs, d.t,
-                           d.epochs);
-        ev_stop(ev, kStDetect, s);
-    }
-    if (d.nms_blocks > 0) {
-        ev_start(ev, kStNms, s);
-        hipLaunchKernelGGL(k_nms, dim3((d.nms_blocks + 3) / 4), dim3(256), 0, s, d.frames, d.nms_block_frame,
-                           d.nms_blocks, d.epochs, d.cand, d.blk_count);
-        ev_stop(ev, kStNms, s);
-    }
-    ev_start(ev, kStSelect, s);
-    hipLaunchKernelGGL(k_select, dim3(d.nframes), dim3(1024), 0, s, d.frames, d.cand, d.blk_count, d.epochs);
-    ev_stop(ev, kStSelect, s);
-    if (compute_desc && d.desc_blocks_per_frame > 0) {
-        ev_start(ev, kStDescribe, s);
-        hipLaunchKernelGGL(k_describe, dim3(d.desc_blocks_per_frame * d.nframes), dim3(256), 0, s, d.frames,
-                           d.desc_blocks_per_frame, d.t, compute_desc);
-        ev_stop(ev, kStDescribe, s);
-    }
+    if (d.det_blocks > 0)
+        launch_stage(ev, kStDetect, k_detect, dim3(d.det_blocks), dim3(256), s, d.frames, d.det_jobs, d.det_njobs,
+                     d.t, d.epochs);
+    if (d.nms_blocks > 0)
+        launch_stage(ev, kStNms, k_nms, dim3((d.nms_blocks + 3) / 4), dim3(256), s, d.frames, d.nms_block_frame,
+                     d.nms_blocks, d.epochs, d.cand, d.blk_count);
+    launch_stage(ev, kStSelect, k_select, dim3(d.nframes), dim3(1024), s, d.frames, d.cand, d.blk_count, d.epochs);
+    if (compute_desc && d.desc_blocks_per_frame > 0)
+        launch_stage(ev, kStDescribe, k_describe, dim3(d.desc_blocks_per_frame * d.nframes), dim3(256), s, d.frames,
+                     d.desc_blocks_per_frame, d.t, compute_desc);
     return hipGetLastError();
 }
 

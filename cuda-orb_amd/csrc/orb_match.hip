@@ -63,20 +63,62 @@ __device__ __forceinline__ uint32_t word_dist(uint32_t x)
     return __popc(x);
 }
 
+/* Outputs of query qi from its final top-2 keys: raw (idx, distance) pairs,
+   and the reference's acceptance rule into the point (orbd.cu:1544-1558: both
+   distances start at 256; a unique best below 64 is accepted). */
+__device__ __forceinline__ void write_match(int qi, uint32_t k1, uint32_t k2, int* idx1, int* d1, int* idx2, int* d2,
+                                            orb::OrbPoint* pts)
+{
+    const int a1 = k1 == kNoKey ? -1 : (int)(k1 & kIdxMask), b1 = k1 == kNoKey ? 257 : (int)(k1 >> kIdxBits);
+    const int a2 = k2 == kNoKey ? -1 : (int)(k2 & kIdxMask), b2 = k2 == kNoKey ? 257 : (int)(k2 >> kIdxBits);
+    if (idx1) idx1[qi] = a1;
+    if (d1) d1[qi] = b1;
+    if (idx2) idx2[qi] = a2;
+    if (d2) d2[qi] = b2;
+    if (pts) {
+        const int s1 = min(b1, 256), s2 = min(b2, 256);
+        orb::OrbPoint* p = pts + qi;
+        if (s1 < s2 && s1 < 64) {
+            p->match = a1;
+            p->distance = s1;
+        } else {
+            p->match = -1;
+            p->distance = -1;
+        }
+    }
+}
+
 /* A workgroup unit is 64 queries x one chunk; its 4 wavefronts scan a
    quarter of the chunk each and merge through LDS, so small problems still
-   spread over many wavefronts without multiplying the merge work. */
+   spread over many wavefronts without multiplying the merge work.
+
+   The chunks of a query tile meet in one 64-bit word per query, (k1 << 32) |
+   k2, merged by compare-and-swap at device scope; a counter per query tile
+   tells the unit that finishes the tile's last chunk, which reads each word
+   (resetting it to "no key"), writes the outputs and resets the counter, so
+   the words and counters are clean for the next call and one launch does the
+   whole match.  Every value that units exchange is a device-scope atomic, so
+   no cache write-back is needed between them. */
 template <bool H2>
 __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q, int nq_cap,
                                                    const int* __restrict__ d_nq, const uint4* __restrict__ db,
                                                    int ndb_cap, const int* __restrict__ d_ndb, int slots,
-                                                   uint2* __restrict__ partial)
+                                                   unsigned long long* __restrict__ state, int* __restrict__ done,
+                                                   int* __restrict__ idx1, int* __restrict__ d1,
+                                                   int* __restrict__ idx2, int* __restrict__ d2,
+                                                   orb::OrbPoint* __restrict__ pts)
 {
     __shared__ uint2 s_k[3][64];
     const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
     const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
     const Plan P = make_plan(nq, ndb, kPopQ, slots);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (P.chunks == 0) {
+        /* empty database: every query has no match */
+        for (int qi = blockIdx.x * 256 + threadIdx.x; qi < nq; qi += gridDim.x * 256)
+            write_match(qi, kNoKey, kNoKey, idx1, d1, idx2, d2, pts);
+        return;
+    }
     for (int u = blockIdx.x; u < P.qunits * P.chunks; u += gridDim.x) {
         const int qu = u % P.qunits, c = u / P.qunits;
         const int qi = qu * kPopQ + lane;
@@ -123,7 +165,41 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
                 top2_push(k1, k2, s_k[w][lane].x);
                 top2_push(k1, k2, s_k[w][lane].y);
             }
-            if (qi < nq) partial[(long long)c * nq + qi] = make_uint2(k1, k2);
+            if (P.chunks == 1) {
+                if (qi < nq) write_match(qi, k1, k2, idx1, d1, idx2, d2, pts);
+            } else {
+                if (qi < nq) {
+                    unsigned long long* wp = state + qi;
+                    unsigned long long old = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (;;) {
+                        const uint32_t o1 = (uint32_t)(old >> 32), o2 = (uint32_t)old;
+                        uint32_t n1 = o1, n2 = o2;
+                        top2_push(n1, n2, k1);
+                        top2_push(n1, n2, k2);
+                        if (n1 == o1 && n2 == o2) break;
+                        if (__hip_atomic_compare_exchange_strong(wp, &old, ((unsigned long long)n1 << 32) | n2,
+                                                                 __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                 __HIP_MEMORY_SCOPE_AGENT))
+                            break;
+                    }
+                }
+                /* the merges above have completed before the counter moves */
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                int last = 0;
+                if (lane == 0)
+                    last = __hip_atomic_fetch_add(done + qu, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                           P.chunks - 1;
+                last = __shfl(last, 0, 64);
+                if (last) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    if (qi < nq) {
+                        const unsigned long long v = __hip_atomic_exchange(state + qi, ~0ull, __ATOMIC_RELAXED,
+                                                                           __HIP_MEMORY_SCOPE_AGENT);
+                        write_match(qi, (uint32_t)(v >> 32), (uint32_t)v, idx1, d1, idx2, d2, pts);
+                    }
+                    if (lane == 0) __hip_atomic_store(done + qu, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
         }
         __syncthreads();
     }
@@ -249,24 +325,7 @@ __global__ __launch_bounds__(256) void k_match_merge(int nq_cap, const int* __re
             top2_push(k1, k2, v.x);
             top2_push(k1, k2, v.y);
         }
-        const int a1 = k1 == kNoKey ? -1 : (int)(k1 & kIdxMask), b1 = k1 == kNoKey ? 257 : (int)(k1 >> kIdxBits);
-        const int a2 = k2 == kNoKey ? -1 : (int)(k2 & kIdxMask), b2 = k2 == kNoKey ? 257 : (int)(k2 >> kIdxBits);
-        if (idx1) idx1[qi] = a1;
-        if (d1) d1[qi] = b1;
-        if (idx2) idx2[qi] = a2;
-        if (d2) d2[qi] = b2;
-        if (pts) {
-            /* orbd.cu:1544-1558: both distances start at 256; accept a unique best below 64 */
-            const int s1 = min(b1, 256), s2 = min(b2, 256);
-            orb::OrbPoint* p = pts + qi;
-            if (s1 < s2 && s1 < 64) {
-                p->match = a1;
-                p->distance = s1;
-            } else {
-                p->match = -1;
-                p->distance = -1;
-            }
-        }
+        write_match(qi, k1, k2, idx1, d1, idx2, d2, pts);
     }
 }
 
@@ -278,6 +337,11 @@ constexpr int kMfmaBlocks = 1024;  /* workgroup slots */
 size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk)
 {
     (void)ndb;
+    if (method != 1) {   /* the popcount matcher merges in place (match_state_words) */
+        *nchunks = 0;
+        *chunk = 0;
+        return 0;
+    }
     /* chunks * nq <= (slots / qunits + 1) * qunits * qtile <= (slots + qunits) * qtile */
     const int qtile = method == 1 ? kMfmaQ : kPopQ;
     const int slots = method == 1 ? kMfmaBlocks : kPopBlocks;
@@ -291,11 +355,9 @@ size_t match_partial_words(int nq, int ndb, int method, int* nchunks, int* chunk
 
 static void launch_merge(const MatchLaunch& m, int qtile, int slots, hipStream_t s, const StageEvents* ev)
 {
-    ev_start(ev, kStMatchMerge, s);
     const int blocks = (m.nq + 255) / 256 < 1024 ? (m.nq + 255) / 256 : 1024;
-    hipLaunchKernelGGL(k_match_merge, dim3(blocks), dim3(256), 0, s, m.nq, m.d_nq, m.ndb, m.d_ndb, qtile, slots,
-                       (const uint2*)m.partial, m.idx1, m.d1, m.idx2, m.d2, m.pts);
-    ev_stop(ev, kStMatchMerge, s);
+    launch_stage(ev, kStMatchMerge, k_match_merge, dim3(blocks), dim3(256), s, m.nq, m.d_nq, m.ndb, m.d_ndb, qtile,
+                 slots, (const uint2*)m.partial, m.idx1, m.d1, m.idx2, m.d2, m.pts);
 }
 
 hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const StageEvents* ev)
@@ -308,15 +370,12 @@ hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const Stag
     const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
     int blocks = (int)(units < kPopBlocks ? units : kPopBlocks);
     if (blocks < 1) blocks = 1;
-    ev_start(ev, kStMatch, s);
     if (m.hamming2)
-        hipLaunchKernelGGL(k_match_pop<true>, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db,
-                           m.ndb, m.d_ndb, slots, (uint2*)m.partial);
+        launch_stage(ev, kStMatch, k_match_pop<true>, dim3(blocks), dim3(256), s, m.q, m.nq, m.d_nq,
+                     (const uint4*)m.db, m.ndb, m.d_ndb, slots, m.state, m.done, m.idx1, m.d1, m.idx2, m.d2, m.pts);
     else
-        hipLaunchKernelGGL(k_match_pop<false>, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, (const uint4*)m.db,
-                           m.ndb, m.d_ndb, slots, (uint2*)m.partial);
-    ev_stop(ev, kStMatch, s);
-    launch_merge(m, kPopQ, slots, s, ev);
+        launch_stage(ev, kStMatch, k_match_pop<false>, dim3(blocks), dim3(256), s, m.q, m.nq, m.d_nq,
+                     (const uint4*)m.db, m.ndb, m.d_ndb, slots, m.state, m.done, m.idx1, m.d1, m.idx2, m.d2, m.pts);
     return hipGetLastError();
 }
 
@@ -329,10 +388,8 @@ hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEve
     const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
     int blocks = units < kMfmaBlocks ? (int)units : kMfmaBlocks;
     if (blocks < 1) blocks = 1;
-    ev_start(ev, kStMatch, s);
-    hipLaunchKernelGGL(k_match_mfma, dim3(blocks), dim3(256), 0, s, m.q, m.nq, m.d_nq, m.db, m.ndb, m.d_ndb, slots,
-                       (uint2*)m.partial);
-    ev_stop(ev, kStMatch, s);
+    launch_stage(ev, kStMatch, k_match_mfma, dim3(blocks), dim3(256), s, m.q, m.nq, m.d_nq, m.db, m.ndb, m.d_ndb,
+                 slots, (uint2*)m.partial);
     launch_merge(m, kMfmaQ, slots, s, ev);
     return hipGetLastError();
 }
