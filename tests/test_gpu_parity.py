@@ -65,6 +65,8 @@ CASES = [
     (640, 480, 40, 2, 8, 1.2, 4, 1, 1000),
     (640, 480, 40, 3, 8, 1.2, 2, 0, 150),      # cap well below the survivors: exact selection
     (1280, 720, 120, 3, 8, 1.25, 2, 0, 5000),
+    (3840, 2160, 1080, 5, 8, 1.2, 2, 0, 2000),  # the largest batched frame size, cap below the survivors
+    (3840, 2160, 1080, 5, 5, 2.0, 2, 0, 5000),
 ]
 
 
@@ -84,6 +86,32 @@ def test_detect_and_compute_matches_oracle(torch, ctx, oracle, w, h, nshapes, se
     got_p2, got_d2 = gpu_detect(torch, ctx, img, w, h, cap)
     assert_same_points(got_p2, want_p)
     assert np.array_equal(got_d2, want_d)
+
+
+@pytest.mark.parametrize("w,h", [(16, 16), (40, 40), (64, 48), (100, 100), (17, 300), (300, 17)])
+def test_tiny_frames_match_oracle(torch, ctx, oracle, w, h):
+    """Frames at the 16-pixel level floor and inside the 31-pixel detection border: the same
+    (often empty) keypoint sets as the oracle, and no failure."""
+    import orb_amd
+    print(f"tiny {w}x{h}", flush=True)
+    img = orb_amd.synth_image(6, w, h, 3)
+    ctx.init(noctaves=8, max_pts=100, scale_factor=1.2)
+    got_p, got_d = gpu_detect(torch, ctx, img, w, h, 100)
+    want_p, want_d, _ = oracle.detect_and_compute(img[:, :w].copy(), noctaves=8, scale=1.2, max_pts=100)
+    assert_same_points(got_p, want_p)
+    assert np.array_equal(got_d, want_d)
+
+
+def test_frame_too_small_for_reference_octaves_is_an_error(torch, ctx):
+    """Scale 2 needs min(w, h) >= 80 for its first octave (orb.cpp:148); smaller frames are
+    rejected with an error, and the context keeps working."""
+    import orb_amd
+    img = orb_amd.synth_image(6, 64, 48, 3)
+    ctx.init(noctaves=5, max_pts=100, scale_factor=2.0)
+    with pytest.raises(orb_amd.OrbError):
+        gpu_detect(torch, ctx, img, 64, 48, 100)
+    img = orb_amd.synth_image(6, 160, 120, 10)
+    gpu_detect(torch, ctx, img, 160, 120, 100)
 
 
 def test_input_padding_bytes_follow_the_flat_layout(torch, ctx, oracle):
