@@ -130,7 +130,8 @@ def dense_match(ctx, torch, dev, reps=3):
 def batch_frames(ctx, torch, dev, reps=5):
     """64 frames of sizes drawn from {640x480, 1280x720, 1920x1080, 3840x2160} (shape count scaled to
     area), detected and described in one launch per stage, each matched to a warped partner (the
-    partners are the second half of a 128-frame batch); <= 5000 keypoints per frame."""
+    partners are the second half of a 128-frame batch; the 64 matches in one launch); <= 5000
+    keypoints per frame."""
     import orb_amd
     rng = np.random.default_rng(3)
     sizes = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
@@ -154,8 +155,7 @@ def batch_frames(ctx, torch, dev, reps=5):
 
     def step():
         ctx.detect_and_compute_batch(frames, ws, hs, pl, dl, counts)
-        for i in range(64):
-            ctx.match_dcount(pts[i], desc[i], counts[i:i + 1], cap, desc[64 + i], counts[64 + i:65 + i], cap)
+        ctx.match_batch(pl[:64], dl[:64], counts[:64], cap, dl[64:], counts[64:], cap)
 
     step()
     torch.cuda.synchronize()

@@ -163,6 +163,10 @@ struct orb_ctx {
     uint32_t* d_match_partial = nullptr;
     size_t match_partial_cap = 0;   /* words of the partial region */
     size_t match_state_q = 0;       /* queries the merge region holds */
+    /* batched matching: the device pair table and the host copy it was uploaded from */
+    MatchPair* d_pairs = nullptr;
+    size_t pairs_cap = 0;
+    std::vector<MatchPair> h_pairs;
     int partial_pins = 0;                  /* graphs that recorded the matcher workspace */
     std::vector<uint32_t*> retired_partial;/* replaced workspaces still named by a graph */
     std::vector<FrameSet*> captured_sets;  /* during capture: sets the recorded calls use */
@@ -807,6 +811,7 @@ int orb_ctx_destroy(orb_ctx* c)
     if (c->d_pattern) (void)hipFree(c->d_pattern);
     if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     if (c->d_match_partial) (void)hipFree(c->d_match_partial);
+    if (c->d_pairs) (void)hipFree(c->d_pairs);
     delete c;
     return ORB_OK;
 }
@@ -965,6 +970,49 @@ int orb_match_dcount(orb_ctx* c, orb::OrbPoint* pts1, const uint8_t* desc1, cons
     if (!pts1 || !d_n1 || !d_n2) return fail(c, ORB_E_ARG, "null pointer");
     return run_match(c, desc1, cap1, d_n1, desc2, cap2, d_n2, ORB_MATCH_POPCOUNT, nullptr, nullptr, nullptr,
                      nullptr, pts1);
+}
+
+int orb_match_batch(orb_ctx* c, int npairs, orb::OrbPoint* const* pts1, const uint8_t* const* desc1, const int* d_n1,
+                    int cap1, const uint8_t* const* desc2, const int* d_n2, int cap2)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    if (npairs < 0 || cap1 < 0 || cap2 < 0) return fail(c, ORB_E_ARG, "negative count or capacity");
+    if (cap2 >= (1 << 23)) return fail(c, ORB_E_CAPACITY, "database capacity %d exceeds 2^23-1", cap2);
+    if (npairs == 0 || cap1 == 0) return ORB_OK;
+    if (c->capturing) return fail(c, ORB_E_STATE, "orb_match_batch cannot be recorded into a graph");
+    if (!pts1 || !desc1 || !desc2 || !d_n1 || !d_n2) return fail(c, ORB_E_ARG, "null pointer");
+    std::vector<MatchPair> hp((size_t)npairs);
+    for (int i = 0; i < npairs; i++) {
+        if (!pts1[i] || !desc1[i] || (cap2 > 0 && !desc2[i])) return fail(c, ORB_E_ARG, "pair %d: null pointer", i);
+        hp[i] = MatchPair{desc1[i], desc2[i], d_n1 + i, d_n2 + i, pts1[i]};
+    }
+    const int qcap = (cap1 + 63) & ~63;
+    int rc = ensure_partial(c, 0, npairs * qcap);
+    if (rc != ORB_OK) return rc;
+    const bool same = hp.size() == c->h_pairs.size() &&
+                      memcmp(hp.data(), c->h_pairs.data(), sizeof(MatchPair) * hp.size()) == 0;
+    if (!same) {
+        if ((size_t)npairs > c->pairs_cap) {
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            if (c->d_pairs) (void)hipFree(c->d_pairs);
+            c->d_pairs = nullptr;
+            c->pairs_cap = 0;
+            HIP_TRY(c, hipMalloc((void**)&c->d_pairs, sizeof(MatchPair) * (size_t)npairs));
+            c->pairs_cap = (size_t)npairs;
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->d_pairs, hp.data(), sizeof(MatchPair) * hp.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        c->h_pairs.swap(hp);
+    }
+    StageEvents ev;
+    memset(&ev, 0, sizeof(ev));
+    const StageEvents* pev = take_events(c, ev, 0x40u);
+    hipError_t e = launch_match_pairs(c->d_pairs, npairs, cap1, cap2, qcap, (unsigned long long*)c->d_match_partial,
+                                      (int*)(c->d_match_partial + 2 * c->match_state_q), c->stream, pev);
+    commit_events(c, ev);
+    if (e != hipSuccess) return hip_fail(c, e, "batched match");
+    return ORB_OK;
 }
 
 int orb_match_top2(orb_ctx* c, const uint8_t* q, int nq, const uint8_t* db, int ndb, int method, int* idx1, int* d1,

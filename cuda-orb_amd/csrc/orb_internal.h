@@ -155,6 +155,17 @@ struct MatchLaunch {
     bool hamming2;                            /* popcount matcher: count differing bit pairs */
 };
 hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
+/* one pair of a batched reference-rule match: queries q (count *n1), database db (count *n2) */
+struct MatchPair {
+    const uint8_t* q;
+    const uint8_t* db;
+    const int* n1;
+    const int* n2;
+    orb::OrbPoint* pts;
+};
+/* npairs popcount matches in one launch; state/done hold qcap queries (a multiple of 64) per pair */
+hipError_t launch_match_pairs(const MatchPair* d_pairs, int npairs, int cap1, int cap2, int qcap,
+                              unsigned long long* state, int* done, hipStream_t s, const StageEvents* ev);
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
 
 /* Launch one kernel of stage st.  When the stage is timed, its events go with

@@ -100,26 +100,23 @@ __device__ __forceinline__ void write_match(int qi, uint32_t k1, uint32_t k2, in
    whole match.  Every value that units exchange is a device-scope atomic, so
    no cache write-back is needed between them. */
 template <bool H2>
-__global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q, int nq_cap,
-                                                   const int* __restrict__ d_nq, const uint4* __restrict__ db,
-                                                   int ndb_cap, const int* __restrict__ d_ndb, int slots,
-                                                   unsigned long long* __restrict__ state, int* __restrict__ done,
-                                                   int* __restrict__ idx1, int* __restrict__ d1,
-                                                   int* __restrict__ idx2, int* __restrict__ d2,
-                                                   orb::OrbPoint* __restrict__ pts)
+__device__ __forceinline__ void match_pop_units(const uint8_t* __restrict__ q, int nq, const uint4* __restrict__ db,
+                                                int ndb, int slots, int wg, int nwg,
+                                                unsigned long long* __restrict__ state, int* __restrict__ done,
+                                                int* __restrict__ idx1, int* __restrict__ d1,
+                                                int* __restrict__ idx2, int* __restrict__ d2,
+                                                orb::OrbPoint* __restrict__ pts)
 {
     __shared__ uint2 s_k[3][64];
-    const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
-    const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
     const Plan P = make_plan(nq, ndb, kPopQ, slots);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (P.chunks == 0) {
         /* empty database: every query has no match */
-        for (int qi = blockIdx.x * 256 + threadIdx.x; qi < nq; qi += gridDim.x * 256)
+        for (int qi = wg * 256 + threadIdx.x; qi < nq; qi += nwg * 256)
             write_match(qi, kNoKey, kNoKey, idx1, d1, idx2, d2, pts);
         return;
     }
-    for (int u = blockIdx.x; u < P.qunits * P.chunks; u += gridDim.x) {
+    for (int u = wg; u < P.qunits * P.chunks; u += nwg) {
         const int qu = u % P.qunits, c = u / P.qunits;
         const int qi = qu * kPopQ + lane;
         uint4 qa = make_uint4(0, 0, 0, 0), qb = qa;
@@ -203,6 +200,34 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
         }
         __syncthreads();
     }
+}
+
+template <bool H2>
+__global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q, int nq_cap,
+                                                   const int* __restrict__ d_nq, const uint4* __restrict__ db,
+                                                   int ndb_cap, const int* __restrict__ d_ndb, int slots,
+                                                   unsigned long long* __restrict__ state, int* __restrict__ done,
+                                                   int* __restrict__ idx1, int* __restrict__ d1,
+                                                   int* __restrict__ idx2, int* __restrict__ d2,
+                                                   orb::OrbPoint* __restrict__ pts)
+{
+    const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+    const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
+    match_pop_units<H2>(q, nq, db, ndb, slots, blockIdx.x, gridDim.x, state, done, idx1, d1, idx2, d2, pts);
+}
+
+/* Many independent pairs in one launch: grid row y is pair y, with its own
+   device-side plan over the grid row's workgroups and its own merge words
+   and tile counters (qcap queries' worth per pair). */
+__global__ __launch_bounds__(256) void k_match_pop_pairs(const MatchPair* __restrict__ pairs, int cap1, int cap2,
+                                                         int qcap, unsigned long long* __restrict__ state,
+                                                         int* __restrict__ done)
+{
+    const MatchPair pr = pairs[blockIdx.y];
+    const int nq = min(max(*pr.n1, 0), cap1), ndb = min(max(*pr.n2, 0), cap2);
+    match_pop_units<false>(pr.q, nq, (const uint4*)pr.db, ndb, gridDim.x, blockIdx.x, gridDim.x,
+                           state + (long long)blockIdx.y * qcap, done + (long long)blockIdx.y * (qcap / kPopQ),
+                           nullptr, nullptr, nullptr, nullptr, pr.pts);
 }
 
 /* --------------------------------------------------------------- MFMA */
@@ -376,6 +401,23 @@ hipError_t launch_match_popcount(const MatchLaunch& m, hipStream_t s, const Stag
     else
         launch_stage(ev, kStMatch, k_match_pop<false>, dim3(blocks), dim3(256), s, m.q, m.nq, m.d_nq,
                      (const uint4*)m.db, m.ndb, m.d_ndb, slots, m.state, m.done, m.idx1, m.d1, m.idx2, m.d2, m.pts);
+    return hipGetLastError();
+}
+
+hipError_t launch_match_pairs(const MatchPair* d_pairs, int npairs, int cap1, int cap2, int qcap,
+                              unsigned long long* state, int* done, hipStream_t s, const StageEvents* ev)
+{
+    (void)hipGetLastError(); /* report only errors of these launches */
+    if (npairs <= 0 || cap1 <= 0) return hipSuccess;
+    /* the workgroups one pair's capacities allow, a share of kPopBlocks each, at least 16 */
+    const Plan P = make_plan(cap1, cap2, kPopQ, kPopBlocks);
+    const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
+    int per = kPopBlocks / npairs;
+    if (per < 16) per = 16;
+    if (units < per) per = (int)units;
+    if (per < 1) per = 1;
+    launch_stage(ev, kStMatch, k_match_pop_pairs, dim3(per, npairs), dim3(256), s, d_pairs, cap1, cap2, qcap, state,
+                 done);
     return hipGetLastError();
 }
 

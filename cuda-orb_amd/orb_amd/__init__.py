@@ -36,7 +36,7 @@ assert CV_KEYPOINT_DTYPE.itemsize == 28
 EXPORTS = [
     "orb_ctx_create", "orb_ctx_destroy", "orb_set_stream", "orb_init", "orb_num_levels",
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
-    "orb_match_top2", "orb_export_keypoints", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
+    "orb_match_top2", "orb_match_batch", "orb_export_keypoints", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
     "orb_tune_set", "orb_tune_get", "orb_debug_pyramid", "orb_capture_begin", "orb_capture_end",
     "orb_graph_launch", "orb_graph_destroy",
@@ -93,6 +93,7 @@ def lib():
         L.orb_match_dcount.argtypes = [P, V, V, V, I, V, V, I]
         L.orb_match_top2.argtypes = [P, V, I, V, I, I, V, V, V, V]
         L.orb_export_keypoints.argtypes = [P, V, V, V, I, V, V, V]
+        L.orb_match_batch.argtypes = [P, I, V, V, V, I, V, V, I]
         L.orb_sync.argtypes = [P]
         L.orb_last_error.argtypes = [P]
         L.orb_last_error.restype = ctypes.c_char_p
@@ -245,6 +246,18 @@ class Context:
         self._bind_stream()
         self._check(lib().orb_match_dcount(self.h, _dptr(pts1), _dptr(desc1), _dptr(d_n1), cap1,
                                            _dptr(desc2), _dptr(d_n2), cap2), "orb_match_dcount")
+
+    def match_batch(self, pts1_list, desc1_list, d_n1, cap1, desc2_list, d_n2, cap2):
+        """orb_match_batch: pair i matches desc1_list[i] (count d_n1[i]) against desc2_list[i]
+        (count d_n2[i]) into pts1_list[i], all pairs in one launch.  d_n1, d_n2: device int32
+        tensors of len(pts1_list) counts.  Asynchronous."""
+        self._bind_stream()
+        n = len(pts1_list)
+        arr = lambda vals: (ctypes.c_void_p * n)(*vals)
+        self._check(lib().orb_match_batch(self.h, n, arr([p.data_ptr() for p in pts1_list]),
+                                          arr([d.data_ptr() for d in desc1_list]), _dptr(d_n1), cap1,
+                                          arr([d.data_ptr() for d in desc2_list]), _dptr(d_n2), cap2),
+                    "orb_match_batch")
 
     def match_top2(self, q, db, method=ORB_MATCH_POPCOUNT, out=None):
         """Exact top-2 of each query row in db; returns (idx1, d1, idx2, d2) int32 device tensors."""

@@ -102,6 +102,17 @@ int orb_match(orb_ctx* ctx, ORB_POINT* d_pts1, const uint8_t* d_desc1, int n1,
 int orb_match_dcount(orb_ctx* ctx, ORB_POINT* d_pts1, const uint8_t* d_desc1, const int* d_n1,
                      int cap1, const uint8_t* d_desc2, const int* d_n2, int cap2);
 
+/* npairs reference-rule matches in one launch: pair i matches the d_n1[i]
+   descriptors at d_desc1[i] (at most cap1) against the d_n2[i] at d_desc2[i]
+   (at most cap2) and writes match/distance into d_pts1[i], as
+   orb_match_dcount does for one pair.  d_pts1, d_desc1, d_desc2 are host
+   arrays of device pointers; d_n1 and d_n2 are device int arrays of npairs
+   counts (the d_counts of a batched detection, for instance).  Not
+   recordable into a graph.  No reference counterpart beyond Orbor::match
+   (orb.cpp:104-113) per pair. */
+int orb_match_batch(orb_ctx* ctx, int npairs, ORB_POINT* const* d_pts1, const uint8_t* const* d_desc1,
+                    const int* d_n1, int cap1, const uint8_t* const* d_desc2, const int* d_n2, int cap2);
+
 #define ORB_MATCH_POPCOUNT 0 /* VALU xor + popcount */
 #define ORB_MATCH_MFMA 1     /* +/-1 INT8 contraction on the matrix cores */
 #define ORB_MATCH_HAMMING2 2 /* VALU, OpenCV's NORM_HAMMING2: differing 2-bit pairs (WTA_K 3/4 codes) */

@@ -298,6 +298,36 @@ def test_match_reference_rule_and_device_counts(torch, ctx, oracle):
     assert np.array_equal(got["match"], want_m) and np.array_equal(got["distance"], want_dist)
 
 
+def test_match_batch_matches_oracle_per_pair(torch, ctx, oracle):
+    """Several pairs in one orb_match_batch launch, counts read on the device: each pair's
+    match/distance equal the reference rule of the oracle; includes an empty database, an
+    empty query set and a one-entry database; a second call with other counts reuses the
+    merge state cleanly."""
+    import orb_amd
+    rng = np.random.default_rng(21)
+    cap = 1200
+    sizes = [(640, 700), (1100, 1200), (0, 300), (50, 0), (1, 1), (300, 1)]
+    q = [rng.integers(0, 256, (cap, 32), dtype=np.uint8) for _ in sizes]
+    db = [rng.integers(0, 256, (cap, 32), dtype=np.uint8) for _ in sizes]
+    for i, (n1, n2) in enumerate(sizes):   # plant near-duplicates so some matches are accepted
+        k = min(n1, n2)
+        db[i][:k // 2] = q[i][:k // 2] ^ (rng.random((k // 2, 32)) < 0.02).astype(np.uint8)
+    dq = [torch.from_numpy(x).cuda() for x in q]
+    ddb = [torch.from_numpy(x).cuda() for x in db]
+    pts = [torch.full((cap, 7), 7, dtype=torch.int32, device="cuda") for _ in sizes]
+    for rnd in range(2):
+        szs = sizes if rnd == 0 else [(n2 // 2, n1) for (n1, n2) in sizes]
+        n1 = torch.tensor([a for a, _ in szs], dtype=torch.int32, device="cuda")
+        n2 = torch.tensor([b for _, b in szs], dtype=torch.int32, device="cuda")
+        ctx.match_batch(pts, dq, n1, cap, ddb, n2, cap)
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(szs):
+            got = orb_amd.points_to_numpy(pts[i], a)
+            want_m, want_d = oracle.match(q[i][:a], db[i][:b])
+            assert np.array_equal(got["match"], want_m), (rnd, i)
+            assert np.array_equal(got["distance"], want_d), (rnd, i)
+
+
 @pytest.mark.parametrize("method", [0, 1], ids=["popcount", "mfma"])
 @pytest.mark.parametrize("nq,ndb", [(1, 1), (7, 300), (1000, 4097), (300, 70000), (5000, 0), (4100, 9000),
                                     (257, 63), (65, 129)])
