@@ -142,8 +142,13 @@ __global__ __launch_bounds__(256) void k_bucket_search(const uint8_t* __restrict
             query_operands(d0, d1, g, bq[qb]);
         }
         uint32_t k1[4], k2[4];
+        /* a result can enter the top-2 only if its dot reaches thr = 256 - (k2 >> 22) */
+        int thr[4];
 #pragma unroll
-        for (int qb = 0; qb < 4; qb++) k1[qb] = k2[qb] = kNoKey;
+        for (int qb = 0; qb < 4; qb++) {
+            k1[qb] = k2[qb] = kNoKey;
+            thr[qb] = 256 - (int)(kNoKey >> 22);
+        }
         const int j0 = offsets[b], j1 = offsets[b + 1];
         for (int jt = j0; jt < j1; jt += kLxTile) {
             __syncthreads();
@@ -177,9 +182,15 @@ __global__ __launch_bounds__(256) void k_bucket_search(const uint8_t* __restrict
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++)
                         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
+                    /* dots are even: below thr, all 4 keys lie above k2 */
+                    const int m = max(max(acc[0], acc[1]), max(acc[2], acc[3]));
+                    if (m >= thr[qb]) {
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (jt + rb + r < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)s_idx[rb + r]));
+                        for (int r = 0; r < 4; r++)
+                            if (jt + rb + r < j1)
+                                top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)s_idx[rb + r]));
+                        thr[qb] = 256 - (int)(k2[qb] >> 22);
+                    }
                 }
             }
         }
