@@ -753,9 +753,12 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
    segment reads its 64 bitmap words (and clears them for the next call),
    compacts the marked pixels in x order, checks one per lane and writes the
    survivors of the segment in x order. */
+/* The square of radius r <= R around x (R is the wavefront's largest radius,
+   so every lane loads the same (2R+1)^2 neighbourhood in one batch and
+   compares only inside its own radius). */
 template <int R>
 __device__ __forceinline__ bool nms_suppressed(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0,
-                                               int x, uint32_t s, uint32_t epoch)
+                                               int x, uint32_t s, uint32_t epoch, int r)
 {
     bool sup = false;
 #pragma unroll
@@ -768,7 +771,8 @@ __device__ __forceinline__ bool nms_suppressed(const ORB_GLOBAL unsigned long lo
             if (dx == 0 && dy == 0) continue;
             const uint32_t ns = (uint32_t)(nk[dx + R] >> 3) & 0x7fffffffu;
             const bool live = (uint32_t)(nk[dx + R] >> kEpochShift) == epoch;
-            sup |= live && (ns > s || (ns == s && dx <= 0 && dy <= 0));
+            const bool inside = abs(dx) <= r && abs(dy) <= r;
+            sup |= inside && live && (ns > s || (ns == s && dx <= 0 && dy <= 0));
         }
     }
     return sup;
@@ -822,35 +826,42 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
         Candidate cd;
         cd.pos = 0;
         cd.score = 0;
+        /* the lane's marked pixel and its key's radius (0: no live key) */
+        int x = 0, r = 0;
+        uint32_t s = 0, lv = 0;
         if (j < total) {
-            const int x = s_x[wid][j];
+            x = s_x[wid][j];
             const unsigned long long key = rowp[x];
             if ((uint32_t)(key >> kEpochShift) == epoch) {
-                const uint32_t s = (uint32_t)(key >> 3) & 0x7fffffffu;
-                const uint32_t lv = 7u - (uint32_t)(key & 7u);
-                const int r = F.radius[lv];
-                bool sup;
-                switch (r) {
-                case 1: sup = nms_suppressed<1>(rowp, F.p0, x, s, epoch); break;
-                case 2: sup = nms_suppressed<2>(rowp, F.p0, x, s, epoch); break;
-                case 3: sup = nms_suppressed<3>(rowp, F.p0, x, s, epoch); break;
-                case 4: sup = nms_suppressed<4>(rowp, F.p0, x, s, epoch); break;
-                default:
-                    sup = false;
-                    for (int dy = -r; dy <= r; dy++)
-                        for (int dx = -r; dx <= r; dx++) {
-                            if (dx == 0 && dy == 0) continue;
-                            const unsigned long long nk = rowp[(long long)dy * F.p0 + x + dx];
-                            const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
-                            if ((uint32_t)(nk >> kEpochShift) == epoch && (ns > s || (ns == s && dx <= 0 && dy <= 0)))
-                                sup = true;
-                        }
-                }
-                if (!sup) {
-                    surv = true;
-                    cd.pos = (lv << 28) | ((uint32_t)y << 14) | (uint32_t)x;
-                    cd.score = s;
-                }
+                s = (uint32_t)(key >> 3) & 0x7fffffffu;
+                lv = 7u - (uint32_t)(key & 7u);
+                r = F.radius[lv];
+            }
+        }
+        int rw = r;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) rw = max(rw, __shfl_xor(rw, o, 64));
+        bool sup = false;
+        if (r > 0) {
+            switch (rw) {
+            case 1: sup = nms_suppressed<1>(rowp, F.p0, x, s, epoch, r); break;
+            case 2: sup = nms_suppressed<2>(rowp, F.p0, x, s, epoch, r); break;
+            case 3: sup = nms_suppressed<3>(rowp, F.p0, x, s, epoch, r); break;
+            case 4: sup = nms_suppressed<4>(rowp, F.p0, x, s, epoch, r); break;
+            default:   /* radii above 4 (scale-2 octaves >= 3) */
+                for (int dy = -r; dy <= r; dy++)
+                    for (int dx = -r; dx <= r; dx++) {
+                        if (dx == 0 && dy == 0) continue;
+                        const unsigned long long nk = rowp[(long long)dy * F.p0 + x + dx];
+                        const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
+                        if ((uint32_t)(nk >> kEpochShift) == epoch && (ns > s || (ns == s && dx <= 0 && dy <= 0)))
+                            sup = true;
+                    }
+            }
+            if (!sup) {
+                surv = true;
+                cd.pos = (lv << 28) | ((uint32_t)y << 14) | (uint32_t)x;
+                cd.score = s;
             }
         }
         const unsigned long long m = __ballot(surv);
