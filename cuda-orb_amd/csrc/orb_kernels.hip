@@ -885,6 +885,9 @@ __device__ __forceinline__ void emit_point(ORB_GLOBAL orb::OrbPoint* o, Candidat
     o->distance = -1;
 }
 
+/* segment offsets a select workgroup keeps in LDS (3840x2160 needs 4196) */
+constexpr int kSelSegs = 6144;
+
 /* One workgroup per frame: prefix sum of the NMS segment counts and the
    keypoints written in raster order.  With more than max_pts survivors, an
    exact radix select finds the score of the max_pts-th best; all higher
@@ -895,6 +898,7 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
     __shared__ int s_tmp[32];
     __shared__ int s_hist[256];
     __shared__ int s_pick[2];
+    __shared__ int s_off[kSelSegs + 1];
     const int f = blockIdx.x;
     const FrameDev& F = frames[f];
     const int nb = F.nms_rows * F.nms_segs, first = F.nms_first;
@@ -909,7 +913,25 @@ __global__ __launch_bounds__(1024) void k_select(const FrameDev* __restrict__ fr
     int total;
     int out = block_scan<1024>(sum, s_tmp, total);
 
-    if (total <= cap) {
+    if (total <= cap && nb <= kSelSegs) {
+        /* every output slot in parallel: segment offsets in LDS, the slot's
+           segment by binary search (the last offset at or below the slot) */
+        for (int b = b0; b < b1; b++) {
+            s_off[b] = out;
+            out += blk_count[first + b];
+        }
+        if (tid == 0) s_off[nb] = total;
+        __syncthreads();
+        for (int k = tid; k < total; k += 1024) {
+            int lo = 0, hi = nb - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= k) lo = mid;
+                else hi = mid - 1;
+            }
+            emit_point(as_global(F.pts) + k, cand[(long long)(first + lo) * kNmsCap + (k - s_off[lo])]);
+        }
+    } else if (total <= cap) {
         for (int b = b0; b < b1; b++) {
             const int n = blk_count[first + b];
             const Candidate* cb = cand + (long long)(first + b) * kNmsCap;
