@@ -1127,8 +1127,7 @@ __device__ __forceinline__ int sample_lds(const uint8_t* bp, int2 pt, float sn, 
 __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ frames, int blocks_per_frame, Tables T,
                                                   int compute_desc)
 {
-    __shared__ int2 s_pat[512];
-    __shared__ int s_umax[32];
+    __shared__ int s_umax[kDescPerBlock][32];
     __shared__ uint8_t s_code[kDescPerBlock][128];
     __shared__ __align__(16) uint8_t s_raw[kDescPerBlock][31 * kRawRow];
     __shared__ __align__(16) uint8_t s_blur[kDescPerBlock][37 * kBlurRow];
@@ -1137,12 +1136,11 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
     const int f = bid / blocks_per_frame;
     const int lb = bid - f * blocks_per_frame;
     const FrameDev& F = frames[f];
-    for (int i = threadIdx.x; i < 512; i += 256) s_pat[i] = T.pattern[i];
-    if (threadIdx.x < 17) s_umax[threadIdx.x] = T.umax[threadIdx.x];
-    __syncthreads();
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int hk = T.half_patch, side = 2 * hk + 1;
-    /* the count and the wavefront's first point load together */
+    /* the wavefronts work independently (no workgroup barrier): the count,
+       the first point, this lane's pattern points (WTA_K 2) and the wave's
+       copy of umax are all requested at once */
     int pi = lb * kDescPerBlock + wid;
     int px = 0, py = 0, pl = 0;
     if (pi < F.cap) {
@@ -1152,6 +1150,16 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
         pl = P->octave;
     }
     const int n = *as_global(F.count);
+    int2 pat2[8];
+    if (T.wta_k == 2) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            pat2[2 * j] = as_global(T.pattern)[2 * (lane + 64 * j)];
+            pat2[2 * j + 1] = as_global(T.pattern)[2 * (lane + 64 * j) + 1];
+        }
+    }
+    if (lane < 17) s_umax[wid][lane] = T.umax[lane];
+    wave_sync_lds();
     for (; pi < n; pi += blocks_per_frame * kDescPerBlock) {
         ORB_GLOBAL orb::OrbPoint* P = as_global(F.pts) + pi;
         if (pi >= lb * kDescPerBlock + wid + blocks_per_frame * kDescPerBlock) {
@@ -1207,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
         int m10 = 0, m01 = 0;
         for (int i = lane; i < side * side; i += 64) {
             const int v = i / side - hk, u = i % side - hk;
-            if (abs(u) <= s_umax[abs(v)]) {
+            if (abs(u) <= s_umax[wid][abs(v)]) {
                 const int I = rc[v * kRawRow + u];
                 m10 += u * I;
                 m01 += v * I;
@@ -1226,8 +1234,7 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
                 ORB_GLOBAL unsigned long long* outd = (ORB_GLOBAL unsigned long long*)(as_global(F.desc) + (long long)pi * 32);
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const int k = lane + 64 * j;
-                    const bool bit = sample_lds(bp, s_pat[2 * k], sn, cs) < sample_lds(bp, s_pat[2 * k + 1], sn, cs);
+                    const bool bit = sample_lds(bp, pat2[2 * j], sn, cs) < sample_lds(bp, pat2[2 * j + 1], sn, cs);
                     const unsigned long long m = __ballot(bit);
                     if (lane == 0) outd[j] = m;
                 }
@@ -1237,14 +1244,14 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
                     const int q = lane + 64 * j, byte = q >> 2, slot = q & 3;
                     int code;
                     if (T.wta_k == 3) {
-                        const int2* pp = s_pat + 12 * byte;
+                        const ORB_GLOBAL int2* pp = as_global(T.pattern) + 12 * byte;
                         const int i0 = slot < 3 ? 3 * slot : 8, i1 = slot < 3 ? 3 * slot + 1 : 9,
                                   i2 = slot < 3 ? 3 * slot + 2 : 11;
                         const int t0 = sample_lds(bp, pp[i0], sn, cs), t1 = sample_lds(bp, pp[i1], sn, cs),
                                   t2 = sample_lds(bp, pp[i2], sn, cs);
                         code = t2 > t1 ? (t2 > t0 ? 2 : 0) : (t1 > t0);
                     } else {
-                        const int2* pp = s_pat + 16 * byte + 4 * slot;
+                        const ORB_GLOBAL int2* pp = as_global(T.pattern) + 16 * byte + 4 * slot;
                         int t0 = sample_lds(bp, pp[0], sn, cs), t1 = sample_lds(bp, pp[1], sn, cs);
                         int t2 = sample_lds(bp, pp[2], sn, cs), t3 = sample_lds(bp, pp[3], sn, cs);
                         int u = 0, v = 2;
