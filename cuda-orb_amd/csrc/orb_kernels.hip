@@ -2,7 +2,8 @@
  * gfx950 kernels of the ORB detect / describe path.
  *
  * One launch per stage covers every (frame, level) of a frame set:
- *   k_pyramid   raw + 7x7-blurred levels from the input, LDS halo tiles
+ *   k_pyramid   raw + 7x7-blurred levels from the input, one wavefront per
+ *               strip walking its rows in registers, neighbours across lanes
  *               (orbd.cu:514-535 pyramid, orbd.cu:1273-1338 blur)
  *   k_detect    FAST-9 segment test + Harris / FAST score on every level,
  *               64-bit atomic max into the level-0 key map (orbd.cu:672-756)
@@ -14,6 +15,8 @@
  *               radix select of the score cut
  *   k_describe  intensity-centroid angle + rBRIEF, one wavefront per keypoint
  *               (orbd.cu:1001-1036, 1341-1468)
+ *   k_export    OpenCV cv::KeyPoint-layout rows of a keypoint set, honouring
+ *               retain_topn
  * Every floating-point step is the explicit IEEE operation (__f*_rn) the CPU
  * oracle performs, so results are bit-identical to it.
  */

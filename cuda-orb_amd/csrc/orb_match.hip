@@ -5,8 +5,10 @@
  * 32-thread block that re-reads the whole train set.  Here both matchers cut
  * the (query, database) plane into units of a query tile x a database chunk,
  * sized on the device from the real counts so a grid launched for the
- * capacities still spreads the actual work over every CU; a merge pass
- * combines the per-chunk results.
+ * capacities still spreads the actual work over every CU.  The popcount
+ * matcher merges the chunks inside its one launch (device-scope atomic
+ * words, see k_match_pop); the MFMA matcher, used for large dense problems,
+ * writes per-chunk results that a merge pass combines.
  *
  * Each (query, entry) pair becomes a packed key (distance << 23 | index): the
  * minimum key is the nearest entry with ties to the lowest index, and the
@@ -18,7 +20,8 @@
  *     popcount(a ^ b) = (256 - <a', b'>) / 2 with a', b' the +/-1 expansions
  *     of the bits, computed by v_mfma_i32_16x16x64_i8 (INT8 in, INT32
  *     accumulate).  A workgroup keeps 256 queries as +/-1 B operands in
- *     registers and unpacks each 64-entry database tile into LDS once.
+ *     registers and unpacks each 64-entry database tile into LDS once; a
+ *     lane builds keys only for results that can still enter its top-2.
  */
 #include "orb_internal.h"
 #include "orb_match_common.h"
