@@ -21,6 +21,7 @@
  * oracle performs, so results are bit-identical to it.
  */
 #include "orb_internal.h"
+#include "orb_wave.h"
 
 namespace orbk
 {
@@ -149,14 +150,6 @@ __device__ __forceinline__ int from_l0(const FrameDev& F, const int* __restrict_
     if (m < x0 && v + 1 < n) v++;
     else if (m > x0 && v > 0) v--;
     return v;
-}
-
-/* LDS written by lanes of this wavefront is visible to its other lanes */
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ int wave_sum(int v)

@@ -9,6 +9,7 @@
 #include "orb_internal.h"
 #include "orb_match_common.h"
 #include "orb_mlp.h"
+#include "orb_wave.h"
 
 namespace orbk
 {
@@ -29,34 +30,58 @@ __global__ void k_bucket_hist(const int* __restrict__ assign, int n, int* __rest
     if (i < n) atomicAdd(&hist[assign[i]], 1);
 }
 
-/* single workgroup: exclusive scan of nb counts (nb <= 4096) into offsets[0..nb] */
+/* single workgroup: exclusive scan of nb counts (nb <= 4096) into offsets[0..nb]
+   and of the unit counts ceil(count / unit) into units[0..nb] -- taken in the
+   bucket order order[] when given; thread t holds positions 4t .. 4t+3 */
 __global__ __launch_bounds__(1024) void k_scan_offsets(const int* __restrict__ count, int nb, int* __restrict__ offsets,
-                                                       int* __restrict__ cursor, int unit, int* __restrict__ units)
+                                                       int* __restrict__ cursor, int unit, int* __restrict__ units,
+                                                       const int* __restrict__ order)
 {
-    __shared__ int s[4096 + 1];
-    __shared__ int su[4096 + 1];
-    for (int i = threadIdx.x; i < nb; i += 1024) {
-        s[i] = count[i];
-        su[i] = unit > 0 ? (count[i] + unit - 1) / unit : 0;
+    __shared__ int s_wc[16], s_wu[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int c[4], u[4], sc = 0, su = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int i = 4 * t + j;
+        c[j] = i < nb ? count[i] : 0;
+        const int cu = order ? (i < nb ? count[order[i]] : 0) : c[j];
+        u[j] = unit > 0 ? (cu + unit - 1) / unit : 0;
+        sc += c[j];
+        su += u[j];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int a = 0, b = 0;
-        for (int i = 0; i < nb; i++) {
-            const int c = s[i], u = su[i];
-            s[i] = a;
-            su[i] = b;
-            a += c;
-            b += u;
+    int xc = sc, xu = su;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int yc = __shfl_up(xc, o, 64), yu = __shfl_up(xu, o, 64);
+        if (lane >= o) {
+            xc += yc;
+            xu += yu;
         }
-        s[nb] = a;
-        su[nb] = b;
+    }
+    if (lane == 63) {
+        s_wc[w] = xc;
+        s_wu[w] = xu;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i <= nb; i += 1024) {
-        offsets[i] = s[i];
-        if (cursor && i < nb) cursor[i] = s[i];
-        if (units) units[i] = su[i];
+    int ac = xc - sc, au = xu - su;
+    for (int i = 0; i < w; i++) {
+        ac += s_wc[i];
+        au += s_wu[i];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int i = 4 * t + j;
+        if (i < nb) {
+            offsets[i] = ac;
+            if (cursor) cursor[i] = ac;
+            if (units) units[i] = au;
+        }
+        ac += c[j];
+        au += u[j];
+    }
+    if (t == 1023) {
+        offsets[nb] = ac;
+        if (units) units[nb] = au;
     }
 }
 
@@ -89,44 +114,77 @@ __global__ __launch_bounds__(256) void k_kmodes_update(const uint8_t* __restrict
     if ((k & 63) == 0) ((unsigned long long*)(centroids + (long long)b * 32))[k >> 6] = m;
 }
 
-__global__ void k_probe_count(const int* __restrict__ probes, int n, int* __restrict__ count)
+constexpr int kPlChunk = 8192; /* probe-list entries per workgroup */
+
+/* bucket histogram of a chunk of probe entries in LDS, then one global add per
+   bucket (nb <= 1024) */
+__global__ __launch_bounds__(256) void k_probe_count(const int* __restrict__ probes, int n, int nb,
+                                                     int* __restrict__ count)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&count[probes[i]], 1);
+    __shared__ int h[1024];
+    for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
+    __syncthreads();
+    const int i0 = blockIdx.x * kPlChunk, i1 = min(n, i0 + kPlChunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) atomicAdd(&h[probes[i]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 256)
+        if (h[b]) atomicAdd(&count[b], h[b]);
 }
 
-__global__ void k_probe_fill(const int* __restrict__ probes, int n, int* __restrict__ cursor, int* __restrict__ lists)
+/* the same chunk again: one global range reserved per bucket, entries placed
+   inside it through LDS cursors (i = query * nprobe + slot) */
+__global__ __launch_bounds__(256) void k_probe_fill(const int* __restrict__ probes, int n, int nb,
+                                                    int* __restrict__ cursor, int* __restrict__ lists)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) lists[atomicAdd(&cursor[probes[i]], 1)] = i; /* i = query * nprobe + slot */
+    __shared__ int h[1024];
+    for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
+    __syncthreads();
+    const int i0 = blockIdx.x * kPlChunk, i1 = min(n, i0 + kPlChunk);
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) atomicAdd(&h[probes[i]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 256)
+        if (h[b]) h[b] = atomicAdd(&cursor[b], h[b]);
+    __syncthreads();
+    for (int i = i0 + threadIdx.x; i < i1; i += 256) lists[atomicAdd(&h[probes[i]], 1)] = i;
 }
 
 /* Each unit: up to 256 list entries (query, slot) of one bucket x all of the
    bucket's entries.  A wave holds 64 queries as +/-1 B operands; each 64-entry
-   tile of the bucket is unpacked into LDS once per workgroup. */
-__global__ __launch_bounds__(256) void k_bucket_search(const uint8_t* __restrict__ q, int nprobe,
-                                                       const int* __restrict__ lists,
-                                                       const int* __restrict__ list_offsets,
-                                                       const int* __restrict__ unit_offsets, int nb,
-                                                       const uint8_t* __restrict__ db_sorted,
-                                                       const int* __restrict__ perm, const int* __restrict__ offsets,
-                                                       uint2* __restrict__ partial)
+   tile of the bucket is unpacked into LDS once per workgroup.  Units are
+   numbered along order[] (buckets by decreasing size) and handed out through
+   the counter *next, so the largest come first and the workgroups finish
+   together. */
+__global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restrict__ q, int nprobe,
+                                                          const int* __restrict__ lists,
+                                                          const int* __restrict__ list_offsets,
+                                                          const int* __restrict__ unit_offsets,
+                                                          const int* __restrict__ order, int nb,
+                                                          const uint8_t* __restrict__ db_sorted,
+                                                          const int* __restrict__ perm,
+                                                          const int* __restrict__ offsets, int* __restrict__ next,
+                                                          uint2* __restrict__ partial)
 {
     __shared__ __align__(16) int8_t s_db[kLxTile * kLxRow];
     __shared__ int s_idx[kLxTile];
+    __shared__ int s_u;
     const int total_units = unit_offsets[nb];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int g = lane >> 4, col = lane & 15;
-    for (int u = blockIdx.x; u < total_units; u += gridDim.x) {
-        /* bucket of unit u: the last b with unit_offsets[b] <= u */
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_u = atomicAdd(next, 1);
+        __syncthreads();
+        const int u = s_u;
+        if (u >= total_units) break;
+        /* position of unit u in the bucket order: the last i with unit_offsets[i] <= u */
         int lo = 0, hi = nb - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (unit_offsets[mid] <= u) lo = mid;
             else hi = mid - 1;
         }
-        const int b = lo;
-        const int e0 = list_offsets[b] + (u - unit_offsets[b]) * kLxQ, e1 = min(list_offsets[b + 1], e0 + kLxQ);
+        const int b = order[lo];
+        const int e0 = list_offsets[b] + (u - unit_offsets[lo]) * kLxQ, e1 = min(list_offsets[b + 1], e0 + kLxQ);
         v4i bq[4][4];
         int ent[4];
 #pragma unroll
@@ -275,39 +333,28 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
                                                         int* __restrict__ probes, float* __restrict__ pmax)
 {
     __shared__ uint4 s_c[1024 * 2];
+    __shared__ uint2 s_cand[4][1024];
     for (int i = threadIdx.x; i < nb * 2; i += 256) s_c[i] = ((const uint4*)cen)[i];
     __syncthreads();
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wv;
     if (r >= nq) return;
     const uint4 qa = ((const uint4*)(q + (long long)r * 32))[0], qb = ((const uint4*)(q + (long long)r * 32))[1];
-    constexpr int kPer = 16;
-    int d[kPer];
+    /* key 0x10000 - distance: nearer centroids rank higher; 0 marks no centroid */
+    uint32_t key[16];
 #pragma unroll
-    for (int t = 0; t < kPer; t++) {
+    for (int t = 0; t < 16; t++) {
         const int c = lane + 64 * t;
         if (c < nb) {
             const uint4 a = s_c[2 * c], b = s_c[2 * c + 1];
-            d[t] = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
-                   __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
+            const int d = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
+                          __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
+            key[t] = 0x10000u - (uint32_t)d;
         } else {
-            d[t] = 1 << 20;
+            key[t] = 0u;
         }
     }
-    for (int j = 0; j < k; j++) {
-        int bv = 1 << 21, bi = 0x7fffffff;
-#pragma unroll
-        for (int t = 0; t < kPer; t++) {
-            const int c = lane + 64 * t;
-            if (d[t] < bv || (d[t] == bv && c < bi)) { bv = d[t]; bi = c; }
-        }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const int ov = __shfl_xor(bv, o, 64), oi = __shfl_xor(bi, o, 64);
-            if (ov < bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-        }
-        if (lane == 0) probes[(long long)r * k + j] = bi;
-        if ((bi & 63) == lane) d[bi >> 6] = 1 << 20;
-    }
+    wave_topk(key, k, s_cand[wv], probes + (long long)r * k);
     if (lane == 0) pmax[r] = 1.f;
 }
 
@@ -339,7 +386,8 @@ hipError_t launch_bucket_hist(const int* assign, int n, int* hist, hipStream_t s
 
 hipError_t launch_bucket_offsets(const int* hist, int nb, int* offsets, int* cursor, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, hist, nb, offsets, cursor, 0, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, hist, nb, offsets, cursor, 0, (int*)nullptr,
+                       (const int*)nullptr);
     return hipGetLastError();
 }
 
@@ -357,24 +405,27 @@ hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, in
     return hipGetLastError();
 }
 
-hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, int* list_count, int* list_offsets,
-                              int* list_cursor, int* lists, int* unit_offsets, hipStream_t s)
+hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, const int* order, int* list_count,
+                              int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, hipStream_t s)
 {
     const long long n = (long long)nq * nprobe;
-    hipLaunchKernelGGL(k_zero_int, grid1(nb), dim3(256), 0, s, list_count, nb);
-    if (n > 0) hipLaunchKernelGGL(k_probe_count, grid1(n), dim3(256), 0, s, probes, (int)n, list_count);
+    hipLaunchKernelGGL(k_zero_int, grid1(nb + 1), dim3(256), 0, s, list_count, nb + 1);
+    if (nb > 1024) return hipErrorInvalidValue;
+    const dim3 chunks((unsigned)((n + kPlChunk - 1) / kPlChunk));
+    if (n > 0) hipLaunchKernelGGL(k_probe_count, chunks, dim3(256), 0, s, probes, (int)n, nb, list_count);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, list_count, nb, list_offsets, list_cursor, kLxQ,
-                       unit_offsets);
-    if (n > 0) hipLaunchKernelGGL(k_probe_fill, grid1(n), dim3(256), 0, s, probes, (int)n, list_cursor, lists);
+                       unit_offsets, order);
+    if (n > 0) hipLaunchKernelGGL(k_probe_fill, chunks, dim3(256), 0, s, probes, (int)n, nb, list_cursor, lists);
     return hipGetLastError();
 }
 
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
-                                const int* unit_offsets, int nb, const uint8_t* db_sorted, const int* perm,
-                                const int* offsets, uint2* partial, int blocks, hipStream_t s)
+                                const int* unit_offsets, const int* order, int nb, const uint8_t* db_sorted,
+                                const int* perm, const int* offsets, int* next, uint2* partial, int blocks,
+                                hipStream_t s)
 {
     hipLaunchKernelGGL(k_bucket_search, dim3(blocks), dim3(256), 0, s, q, nprobe, lists, list_offsets, unit_offsets,
-                       nb, db_sorted, perm, offsets, partial);
+                       order, nb, db_sorted, perm, offsets, next, partial);
     return hipGetLastError();
 }
 

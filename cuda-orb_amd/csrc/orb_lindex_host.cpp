@@ -8,6 +8,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,10 +74,13 @@ struct orb_lindex {
     orb_ctx* ctx = nullptr;
     orb_lindex_params prm{};
     int H = 0, C = 0;
+    int ncu = 256; /* compute units of the device (sizes the bucket-search grid) */
     /* parameters: offsets of W1, b1, W2, b2, W3, b3 in one fp32 vector */
     size_t off[7] = {};
     DBuf<float> w, g, m, v;
     DBuf<uint16_t> w16; /* bf16 copies of W1, W2, W3 (same offsets minus biases is awkward: full copy) */
+    DBuf<uint16_t> w16t; /* W1^T (H x 256), W2^T (H x H), W3^T (C x H): K-contiguous forward operands */
+    bool wt_fresh = false; /* w16t matches w16 */
     int step = 0;
     /* batch workspace */
     int cap_batch = 0;
@@ -89,6 +93,7 @@ struct orb_lindex {
     DBuf<uint16_t> cpm;      /* centroids as +/-centroid_scale bf16, nbuckets x 256 */
     uint16_t cscale16 = 0;   /* bf16 bits of centroid_scale (0: no centroid term) */
     DBuf<int> assign, hist, offsets, cursor, perm;
+    DBuf<int> border; /* bucket ids by decreasing size (ties by id): the search's unit order */
     /* query workspace */
     DBuf<int> probes, list_count, list_offsets, list_cursor, lists, unit_offsets, fb_count, fb_index, fb_qidx;
     DBuf<int> ex_idx1, ex_d1, ex_idx2, ex_d2, tmp_idx;
@@ -169,32 +174,44 @@ GemmArgs gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B
 }
 
 /* forward of B rows already expanded in L->X: Z1, A1, Z2, A2, Z3 */
-int forward(orb_lindex* L, int B, hipStream_t s)
+/* Network forward of B rows of X into Z3 (logits); keep_z also stores the
+   pre-activations Z1, Z2 the backward pass needs. */
+int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z)
 {
     const int H = L->H, C = L->C;
     const float* w = L->w.p;
     const uint16_t* w16 = L->w16.p;
-    GemmArgs g = gemm(L->X.p, kIn, 1, w16 + L->off[0], H, 1, B, H, kIn);
+    uint16_t* wt = L->w16t.p;
+    const size_t t1 = (size_t)kIn * H, t2 = t1 + (size_t)H * H;
+    if (!L->wt_fresh) {
+        LTRY(L, launch_transpose16(w16 + L->off[0], kIn, H, wt, s));
+        LTRY(L, launch_transpose16(w16 + L->off[2], H, H, wt + t1, s));
+        LTRY(L, launch_transpose16(w16 + L->off[4], H, C, wt + t2, s));
+        L->wt_fresh = true;
+    }
+    GemmArgs g = gemm(L->X.p, kIn, 1, wt, 1, kIn, B, H, kIn);
     g.bias = w + L->off[1];
     g.gelu = 1;
-    g.out32 = L->Z1.p;
+    g.out32 = keep_z ? L->Z1.p : nullptr;
     g.out16 = L->A1.p;
     LTRY(L, launch_gemm(g, s));
-    g = gemm(L->A1.p, H, 1, w16 + L->off[2], H, 1, B, H, H);
+    g = gemm(L->A1.p, H, 1, wt + t1, 1, H, B, H, H);
     g.bias = w + L->off[3];
     g.gelu = 1;
-    g.out32 = L->Z2.p;
+    g.out32 = keep_z ? L->Z2.p : nullptr;
     g.out16 = L->A2.p;
     LTRY(L, launch_gemm(g, s));
+    g = gemm(L->A2.p, H, 1, wt + t2, 1, H, B, C, H);
     if (L->cscale16) {
-        /* centroid term: Z3 = <x, c_b> * centroid_scale, exact in fp32 */
+        /* centroid term <x, c_b> * centroid_scale as a second K segment of the
+           same product (each of its terms is exact in fp32) */
         if (!L->cpm.p) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
-        g = gemm(L->X.p, kIn, 1, L->cpm.p, 1, kIn, B, C, kIn);
-        g.out32 = L->Z3.p;
-        LTRY(L, launch_gemm(g, s));
+        g.A2 = L->X.p;
+        g.sa2m = kIn;
+        g.B2 = L->cpm.p;
+        g.sb2n = kIn;
+        g.K2 = kIn;
     }
-    g = gemm(L->A2.p, H, 1, w16 + L->off[4], C, 1, B, C, H);
-    g.addin = L->cscale16 ? L->Z3.p : nullptr;
     g.bias = w + L->off[5];
     g.out32 = L->Z3.p;
     LTRY(L, launch_gemm(g, s));
@@ -240,6 +257,29 @@ int backward(orb_lindex* L, int B, hipStream_t s)
     return ORB_OK;
 }
 
+/* the nprobe buckets of each of nq queries and their max softmax probability
+   (1 without the network) */
+int compute_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* probes, float* pmax, hipStream_t s)
+{
+    const int np = L->prm.nprobe, nb = L->C;
+    if (mode == ORB_LINDEX_MLP) {
+        /* network forward in chunks, top-nprobe buckets and max softmax */
+        const int chunk = 65536;
+        int rc = ensure_batch(L, nq < chunk ? nq : chunk);
+        if (rc != ORB_OK) return rc;
+        for (int c0 = 0; c0 < nq; c0 += chunk) {
+            const int B = nq - c0 < chunk ? nq - c0 : chunk;
+            LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
+            rc = forward(L, B, s, false);
+            if (rc != ORB_OK) return rc;
+            LTRY(L, launch_probe(L->Z3.p, B, nb, np, probes + (size_t)c0 * np, pmax + c0, s));
+        }
+    } else {
+        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, probes, pmax, s));
+    }
+    return ORB_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -255,6 +295,12 @@ int orb_lindex_create(orb_ctx* ctx, const orb_lindex_params* p, orb_lindex** out
     if (p->nprobe < 1 || p->nprobe > p->nbuckets) return orb__ctx_fail(ctx, ORB_E_ARG, "nprobe out of range");
     orb_lindex* L = new orb_lindex();
     L->ctx = ctx;
+    {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            L->ncu = ncu;
+    }
     L->prm = *p;
     L->H = p->hidden;
     L->C = p->nbuckets;
@@ -263,7 +309,8 @@ int orb_lindex_create(orb_ctx* ctx, const orb_lindex_params* p, orb_lindex** out
     L->off[0] = 0;
     for (int i = 0; i < 6; i++) L->off[i + 1] = L->off[i] + sizes[i];
     const size_t np = nparams(L);
-    if (!L->w.ensure(np) || !L->g.ensure(np) || !L->m.ensure(np) || !L->v.ensure(np) || !L->w16.ensure(np)) {
+    if (!L->w.ensure(np) || !L->g.ensure(np) || !L->m.ensure(np) || !L->v.ensure(np) || !L->w16.ensure(np) ||
+        !L->w16t.ensure((size_t)kIn * L->H + (size_t)L->H * L->H + (size_t)L->H * L->C)) {
         delete L;
         return orb__ctx_fail(ctx, ORB_E_NOMEM, "device allocation failed");
     }
@@ -302,6 +349,11 @@ int orb_lindex_num_params(const orb_lindex* L)
     return L ? (int)nparams(L) : 0;
 }
 
+int orb_lindex_nprobe(const orb_lindex* L)
+{
+    return L ? L->prm.nprobe : 0;
+}
+
 int orb_lindex_set_weights(orb_lindex* L, const float* h_w)
 {
     if (!L || !h_w) return lfail(L, ORB_E_ARG, "null argument");
@@ -311,6 +363,7 @@ int orb_lindex_set_weights(orb_lindex* L, const float* h_w)
     LTRY(L, hipMemsetAsync(L->m.p, 0, sizeof(float) * np, s));
     LTRY(L, hipMemsetAsync(L->v.p, 0, sizeof(float) * np, s));
     LTRY(L, launch_to_bf16(L->w.p, L->w16.p, (long long)np, s));
+    L->wt_fresh = false;
     LTRY(L, hipStreamSynchronize(s));
     L->step = 0;
     return ORB_OK;
@@ -367,6 +420,16 @@ int orb_lindex_build(orb_lindex* L, const uint8_t* d_db, int ndb, int iters)
     }
     int rc = make_cpm(L, s);
     if (rc != ORB_OK) return rc;
+    {
+        std::vector<int> off(nb + 1), ord(nb);
+        LTRY(L, hipMemcpyAsync(off.data(), L->offsets.p, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, s));
+        LTRY(L, hipStreamSynchronize(s));
+        for (int b = 0; b < nb; b++) ord[b] = b;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+        LALLOC(L, L->border, nb);
+        LTRY(L, hipMemcpyAsync(L->border.p, ord.data(), sizeof(int) * nb, hipMemcpyHostToDevice, s));
+    }
     LTRY(L, hipStreamSynchronize(s));
     return ORB_OK;
 }
@@ -394,7 +457,7 @@ int orb_lindex_grads(orb_lindex* L, const uint8_t* d_q, const int* d_labels, int
     LALLOC(L, L->fb_q, (size_t)n * 32);
     LTRY(L, launch_gather(d_q, nullptr, d_labels, n, L->fb_q.p, L->labels.p, s));
     LTRY(L, launch_bits_pm1(L->fb_q.p, n, L->X.p, s));
-    rc = forward(L, n, s);
+    rc = forward(L, n, s, true);
     if (rc != ORB_OK) return rc;
     rc = backward(L, n, s);
     if (rc != ORB_OK) return rc;
@@ -456,7 +519,7 @@ int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int
             LTRY(L, launch_gather(d_q, L->order.p + (size_t)bi * batch, L->tmp_idx.p, batch, L->fb_q.p,
                                   L->labels.p, s));
             LTRY(L, launch_bits_pm1(L->fb_q.p, batch, L->X.p, s));
-            rc = forward(L, batch, s);
+            rc = forward(L, batch, s, true);
             if (rc != ORB_OK) return rc;
             rc = backward(L, batch, s);
             if (rc != ORB_OK) return rc;
@@ -465,6 +528,7 @@ int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int
             const float lr_t = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(3.14159265358979323846 * frac))));
             LTRY(L, launch_adam(L->w.p, L->w16.p, L->g.p, L->m.p, L->v.p, (long long)nparams(L), lr_t, 0.9f, 0.999f,
                                 1e-8f, L->step, s));
+            L->wt_fresh = false;
             if (h_epoch_loss && (bi == nbatch - 1)) {
                 LTRY(L, hipMemcpyAsync(lv.data(), L->loss.p, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
                 LTRY(L, hipStreamSynchronize(s));
@@ -481,13 +545,13 @@ int orb_lindex_logits(orb_lindex* L, const uint8_t* d_q, int n, float* d_logits)
 {
     if (!L || !d_q || !d_logits || n <= 0 || n % 64) return lfail(L, ORB_E_ARG, "n must be a positive multiple of 64");
     hipStream_t s = stream_of(L);
-    const int chunk = 8192;
+    const int chunk = 65536;
     int rc = ensure_batch(L, n < chunk ? n : chunk);
     if (rc != ORB_OK) return rc;
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int B = n - c0 < chunk ? n - c0 : chunk;
         LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
-        rc = forward(L, B, s);
+        rc = forward(L, B, s, false);
         if (rc != ORB_OK) return rc;
         LTRY(L, hipMemcpyAsync(d_logits + (size_t)c0 * L->C, L->Z3.p, sizeof(float) * B * L->C,
                                hipMemcpyDeviceToDevice, s));
@@ -505,7 +569,7 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     const int np = L->prm.nprobe, nb = L->C;
     LALLOC(L, L->probes, (size_t)nq * np);
     LALLOC(L, L->pmax, nq);
-    LALLOC(L, L->list_count, nb);
+    LALLOC(L, L->list_count, nb + 1);
     LALLOC(L, L->list_offsets, nb + 1);
     LALLOC(L, L->list_cursor, nb);
     LALLOC(L, L->unit_offsets, nb + 1);
@@ -519,30 +583,19 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     LALLOC(L, L->ex_d1, nq);
     LALLOC(L, L->ex_idx2, nq);
     LALLOC(L, L->ex_d2, nq);
-    if (mode == ORB_LINDEX_MLP) {
-        /* network forward in chunks, top-nprobe buckets and max softmax */
-        const int chunk = 8192;
-        int rc = ensure_batch(L, nq < chunk ? nq : chunk);
-        if (rc != ORB_OK) return rc;
-        for (int c0 = 0; c0 < nq; c0 += chunk) {
-            const int B = nq - c0 < chunk ? nq - c0 : chunk;
-            LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
-            rc = forward(L, B, s);
-            if (rc != ORB_OK) return rc;
-            LTRY(L, launch_probe(L->Z3.p, B, nb, np, L->probes.p + (size_t)c0 * np, L->pmax.p + c0, s));
-        }
-    } else {
-        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, L->probes.p, L->pmax.p, s));
-    }
-    LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->list_count.p, L->list_offsets.p, L->list_cursor.p,
-                               L->lists.p, L->unit_offsets.p, s));
-    LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, nb, L->db_sorted.p,
-                                 L->perm.p, L->offsets.p, L->partial.p, 2048, s));
+    int rc = compute_probes(L, d_q, nq, mode, L->probes.p, L->pmax.p, s);
+    if (rc != ORB_OK) return rc;
+    LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->border.p, L->list_count.p, L->list_offsets.p,
+                               L->list_cursor.p, L->lists.p, L->unit_offsets.p, s));
+    /* four workgroups per CU stay resident and take units until none are left */
+    LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, L->border.p, nb,
+                                 L->db_sorted.p, L->perm.p, L->offsets.p, L->list_count.p + nb, L->partial.p,
+                                 4 * L->ncu, s));
     const float thresh = mode == ORB_LINDEX_MLP ? L->prm.fallback : -1.f;
     LTRY(L, launch_fallback_compact(d_q, nq, L->pmax.p, thresh, L->fb_count.p, L->fb_index.p, L->fb_qidx.p,
                                     L->fb_q.p, s));
     /* exact search of the low-confidence queries (count read on the device) */
-    int rc = orb__match_dcount_top2(L->ctx, L->fb_q.p, nq, L->fb_count.p, L->db.p, L->ndb, L->ex_idx1.p, L->ex_d1.p,
+    rc = orb__match_dcount_top2(L->ctx, L->fb_q.p, nq, L->fb_count.p, L->db.p, L->ndb, L->ex_idx1.p, L->ex_d1.p,
                                     L->ex_idx2.p, L->ex_d2.p);
     if (rc != ORB_OK) return rc;
     LTRY(L, launch_probe_merge(L->partial.p, nq, np, L->fb_index.p, L->ex_idx1.p, L->ex_d1.p, L->ex_idx2.p,
@@ -552,6 +605,17 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
         LTRY(L, hipStreamSynchronize(s));
     }
     return ORB_OK;
+}
+
+/* The probed buckets of nq queries (nq x nprobe, best first, ties to the
+   lower bucket id) and their max softmax probabilities (1 in centroid mode). */
+int orb_lindex_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d_probes, float* d_pmax)
+{
+    if (!L || !d_q || !d_probes || !d_pmax || nq <= 0 || nq % 64)
+        return lfail(L, ORB_E_ARG, "nq must be a positive multiple of 64");
+    if (L->ndb <= 0) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
+    if (mode != ORB_LINDEX_MLP && mode != ORB_LINDEX_CENTROIDS) return lfail(L, ORB_E_ARG, "unknown mode");
+    return compute_probes(L, d_q, nq, mode, d_probes, d_pmax, stream_of(L));
 }
 
 int orb_lindex_set_probe(orb_lindex* L, int nprobe, float fallback)

@@ -11,7 +11,8 @@ namespace orbk
 {
 
 /* C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak],
-   B(k, n) = B[k*sbk + n*sbn] (bf16 bits); M, N, K multiples of 64.
+   B(k, n) = B[k*sbk + n*sbn] (bf16 bits); M, N, K multiples of 64 (an error
+   otherwise).
    Epilogue, in order: z += addin[m*ldc + n] if addin; z += bias[n];
    z *= gelu'(zmul[m*ldc + n]) if zmul; out32[m*ldc + n] = z;
    out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
@@ -28,9 +29,18 @@ struct GemmArgs {
     float* out32;
     uint16_t* out16;
     long long ldc;
+    /* optional second K segment (both operands K-contiguous, K2 a multiple of
+       64): z also accumulates sum_k A2[m*sa2m + k] B2[n*sb2n + k], k < K2 */
+    const uint16_t* A2;
+    long long sa2m;
+    const uint16_t* B2;
+    long long sb2n;
+    int K2;
 };
 
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
+/* dst (C x R) = src (R x C) transposed (bf16 bits); R and C multiples of 64 */
+hipError_t launch_transpose16(const uint16_t* src, int R, int C, uint16_t* dst, hipStream_t s);
 hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s);
 /* as launch_bits_pm1 with the two values given as bf16 bits (set bit -> pos, clear bit -> neg) */
 hipError_t launch_bits_pm(const uint8_t* desc, int n, uint16_t pos, uint16_t neg, uint16_t* X, hipStream_t s);
@@ -50,14 +60,18 @@ hipError_t launch_bucket_scatter(const int* assign, int n, const uint8_t* db, in
 hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, int nb, uint8_t* centroids,
                                 hipStream_t s);
 
-/* probe lists: queries x nprobe (bucket ids) -> per-bucket lists of q*nprobe+slot */
-hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, int* list_count, int* list_offsets,
-                              int* list_cursor, int* lists, int* unit_offsets, hipStream_t s);
+/* probe lists: queries x nprobe (bucket ids) -> per-bucket lists of q*nprobe+slot;
+   list_count has nb + 1 ints (the last is the search's unit counter, zeroed);
+   unit_offsets counts search units along order[] (buckets by decreasing size) */
+hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, const int* order, int* list_count,
+                              int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, hipStream_t s);
 /* bucketed exact search: for every (query, probed bucket) the top-2 keys
-   (distance << 23 | original index) over that bucket's entries */
+   (distance << 23 | original index) over that bucket's entries; units taken
+   from the zeroed counter *next */
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
-                                const int* unit_offsets, int nb, const uint8_t* db_sorted, const int* perm,
-                                const int* offsets, uint2* partial, int blocks, hipStream_t s);
+                                const int* unit_offsets, const int* order, int nb, const uint8_t* db_sorted,
+                                const int* perm, const int* offsets, int* next, uint2* partial, int blocks,
+                                hipStream_t s);
 /* per query: merge of the probed buckets, or the exact result when the query
    was sent to the fallback (fb_index >= 0) */
 hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,

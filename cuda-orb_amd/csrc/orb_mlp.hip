@@ -10,6 +10,10 @@
  *               transposed copies), epilogue fused: + bias, GELU, or the
  *               GELU-derivative product of the backward pass, FP32 and/or
  *               BF16 stores
+ *   k_gemm_nt   the same product when both operands are contiguous along K
+ *               (forward pass and the data-gradient products): 128 x 128
+ *               tiles, 64-deep K steps double-buffered in LDS
+ *   k_transpose16  bf16 transposes (K-contiguous weight copies for k_gemm_nt)
  *   k_bits_pm1  descriptor bits -> +/-1 BF16 inputs
  *   k_softmax_ce  row softmax, cross-entropy and its gradient (scaled 1/n)
  *   k_colsum    bias gradients
@@ -18,6 +22,7 @@
  */
 #include "orb_internal.h"
 #include "orb_mlp.h"
+#include "orb_wave.h"
 
 namespace orbk
 {
@@ -132,6 +137,187 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     }
 }
 
+/* C = A B with both operands contiguous along K (A(m, k) = A[m*sam + k],
+   B(k, n) = B[n*sbn + k]), plus the optional second K segment A2 B2 in the
+   same accumulators: 128 x 128 workgroup tiles, four wavefronts of
+   64 x 64 (4 x 4 MFMA blocks), 64-deep K steps double-buffered in LDS with the
+   next step's global loads issued before the current step's MFMAs.  Rows
+   beyond M or N are read clamped and never stored.  Same epilogue as k_gemm. */
+constexpr int kNT = 128;
+constexpr int kNK = 64;
+constexpr int kNLd = kNK + 8; /* LDS row (bf16 elements): 144 B, conflict-free b128 reads */
+
+__global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
+{
+    /* A and B tiles (two buffers each); after the loop, the fp32 output tile */
+    __shared__ __align__(16) uint16_t s_lds[4 * kNT * kNLd];
+    uint16_t (*sA)[kNT * kNLd] = (uint16_t (*)[kNT * kNLd])s_lds;
+    uint16_t (*sB)[kNT * kNLd] = (uint16_t (*)[kNT * kNLd])(s_lds + 2 * kNT * kNLd);
+    const int tiles_n = (g.N + kNT - 1) / kNT;
+    const int m0 = (blockIdx.x / tiles_n) * kNT, n0 = (blockIdx.x % tiles_n) * kNT;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
+    const int row = lane & 15, grp = lane >> 4;
+    /* loader: thread -> tile rows lr + 32 j, k offset lk (8 elements) */
+    const int lr = tid >> 3, lk = (tid & 7) * 8;
+    const uint16_t* pa[4];
+    const uint16_t* pb[4];
+    const uint16_t* pa2[4];
+    const uint16_t* pb2[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const long long mr = min(m0 + lr + 32 * j, g.M - 1), nr = min(n0 + lr + 32 * j, g.N - 1);
+        pa[j] = g.A + mr * g.sam + lk;
+        pb[j] = g.B + nr * g.sbn + lk;
+        pa2[j] = g.K2 ? g.A2 + mr * g.sa2m + lk : pa[j];
+        pb2[j] = g.K2 ? g.B2 + nr * g.sb2n + lk : pb[j];
+    }
+    const int sto = (lr * kNLd + lk) * 2; /* byte offset of this thread's LDS stores */
+    char* const lA = (char*)&sA[0][0] + sto;
+    char* const lB = (char*)&sB[0][0] + sto;
+    constexpr int kBuf = kNT * kNLd * 2, kRow32 = 32 * kNLd * 2;
+    const char* rA = (const char*)&sA[0][0] + ((wm + row) * kNLd + 8 * grp) * 2;
+    const char* rB = (const char*)&sB[0][0] + ((wn + row) * kNLd + 8 * grp) * 2;
+    const int nk1 = g.K / kNK, nk = nk1 + g.K2 / kNK;
+    uint4 a0 = *(const uint4*)pa[0], a1 = *(const uint4*)pa[1], a2 = *(const uint4*)pa[2], a3 = *(const uint4*)pa[3];
+    uint4 b0 = *(const uint4*)pb[0], b1 = *(const uint4*)pb[1], b2 = *(const uint4*)pb[2], b3 = *(const uint4*)pb[3];
+    *(uint4*)(lA) = a0;
+    *(uint4*)(lA + kRow32) = a1;
+    *(uint4*)(lA + 2 * kRow32) = a2;
+    *(uint4*)(lA + 3 * kRow32) = a3;
+    *(uint4*)(lB) = b0;
+    *(uint4*)(lB + kRow32) = b1;
+    *(uint4*)(lB + 2 * kRow32) = b2;
+    *(uint4*)(lB + 3 * kRow32) = b3;
+    __syncthreads();
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = (kt & 1) * kBuf, nxt = kBuf - cur;
+        /* the next step's tiles, in flight during this step's MFMAs (the last
+           step reloads its own: harmless, never stored over live data) */
+        const int kn = min(kt + 1, nk - 1);
+        const bool seg2 = kn >= nk1;
+        const int k1 = (seg2 ? kn - nk1 : kn) * kNK;
+        a0 = *(const uint4*)((seg2 ? pa2[0] : pa[0]) + k1);
+        a1 = *(const uint4*)((seg2 ? pa2[1] : pa[1]) + k1);
+        a2 = *(const uint4*)((seg2 ? pa2[2] : pa[2]) + k1);
+        a3 = *(const uint4*)((seg2 ? pa2[3] : pa[3]) + k1);
+        b0 = *(const uint4*)((seg2 ? pb2[0] : pb[0]) + k1);
+        b1 = *(const uint4*)((seg2 ? pb2[1] : pb[1]) + k1);
+        b2 = *(const uint4*)((seg2 ? pb2[2] : pb[2]) + k1);
+        b3 = *(const uint4*)((seg2 ? pb2[3] : pb[3]) + k1);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            bf16x8 av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = *(const bf16x8*)(rA + cur + (16 * a * kNLd + 32 * ks) * 2);
+#pragma unroll
+            for (int b = 0; b < 4; b++) bv[b] = *(const bf16x8*)(rB + cur + (16 * b * kNLd + 32 * ks) * 2);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        *(uint4*)(lA + nxt) = a0;
+        *(uint4*)(lA + nxt + kRow32) = a1;
+        *(uint4*)(lA + nxt + 2 * kRow32) = a2;
+        *(uint4*)(lA + nxt + 3 * kRow32) = a3;
+        *(uint4*)(lB + nxt) = b0;
+        *(uint4*)(lB + nxt + kRow32) = b1;
+        *(uint4*)(lB + nxt + 2 * kRow32) = b2;
+        *(uint4*)(lB + nxt + 3 * kRow32) = b3;
+        __syncthreads();
+    }
+    if (!g.addin && !g.zmul) {
+        /* z = acc + bias staged as a 128 x 128 fp32 tile in LDS (the loop ended
+           on a barrier), then written along rows: 16 B per lane */
+        constexpr int kOLd = kNT + 4;
+        float* st = (float*)s_lds;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int nl = wn + 16 * b + row;
+            const float bias = g.bias && n0 + nl < g.N ? g.bias[n0 + nl] : 0.f;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) st[(wm + 16 * a + 4 * grp + r) * kOLd + nl] = acc[a][b][r] + bias;
+        }
+        __syncthreads();
+        if (g.out32) {
+#pragma unroll 4
+            for (int i = 0; i < 16; i++) {
+                const int c = tid + 256 * i, rl = c >> 5, cl = (c & 31) * 4;
+                const int m = m0 + rl, n = n0 + cl;
+                if (m < g.M && n < g.N)
+                    *(float4*)(g.out32 + (long long)m * g.ldc + n) = *(const float4*)(st + rl * kOLd + cl);
+            }
+        }
+        if (g.out16) {
+#pragma unroll 2
+            for (int i = 0; i < 8; i++) {
+                const int c = tid + 256 * i, rl = c >> 4, cl = (c & 15) * 8;
+                const int m = m0 + rl, n = n0 + cl;
+                if (m < g.M && n < g.N) {
+                    const float4 lo = *(const float4*)(st + rl * kOLd + cl);
+                    const float4 hi = *(const float4*)(st + rl * kOLd + cl + 4);
+                    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    uint32_t w[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
+                        const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
+                        w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
+                    }
+                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            }
+        }
+        return;
+    }
+    /* lane holds C[4*grp + r][row] of each 16 x 16 block */
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int n = n0 + wn + 16 * b + row;
+            if (n >= g.N) continue;
+            const float bias = g.bias ? g.bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * a + 4 * grp + r;
+                if (m >= g.M) continue;
+                const long long o = (long long)m * g.ldc + n;
+                float z = acc[a][b][r];
+                if (g.addin) z = z + g.addin[o];
+                z = z + bias;
+                if (g.zmul) z = z * gelu_grad(g.zmul[o]);
+                if (g.out32) g.out32[o] = z;
+                if (g.out16) g.out16[o] = f2bf(g.gelu ? gelu(z) : z);
+            }
+        }
+    }
+}
+
+/* dst (C x R) = src (R x C) transposed, bf16 bits; R and C multiples of 64 */
+__global__ __launch_bounds__(256) void k_transpose16(const uint16_t* __restrict__ src, int R, int C,
+                                                     uint16_t* __restrict__ dst)
+{
+    __shared__ uint16_t t[64][66];
+    const int tc = C / 64;
+    const int r0 = (blockIdx.x / tc) * 64, c0 = (blockIdx.x % tc) * 64;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) t[e >> 6][e & 63] = src[(long long)(r0 + (e >> 6)) * C + c0 + (e & 63)];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += 256)
+        dst[(long long)(c0 + (e >> 6)) * R + r0 + (e & 63)] = t[e & 63][e >> 6];
+}
+
 /* ------------------------------------------------------------ elementwise */
 
 /* descriptor bits -> +/-1 bf16 (bit k of the descriptor -> column k) */
@@ -205,56 +391,59 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
 }
 
 /* one wavefront per query: max softmax probability and the top-k logits
-   (ties to the lower bucket id) */
+   (ties to the lower bucket id; C <= 1024) */
 __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k,
                                                int* __restrict__ probes, float* __restrict__ pmax)
 {
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    __shared__ uint2 s_cand[4][1024];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wv;
     if (r >= n) return;
     const float* z = Z + (long long)r * C;
-    constexpr int kPer = 16; /* C <= 1024 */
-    float v[kPer];
+    float v[16];
 #pragma unroll
-    for (int t = 0; t < kPer; t++) {
+    for (int t = 0; t < 16; t++) {
         const int c = lane + 64 * t;
         v[t] = c < C ? z[c] : -INFINITY;
     }
     float m = -INFINITY;
 #pragma unroll
-    for (int t = 0; t < kPer; t++) m = fmaxf(m, v[t]);
+    for (int t = 0; t < 16; t++) m = fmaxf(m, v[t]);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     float s = 0.f;
 #pragma unroll
-    for (int t = 0; t < kPer; t++) s += (lane + 64 * t) < C ? __expf(v[t] - m) : 0.f;
+    for (int t = 0; t < 16; t++) s += (lane + 64 * t) < C ? __expf(v[t] - m) : 0.f;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) pmax[r] = 1.f / s;
-    for (int j = 0; j < k; j++) {
-        /* argmax with the lowest index on ties */
-        float bv = -INFINITY;
-        int bi = 0x7fffffff;
+    uint32_t key[16];
 #pragma unroll
-        for (int t = 0; t < kPer; t++) {
-            const int c = lane + 64 * t;
-            if (c < C && (v[t] > bv || (v[t] == bv && c < bi))) { bv = v[t]; bi = c; }
-        }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const float ov = __shfl_xor(bv, o, 64);
-            const int oi = __shfl_xor(bi, o, 64);
-            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-        }
-        if (lane == 0) probes[(long long)r * k + j] = bi;
-        if ((bi & 63) == lane) v[bi >> 6] = -INFINITY;
-    }
+    for (int t = 0; t < 16; t++) key[t] = (lane + 64 * t) < C ? float_order_key(v[t]) : 0u;
+    wave_topk(key, k, s_cand[wv], probes + (long long)r * k);
 }
 
 /* ---------------------------------------------------------------- launchers */
 
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gemm, dim3((g.M / kGT) * (g.N / kGT)), dim3(256), 0, s, g);
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.M % kGT || g.N % kGT || g.K % kNK) return hipErrorInvalidValue;
+    const bool nt = g.sak == 1 && g.sbk == 1 && g.sam % 8 == 0 && g.sbn % 8 == 0;
+    if (g.K2 && (!nt || g.K2 < 0 || g.K2 % kNK || !g.A2 || !g.B2 || g.sa2m % 8 || g.sb2n % 8))
+        return hipErrorInvalidValue;
+    if (nt) {
+        const int tiles = ((g.M + kNT - 1) / kNT) * ((g.N + kNT - 1) / kNT);
+        hipLaunchKernelGGL(k_gemm_nt, dim3(tiles), dim3(256), 0, s, g);
+    } else {
+        hipLaunchKernelGGL(k_gemm, dim3((g.M / kGT) * (g.N / kGT)), dim3(256), 0, s, g);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose16(const uint16_t* src, int R, int C, uint16_t* dst, hipStream_t s)
+{
+    if (R <= 0 || C <= 0 || R % 64 || C % 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_transpose16, dim3((R / 64) * (C / 64)), dim3(256), 0, s, src, R, C, dst);
     return hipGetLastError();
 }
 

@@ -1,0 +1,126 @@
+/*
+ * Wavefront-level helpers shared by the HIP sources (device code only):
+ * LDS visibility inside one wavefront, lane prefix counts, and an exact
+ * top-k selection over 1024 keys held 16 per lane.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orbk
+{
+
+/* LDS written by lanes of this wavefront is visible to its other lanes */
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* number of lanes below this one whose bit is set in m */
+__device__ __forceinline__ int lanes_below(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* order-preserving key of a finite float (-0 and +0 share one key, above 0) */
+__device__ __forceinline__ uint32_t float_order_key(float f)
+{
+    const uint32_t u = f == 0.f ? 0u : __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+/* The k-th largest key among slots s < S of the wavefront (64 S keys), found
+   bit by bit from the top: T grows while at least k keys are >= T. */
+template <int NS>
+__device__ __forceinline__ uint32_t wave_kth_key(const uint32_t (&key)[NS], int S, int k)
+{
+    uint32_t T = 0;
+    for (int b = 31; b >= 0; b--) {
+        const uint32_t c = T | (1u << b);
+        int cnt = 0;
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            if (s < S) cnt += __popcll(__ballot(key[s] >= c));
+        if (cnt >= k) T = c;
+    }
+    return T;
+}
+
+/* Top-k of up to 1024 entries, entry lane + 64 t holding key[t] (key 0 marks
+   an absent entry; at least k entries present, 1 <= k <= 1024): writes the
+   entry numbers to out[0 .. k) by descending key, equal keys by ascending
+   entry number.  cand is this wavefront's LDS buffer of 1024 entries.
+
+   1. For k <= 64 the k-th largest lane maximum L bounds the answer from below
+      (k lanes hold a key >= L); entries >= L are compacted into cand in entry
+      order.  2. The exact k-th largest key T of those.  3. Entries above T
+      (fewer than k) are ranked against each other; entries equal to T take
+      the remaining ranks in entry order. */
+__device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint2* cand, int* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t L = 1;
+    if (k <= 64) {
+        uint32_t lm[1] = {0};
+#pragma unroll
+        for (int t = 0; t < 16; t++) lm[0] = max(lm[0], key[t]);
+        L = wave_kth_key<1>(lm, 1, k);
+    }
+    int nc = 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const bool p = key[t] >= L;
+        const uint64_t m = __ballot(p);
+        if (p) cand[nc + lanes_below(m)] = make_uint2(key[t], (uint32_t)(lane + 64 * t));
+        nc += __popcll(m);
+    }
+    wave_sync_lds();
+    const int S = (nc + 63) >> 6;
+    uint32_t ck[16];
+    int ci[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        const int j = lane + 64 * s;
+        const uint2 e = (s < S && j < nc) ? cand[j] : make_uint2(0u, 0u);
+        ck[s] = e.x;
+        ci[s] = (int)e.y;
+    }
+    const uint32_t T = wave_kth_key<16>(ck, S, k);
+    wave_sync_lds();
+    int g = 0;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        if (s < S) {
+            const bool p = ck[s] > T;
+            const uint64_t m = __ballot(p);
+            if (p) cand[g + lanes_below(m)] = make_uint2(ck[s], (uint32_t)ci[s]);
+            g += __popcll(m);
+        }
+    }
+    wave_sync_lds();
+    for (int i = lane; i < g; i += 64) {
+        const uint2 e = cand[i];
+        int rank = 0;
+        for (int j = 0; j < g; j++) {
+            const uint2 o = cand[j];
+            rank += (o.x > e.x) | ((o.x == e.x) & (j < i));
+        }
+        out[rank] = (int)e.y;
+    }
+    int base = g;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        if (s < S && base < k) {
+            const bool p = ck[s] == T;
+            const uint64_t m = __ballot(p);
+            const int rank = base + lanes_below(m);
+            if (p && rank < k) out[rank] = ci[s];
+            base += __popcll(m);
+        }
+    }
+}
+
+} // namespace orbk

@@ -42,7 +42,7 @@ EXPORTS = [
     "orb_graph_launch", "orb_graph_destroy",
     "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
     "orb_lindex_logits", "orb_lindex_grads", "orb_lindex_num_params", "orb_lindex_get_weights",
-    "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_save", "orb_lindex_load", "orb_lindex_set_probe",
+    "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_probes", "orb_lindex_nprobe", "orb_lindex_save", "orb_lindex_load", "orb_lindex_set_probe",
 ]
 
 ORB_LINDEX_MLP = 0
@@ -123,6 +123,8 @@ def lib():
         L.orb_lindex_get_weights.argtypes = [P, V]
         L.orb_lindex_set_weights.argtypes = [P, V]
         L.orb_lindex_query.argtypes = [P, V, I, I, V, V, V, V, V]
+        L.orb_lindex_probes.argtypes = [P, V, I, I, V, V]
+        L.orb_lindex_nprobe.argtypes = [P]
         L.orb_lindex_save.argtypes = [P, ctypes.c_char_p]
         L.orb_lindex_load.argtypes = [P, ctypes.c_char_p]
         L.orb_lindex_set_probe.argtypes = [P, I, ctypes.c_float]
@@ -473,6 +475,17 @@ class LearnedIndex:
         self._c(lib().orb_lindex_query(self.h, _dptr(q), q.shape[0], mode, *[_dptr(o) for o in out],
                                        ctypes.byref(nfb)), "orb_lindex_query")
         return out, nfb.value
+
+    def probes(self, q, mode=ORB_LINDEX_MLP):
+        """(probes, pmax): the nprobe buckets each query probes, best first (ties to the lower
+        bucket id), and its max softmax probability (1 in centroid mode)."""
+        self.ctx._bind_stream()
+        torch = self._torch
+        pr = torch.empty((q.shape[0], lib().orb_lindex_nprobe(self.h)), dtype=torch.int32, device=q.device)
+        pm = torch.empty(q.shape[0], dtype=torch.float32, device=q.device)
+        self._c(lib().orb_lindex_probes(self.h, _dptr(q), q.shape[0], mode, _dptr(pr), _dptr(pm)),
+                "orb_lindex_probes")
+        return pr, pm
 
     def set_probe(self, nprobe, fallback):
         self._c(lib().orb_lindex_set_probe(self.h, nprobe, fallback), "orb_lindex_set_probe")

@@ -226,6 +226,15 @@ int orb_lindex_grads(orb_lindex* lx, const uint8_t* d_q, const int* d_labels, in
 int orb_lindex_num_params(const orb_lindex* lx);
 int orb_lindex_get_weights(orb_lindex* lx, float* h_w);
 int orb_lindex_set_weights(orb_lindex* lx, const float* h_w);
+/* The buckets each of nq queries (multiple of 64) probes, as orb_lindex_query
+   would (d_probes: nq x nprobe ints, best first -- descending logit, or
+   ascending centroid distance in ORB_LINDEX_CENTROIDS mode -- equal scores by
+   ascending bucket id), and its max softmax probability (d_pmax: nq floats,
+   1 in centroid mode).  Asynchronous on the context stream.  No counterpart in
+   the reference (the learned matcher is an addition). */
+int orb_lindex_probes(orb_lindex* lx, const uint8_t* d_q, int nq, int mode, int* d_probes, float* d_pmax);
+/* Current number of probed buckets (set at creation, by orb_lindex_set_probe or by a load). */
+int orb_lindex_nprobe(const orb_lindex* lx);
 /* Change the number of probed buckets (1..nbuckets) and the fallback threshold. */
 int orb_lindex_set_probe(orb_lindex* lx, int nprobe, float fallback);
 /* Nearest and second-nearest database entries of nq queries (multiple of 64);

@@ -35,14 +35,15 @@ def flip_copies(rng, db, n, p=0.05):
     return db[src] ^ flips, src
 
 
-@pytest.mark.parametrize("hidden,nb,cscale", [(512, 1024, 0.0), (128, 256, 0.0), (128, 256, 0.25)])
-def test_gradients_match_oracle(torch, ctx, hidden, nb, cscale):
+# (rows, hidden, buckets, centroid_scale); 192 x 192 x 320 leaves partial 128-row tiles in every product
+@pytest.mark.parametrize("n,hidden,nb,cscale", [(256, 512, 1024, 0.0), (256, 128, 256, 0.0), (256, 128, 256, 0.25),
+                                                (192, 192, 320, 0.0)])
+def test_gradients_match_oracle(torch, ctx, n, hidden, nb, cscale):
     import orb_amd
     from oracle import mlp
-    print(f"grad check hidden={hidden} buckets={nb} centroid_scale={cscale}", flush=True)
+    print(f"grad check rows={n} hidden={hidden} buckets={nb} centroid_scale={cscale}", flush=True)
     lx = orb_amd.LearnedIndex(ctx, nbuckets=nb, hidden=hidden, seed=7, centroid_scale=cscale)
     rng = np.random.default_rng(1)
-    n = 256
     q = rng.integers(0, 256, (n, 32), dtype=np.uint8)
     labels = rng.integers(0, nb, n).astype(np.int32)
     cg = None
@@ -92,6 +93,42 @@ def test_kmodes_partition_is_consistent(torch, ctx, small_index, oracle):
     # every entry sits in the bucket of its nearest centroid (ties to the lower id)
     i1, d1, _, _ = oracle.match_top2(db, cen, nthreads=8)
     assert np.array_equal(bof, i1)
+
+
+@pytest.mark.parametrize("nprobe", [1, 4, 33, 64, 65, 200, 256])
+def test_probe_selection_matches_oracle(torch, ctx, small_index, nprobe):
+    """The probed buckets themselves, best first with ties to the lower bucket id: network
+    logits (continuous), an untrained network with the centroid term (logits are multiples
+    of the centroid scale: many exact ties), and centroid distances (integer: ties)."""
+    import orb_amd
+    from oracle import mlp
+    lx, db, ddb = small_index
+    print(f"probe selection: nprobe {nprobe}", flush=True)
+    rng = np.random.default_rng(11 + nprobe)
+    q, _ = flip_copies(rng, db, 640, p=0.2)
+    dq = torch.from_numpy(q).cuda()
+    lx.set_probe(nprobe, 0.3)
+    try:
+        logits = lx.logits(dq).cpu().numpy().astype(np.float64)
+        pr, pm = lx.probes(dq)
+        want, wpm = mlp.probe(logits, nprobe)
+        assert np.array_equal(pr.cpu().numpy(), want)
+        np.testing.assert_allclose(pm.cpu().numpy(), wpm, rtol=1e-5)
+        pr, pm = lx.probes(dq, mode=orb_amd.ORB_LINDEX_CENTROIDS)
+        _, cen, _ = lx.bucket_info()
+        D = (np.unpackbits(q, axis=1)[:, None, :] != np.unpackbits(cen, axis=1)[None, :, :]).sum(-1)
+        assert np.array_equal(pr.cpu().numpy(), np.argsort(D, axis=1, kind="stable")[:, :nprobe])
+        assert (pm.cpu().numpy() == 1).all()
+    finally:
+        lx.set_probe(4, 0.3)
+    tied = orb_amd.LearnedIndex(ctx, nbuckets=256, hidden=64, nprobe=nprobe, fallback=0.3, seed=8,
+                                centroid_scale=0.25)
+    tied.build(ddb, iters=1)
+    logits = tied.logits(dq).cpu().numpy().astype(np.float64)
+    assert len(np.unique(logits[0])) < 100  # heavily tied rows
+    pr, _ = tied.probes(dq)
+    assert np.array_equal(pr.cpu().numpy(), mlp.probe(logits, nprobe)[0])
+    tied.close()
 
 
 def test_centroid_probe_search_equals_brute_force(torch, ctx, small_index):
