@@ -6,6 +6,8 @@
  * block, with keys carrying the original database index so ties resolve to
  * the lowest index whatever the bucketed order.
  */
+#include <type_traits>
+
 #include "orb_internal.h"
 #include "orb_match_common.h"
 #include "orb_mlp.h"
@@ -22,12 +24,6 @@ __global__ void k_zero_int(int* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
-}
-
-__global__ void k_bucket_hist(const int* __restrict__ assign, int n, int* __restrict__ hist)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&hist[assign[i]], 1);
 }
 
 /* single workgroup: exclusive scan of nb counts (nb <= 4096) into offsets[0..nb]
@@ -85,17 +81,77 @@ __global__ __launch_bounds__(1024) void k_scan_offsets(const int* __restrict__ c
     }
 }
 
-__global__ void k_bucket_scatter(const int* __restrict__ assign, int n, const uint8_t* __restrict__ db,
-                                 int* __restrict__ cursor, int* __restrict__ perm, uint8_t* __restrict__ db_sorted)
+/* Stable counting sort of the database by bucket (entries of a bucket keep
+   their index order, so a position in the sorted copy orders like the index):
+   per chunk of 256 entries the bucket histogram, stored bucket-major
+   H[b * nchunks + c]; per bucket its total, then its chunk counts rebased to
+   exclusive positions; each entry goes to its chunk's position for its bucket
+   plus the number of earlier entries of the chunk in the same bucket. */
+constexpr int kSortChunk = 256;
+
+__global__ __launch_bounds__(256) void k_chunk_hist(const int* __restrict__ assign, int n, int nb, int nchunks,
+                                                    int* __restrict__ H)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int h[1024];
+    for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * kSortChunk + threadIdx.x;
+    if (i < n) atomicAdd(&h[assign[i]], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += 256) H[(long long)b * nchunks + blockIdx.x] = h[b];
+}
+
+/* one workgroup per bucket: rebase = 0 writes the row total to total[b];
+   rebase = 1 turns the row into exclusive positions starting at offsets[b] */
+__global__ __launch_bounds__(256) void k_chunk_rows(int* __restrict__ H, int nchunks, int rebase,
+                                                    int* __restrict__ total, const int* __restrict__ offsets)
+{
+    __shared__ int s_w[4];
+    int* row = H + (long long)blockIdx.x * nchunks;
+    const int per = (nchunks + 255) / 256;
+    const int c0 = min(nchunks, (int)threadIdx.x * per), c1 = min(nchunks, c0 + per);
+    int sum = 0;
+    for (int c = c0; c < c1; c++) sum += row[c];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[w] = x;
+    __syncthreads();
+    if (!rebase) {
+        if (threadIdx.x == 0) total[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        return;
+    }
+    int run = offsets[blockIdx.x] + x - sum;
+    for (int i = 0; i < w; i++) run += s_w[i];
+    for (int c = c0; c < c1; c++) {
+        const int t = row[c];
+        row[c] = run;
+        run += t;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chunk_scatter(const int* __restrict__ assign, int n, int nchunks,
+                                                       const int* __restrict__ H, const uint8_t* __restrict__ db,
+                                                       int* __restrict__ perm, uint8_t* __restrict__ db_sorted)
+{
+    __shared__ int sb[kSortChunk];
+    const int i = blockIdx.x * kSortChunk + threadIdx.x;
+    const int b = i < n ? assign[i] : -1;
+    sb[threadIdx.x] = b;
+    __syncthreads();
     if (i >= n) return;
-    const int pos = atomicAdd(&cursor[assign[i]], 1);
+    int r = 0;
+    for (int t = 0; t < (int)threadIdx.x; t++) r += sb[t] == b;
+    const int pos = H[(long long)b * nchunks + blockIdx.x] + r;
     perm[pos] = i;
-    const uint4* s = (const uint4*)(db + (long long)i * 32);
-    uint4* d = (uint4*)(db_sorted + (long long)pos * 32);
-    d[0] = s[0];
-    d[1] = s[1];
+    const uint4* src = (const uint4*)(db + (long long)i * 32);
+    uint4* dst = (uint4*)(db_sorted + (long long)pos * 32);
+    dst[0] = src[0];
+    dst[1] = src[1];
 }
 
 /* one workgroup per bucket, one thread per bit: the majority bit of the
@@ -148,6 +204,14 @@ __global__ __launch_bounds__(256) void k_probe_fill(const int* __restrict__ prob
     for (int i = i0 + threadIdx.x; i < i1; i += 256) lists[atomicAdd(&h[probes[i]], 1)] = i;
 }
 
+/* unit -> its position in the bucket order, for every unit of position i */
+__global__ void k_unit_pos(const int* __restrict__ unit_offsets, int nb, int* __restrict__ unit_pos)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    for (int u = unit_offsets[i]; u < unit_offsets[i + 1]; u++) unit_pos[u] = i;
+}
+
 /* Each unit: up to 256 list entries (query, slot) of one bucket x all of the
    bucket's entries.  A wave holds 64 queries as +/-1 B operands; each 64-entry
    tile of the bucket is unpacked into LDS once per workgroup.  Units are
@@ -158,6 +222,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                                                           const int* __restrict__ lists,
                                                           const int* __restrict__ list_offsets,
                                                           const int* __restrict__ unit_offsets,
+                                                          const int* __restrict__ unit_pos,
                                                           const int* __restrict__ order, int nb,
                                                           const uint8_t* __restrict__ db_sorted,
                                                           const int* __restrict__ perm,
@@ -165,7 +230,6 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                                                           uint2* __restrict__ partial)
 {
     __shared__ __align__(16) int8_t s_db[kLxTile * kLxRow];
-    __shared__ int s_idx[kLxTile];
     __shared__ int s_u;
     const int total_units = unit_offsets[nb];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -176,24 +240,17 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
         __syncthreads();
         const int u = s_u;
         if (u >= total_units) break;
-        /* position of unit u in the bucket order: the last i with unit_offsets[i] <= u */
-        int lo = 0, hi = nb - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (unit_offsets[mid] <= u) lo = mid;
-            else hi = mid - 1;
-        }
+        const int lo = unit_pos[u]; /* position of unit u in the bucket order */
         const int b = order[lo];
         const int e0 = list_offsets[b] + (u - unit_offsets[lo]) * kLxQ, e1 = min(list_offsets[b + 1], e0 + kLxQ);
         v4i bq[4][4];
-        int ent[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
             const int e = e0 + wid * 64 + qb * 16 + col;
-            ent[qb] = e < e1 ? lists[e] : -1;
+            const int ent = e < e1 ? lists[e] : -1;
             uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
-            if (ent[qb] >= 0) {
-                const uint4* qp = (const uint4*)(q + (long long)(ent[qb] / nprobe) * 32);
+            if (ent >= 0) {
+                const uint4* qp = (const uint4*)(q + (long long)(ent / nprobe) * 32);
                 d0 = qp[0];
                 d1 = qp[1];
             }
@@ -208,32 +265,22 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
             thr[qb] = 256 - (int)(kNoKey >> 22);
         }
         const int j0 = offsets[b], j1 = offsets[b + 1];
-        for (int jt = j0; jt < j1; jt += kLxTile) {
-            __syncthreads();
-            {
-                const int e = tid >> 2, qq = tid & 3, j = jt + e;
-                uint32_t lo2 = 0, hi2 = 0;
-                if (j < j1) {
-                    const uint2 v = *(const uint2*)(db_sorted + (long long)j * 32 + qq * 8);
-                    lo2 = v.x;
-                    hi2 = v.y;
-                    if (qq == 0) s_idx[e] = perm[j];
-                }
-                int8_t* dst = s_db + e * kLxRow + qq * 64;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t hw = k < 2 ? (lo2 >> (16 * k)) & 0xffffu : (hi2 >> (16 * (k - 2))) & 0xffffu;
-                    *(v4i*)(dst + 16 * k) = pm1_16(hw);
-                }
-            }
-            __syncthreads();
+        /* thread -> entry e of a tile, bytes 8 qq .. 8 qq + 7; the next tile's
+           bytes are loaded while this tile is searched */
+        const int e = tid >> 2, qq = tid & 3;
+        uint2 nv = make_uint2(0u, 0u);
+        if (j0 + e < j1) nv = *(const uint2*)(db_sorted + (long long)(j0 + e) * 32 + qq * 8);
+        /* keys carry the sorted position (index order inside the bucket); PART:
+           the bucket's last tile, whose rows past j1 are skipped */
+        auto search_tile = [&](int jt, auto part) {
+            constexpr bool PART = decltype(part)::value;
 #pragma unroll
             for (int ab = 0; ab < 4; ab++) {
                 v4i a[4];
                 const int8_t* src = s_db + (ab * 16 + col) * kLxRow + 16 * g;
 #pragma unroll
                 for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(src + 64 * ks);
-                const int rb = ab * 16 + 4 * g;
+                const int jb = jt + ab * 16 + 4 * g;
 #pragma unroll
                 for (int qb = 0; qb < 4; qb++) {
                     v4i acc = {0, 0, 0, 0};
@@ -245,12 +292,29 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                     if (m >= thr[qb]) {
 #pragma unroll
                         for (int r = 0; r < 4; r++)
-                            if (jt + rb + r < j1)
-                                top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)s_idx[rb + r]));
+                            if (!PART || jb + r < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)(jb + r)));
                         thr[qb] = 256 - (int)(k2[qb] >> 22);
                     }
                 }
             }
+        };
+        for (int jt = j0; jt < j1; jt += kLxTile) {
+            __syncthreads();
+            {
+                const uint32_t lo2 = nv.x, hi2 = nv.y;
+                int8_t* dst = s_db + e * kLxRow + qq * 64;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t hw = k < 2 ? (lo2 >> (16 * k)) & 0xffffu : (hi2 >> (16 * (k - 2))) & 0xffffu;
+                    *(v4i*)(dst + 16 * k) = pm1_16(hw);
+                }
+                const int jn = jt + kLxTile + e;
+                nv = make_uint2(0u, 0u);
+                if (jn < j1) nv = *(const uint2*)(db_sorted + (long long)jn * 32 + qq * 8);
+            }
+            __syncthreads();
+            if (jt + kLxTile <= j1) search_tile(jt, std::false_type{});
+            else search_tile(jt, std::true_type{});
         }
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
@@ -260,7 +324,13 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                 top2_push(k1[qb], k2[qb], o1);
                 top2_push(k1[qb], k2[qb], o2);
             }
-            if (g == 0 && ent[qb] >= 0) partial[ent[qb]] = make_uint2(k1[qb], k2[qb]);
+            const int el = e0 + wid * 64 + qb * 16 + col;
+            if (g == 0 && el < e1) {
+                /* sorted positions back to database indices */
+                const uint32_t a1 = k1[qb] == kNoKey ? kNoKey : (k1[qb] & ~kIdxMask) | (uint32_t)perm[k1[qb] & kIdxMask];
+                const uint32_t a2 = k2[qb] == kNoKey ? kNoKey : (k2[qb] & ~kIdxMask) | (uint32_t)perm[k2[qb] & kIdxMask];
+                partial[lists[el]] = make_uint2(a1, a2);
+            }
         }
     }
 }
@@ -354,7 +424,7 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
             key[t] = 0u;
         }
     }
-    wave_topk(key, k, s_cand[wv], probes + (long long)r * k);
+    wave_topk(key, k, 16, s_cand[wv], probes + (long long)r * k);
     if (lane == 0) pmax[r] = 1.f;
 }
 
@@ -378,23 +448,19 @@ hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, i
 
 static inline dim3 grid1(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
-hipError_t launch_bucket_hist(const int* assign, int n, int* hist, hipStream_t s)
+hipError_t launch_bucket_sort(const int* assign, int n, int nb, const uint8_t* db, int* H, int* hist, int* offsets,
+                              int* perm, uint8_t* db_sorted, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_bucket_hist, grid1(n), dim3(256), 0, s, assign, n, hist);
-    return hipGetLastError();
-}
-
-hipError_t launch_bucket_offsets(const int* hist, int nb, int* offsets, int* cursor, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, hist, nb, offsets, cursor, 0, (int*)nullptr,
-                       (const int*)nullptr);
-    return hipGetLastError();
-}
-
-hipError_t launch_bucket_scatter(const int* assign, int n, const uint8_t* db, int* cursor, int* perm,
-                                 uint8_t* db_sorted, hipStream_t s)
-{
-    if (n > 0) hipLaunchKernelGGL(k_bucket_scatter, grid1(n), dim3(256), 0, s, assign, n, db, cursor, perm, db_sorted);
+    if (nb > 1024) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
+    const int nchunks = (n + kSortChunk - 1) / kSortChunk;
+    hipLaunchKernelGGL(k_chunk_hist, dim3(nchunks), dim3(256), 0, s, assign, n, nb, nchunks, H);
+    hipLaunchKernelGGL(k_chunk_rows, dim3(nb), dim3(256), 0, s, H, nchunks, 0, hist, (const int*)nullptr);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, (const int*)hist, nb, offsets, (int*)nullptr, 0,
+                       (int*)nullptr, (const int*)nullptr);
+    hipLaunchKernelGGL(k_chunk_rows, dim3(nb), dim3(256), 0, s, H, nchunks, 1, (int*)nullptr, (const int*)offsets);
+    hipLaunchKernelGGL(k_chunk_scatter, dim3(nchunks), dim3(256), 0, s, assign, n, nchunks, (const int*)H, db, perm,
+                       db_sorted);
     return hipGetLastError();
 }
 
@@ -406,7 +472,8 @@ hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, in
 }
 
 hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, const int* order, int* list_count,
-                              int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, hipStream_t s)
+                              int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, int* unit_pos,
+                              hipStream_t s)
 {
     const long long n = (long long)nq * nprobe;
     hipLaunchKernelGGL(k_zero_int, grid1(nb + 1), dim3(256), 0, s, list_count, nb + 1);
@@ -415,17 +482,22 @@ hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, con
     if (n > 0) hipLaunchKernelGGL(k_probe_count, chunks, dim3(256), 0, s, probes, (int)n, nb, list_count);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, list_count, nb, list_offsets, list_cursor, kLxQ,
                        unit_offsets, order);
+    hipLaunchKernelGGL(k_unit_pos, grid1(nb), dim3(256), 0, s, unit_offsets, nb, unit_pos);
     if (n > 0) hipLaunchKernelGGL(k_probe_fill, chunks, dim3(256), 0, s, probes, (int)n, nb, list_cursor, lists);
     return hipGetLastError();
 }
 
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
-                                const int* unit_offsets, const int* order, int nb, const uint8_t* db_sorted,
-                                const int* perm, const int* offsets, int* next, uint2* partial, int blocks,
-                                hipStream_t s)
+                                const int* unit_offsets, const int* unit_pos, const int* order, int nb,
+                                const uint8_t* db_sorted, const int* perm, const int* offsets, int* next,
+                                uint2* partial, int ncu, hipStream_t s)
 {
+    /* as many workgroups as stay resident: they take units until none are left */
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_bucket_search, 256, 0) != hipSuccess || per < 1) per = 1;
+    const int blocks = per * (ncu > 0 ? ncu : 1);
     hipLaunchKernelGGL(k_bucket_search, dim3(blocks), dim3(256), 0, s, q, nprobe, lists, list_offsets, unit_offsets,
-                       order, nb, db_sorted, perm, offsets, next, partial);
+                       unit_pos, order, nb, db_sorted, perm, offsets, next, partial);
     return hipGetLastError();
 }
 

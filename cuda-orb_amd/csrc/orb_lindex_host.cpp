@@ -92,10 +92,10 @@ struct orb_lindex {
     DBuf<uint8_t> db, db_sorted, centroids;
     DBuf<uint16_t> cpm;      /* centroids as +/-centroid_scale bf16, nbuckets x 256 */
     uint16_t cscale16 = 0;   /* bf16 bits of centroid_scale (0: no centroid term) */
-    DBuf<int> assign, hist, offsets, cursor, perm;
+    DBuf<int> assign, hist, offsets, chunk_hist, perm;
     DBuf<int> border; /* bucket ids by decreasing size (ties by id): the search's unit order */
     /* query workspace */
-    DBuf<int> probes, list_count, list_offsets, list_cursor, lists, unit_offsets, fb_count, fb_index, fb_qidx;
+    DBuf<int> probes, list_count, list_offsets, list_cursor, lists, unit_offsets, unit_pos, fb_count, fb_index, fb_qidx;
     DBuf<int> ex_idx1, ex_d1, ex_idx2, ex_d2, tmp_idx;
     DBuf<float> pmax;
     DBuf<uint8_t> fb_q;
@@ -391,7 +391,7 @@ int orb_lindex_build(orb_lindex* L, const uint8_t* d_db, int ndb, int iters)
     LALLOC(L, L->perm, ndb);
     LALLOC(L, L->hist, nb);
     LALLOC(L, L->offsets, nb + 1);
-    LALLOC(L, L->cursor, nb);
+    LALLOC(L, L->chunk_hist, (size_t)nb * ((ndb + 255) / 256));
     const bool have_centroids = L->centroids.p != nullptr && L->ndb == -1;
     LALLOC(L, L->centroids, (size_t)nb * 32);
     LTRY(L, hipMemcpyAsync(L->db.p, d_db, (size_t)ndb * 32, hipMemcpyDeviceToDevice, s));
@@ -412,10 +412,8 @@ int orb_lindex_build(orb_lindex* L, const uint8_t* d_db, int ndb, int iters)
         int rc = orb_match_top2(L->ctx, L->db.p, ndb, L->centroids.p, nb, ORB_MATCH_MFMA, L->assign.p, nullptr,
                                 nullptr, nullptr);
         if (rc != ORB_OK) return rc;
-        LTRY(L, hipMemsetAsync(L->hist.p, 0, sizeof(int) * nb, s));
-        LTRY(L, launch_bucket_hist(L->assign.p, ndb, L->hist.p, s));
-        LTRY(L, launch_bucket_offsets(L->hist.p, nb, L->offsets.p, L->cursor.p, s));
-        LTRY(L, launch_bucket_scatter(L->assign.p, ndb, L->db.p, L->cursor.p, L->perm.p, L->db_sorted.p, s));
+        LTRY(L, launch_bucket_sort(L->assign.p, ndb, nb, L->db.p, L->chunk_hist.p, L->hist.p, L->offsets.p,
+                                   L->perm.p, L->db_sorted.p, s));
         if (it < iters) LTRY(L, launch_kmodes_update(L->db_sorted.p, L->offsets.p, nb, L->centroids.p, s));
     }
     int rc = make_cpm(L, s);
@@ -573,6 +571,7 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     LALLOC(L, L->list_offsets, nb + 1);
     LALLOC(L, L->list_cursor, nb);
     LALLOC(L, L->unit_offsets, nb + 1);
+    LALLOC(L, L->unit_pos, (size_t)nq * np / 256 + nb);
     LALLOC(L, L->lists, (size_t)nq * np);
     LALLOC(L, L->partial, (size_t)nq * np);
     LALLOC(L, L->fb_count, 1);
@@ -586,11 +585,10 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     int rc = compute_probes(L, d_q, nq, mode, L->probes.p, L->pmax.p, s);
     if (rc != ORB_OK) return rc;
     LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->border.p, L->list_count.p, L->list_offsets.p,
-                               L->list_cursor.p, L->lists.p, L->unit_offsets.p, s));
-    /* four workgroups per CU stay resident and take units until none are left */
-    LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, L->border.p, nb,
+                               L->list_cursor.p, L->lists.p, L->unit_offsets.p, L->unit_pos.p, s));
+    LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, L->unit_pos.p, L->border.p, nb,
                                  L->db_sorted.p, L->perm.p, L->offsets.p, L->list_count.p + nb, L->partial.p,
-                                 4 * L->ncu, s));
+                                 L->ncu, s));
     const float thresh = mode == ORB_LINDEX_MLP ? L->prm.fallback : -1.f;
     LTRY(L, launch_fallback_compact(d_q, nq, L->pmax.p, thresh, L->fb_count.p, L->fb_index.p, L->fb_qidx.p,
                                     L->fb_q.p, s));

@@ -53,25 +53,28 @@ hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t 
 hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s);
 
 /* bucketed database (orb_lindex.hip) */
-hipError_t launch_bucket_hist(const int* assign, int n, int* hist, hipStream_t s);
-hipError_t launch_bucket_offsets(const int* hist, int nb, int* offsets, int* cursor, hipStream_t s);
-hipError_t launch_bucket_scatter(const int* assign, int n, const uint8_t* db, int* cursor, int* perm,
-                                 uint8_t* db_sorted, hipStream_t s);
+/* stable counting sort of n entries by bucket (assign): offsets[0..nb], perm
+   (sorted position -> index, index order inside a bucket), db_sorted; H holds
+   nb * ceil(n / 256) ints of scratch, hist nb ints */
+hipError_t launch_bucket_sort(const int* assign, int n, int nb, const uint8_t* db, int* H, int* hist, int* offsets,
+                              int* perm, uint8_t* db_sorted, hipStream_t s);
 hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, int nb, uint8_t* centroids,
                                 hipStream_t s);
 
 /* probe lists: queries x nprobe (bucket ids) -> per-bucket lists of q*nprobe+slot;
    list_count has nb + 1 ints (the last is the search's unit counter, zeroed);
-   unit_offsets counts search units along order[] (buckets by decreasing size) */
+   unit_offsets counts search units along order[] (buckets by decreasing size)
+   and unit_pos[u] (nq * nprobe / 256 + nb ints) is the order position of unit u */
 hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, const int* order, int* list_count,
-                              int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, hipStream_t s);
+                              int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, int* unit_pos,
+                              hipStream_t s);
 /* bucketed exact search: for every (query, probed bucket) the top-2 keys
    (distance << 23 | original index) over that bucket's entries; units taken
    from the zeroed counter *next */
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
-                                const int* unit_offsets, const int* order, int nb, const uint8_t* db_sorted,
-                                const int* perm, const int* offsets, int* next, uint2* partial, int blocks,
-                                hipStream_t s);
+                                const int* unit_offsets, const int* unit_pos, const int* order, int nb,
+                                const uint8_t* db_sorted, const int* perm, const int* offsets, int* next,
+                                uint2* partial, int ncu, hipStream_t s);
 /* per query: merge of the probed buckets, or the exact result when the query
    was sent to the fallback (fb_index >= 0) */
 hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,
