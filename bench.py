@@ -345,6 +345,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the dense-matching, batched-frames and learned-matching measurements")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-matching measurement")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only the learned-matching measurement (its JSON object), e.g. under a profiler")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 64 MiB elementwise copy kernel first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -364,6 +366,11 @@ def main():
             torch.bitwise_not(x, out=y)
         torch.cuda.synchronize()
         del x, y
+    if args.learned_only:
+        ctx = orb_amd.Context(0).init(noctaves=LEVELS, max_pts=CAP, scale_factor=SCALE)
+        print(json.dumps({"learned": learned_match(ctx, torch, dev)}), flush=True)
+        ctx.close()
+        return
     a = orb_amd.synth_image(SEED, W, H, NSHAPES)
     b, _ = orb_amd.synth_warp(SEED, a, W, H)
     imgs = [torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)]
