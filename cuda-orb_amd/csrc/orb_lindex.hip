@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                     if (m >= thr[qb]) {
 #pragma unroll
                         for (int r = 0; r < 4; r++)
-                            if (!PART || jb + r < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)(jb + r)));
+                            if (!PART || jb + r < j1) top2_push(k1[qb], k2[qb], pm1_dot_key(acc[r], (uint32_t)(jb + r)));
                         thr[qb] = 256 - (int)(k2[qb] >> 22);
                     }
                 }

@@ -250,6 +250,10 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
         const int qu = u % P.qunits, c = u / P.qunits;
         /* B operands: query (qb*16 + col) of this wave's 64, k-group g */
         v4i bq[4][4];
+        /* thr[qb]: a result can enter the top-2 only if its dot reaches thr =
+           popcount(query) - (k2 >> 23), the current second distance as a dot;
+           "no key" admits every result until two are held */
+        int thr[4];
         int qidx[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
@@ -262,16 +266,12 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
                 d1 = qp[1];
             }
             query_operands(d0, d1, g, bq[qb]);
+            thr[qb] = dot_threshold(popc256(d0, d1), kNoKey);
         }
         uint32_t k1[4], k2[4];
-        /* thr[qb]: a result can enter the top-2 only if its dot reaches thr
-           (256 - (k2 >> 22), the current second distance as a dot; "no key"
-           gives -767, so every result is taken until two are held) */
-        int thr[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
             k1[qb] = k2[qb] = kNoKey;
-            thr[qb] = 256 - (int)(kNoKey >> 22);
         }
         const int j0 = c * P.chunk, j1 = min(ndb, j0 + P.chunk);
         for (int jt = j0; jt < j1; jt += kGranule) {
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t hword = k < 2 ? (lo >> (16 * k)) & 0xffffu : (hi >> (16 * (k - 2))) & 0xffffu;
-                    *(v4i*)(dst + 16 * k) = pm1_16(hword);
+                    *(v4i*)(dst + 16 * k) = b01_16(hword);
                 }
             }
             __syncthreads();
@@ -308,16 +308,19 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++)
                         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
-                    /* dot products are even: a dot below thr means a distance
-                       above the second one, a key above k2, for all 4 entries */
+                    /* a dot below thr means a distance above the second one,
+                       a key above k2, for all 4 entries */
                     const int m = max(max(acc[0], acc[1]), max(acc[2], acc[3]));
                     if (m >= thr[qb]) {
+                        /* thr + (k2 >> 23) is the query's popcount */
+                        const int qpop = thr[qb] + (int)(k2[qb] >> 23);
+                        const uint32_t base = (uint32_t)qpop << 23;
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
                             const int j = jb + r;
-                            if (j < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], (uint32_t)j));
+                            if (j < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], base, (uint32_t)j));
                         }
-                        thr[qb] = 256 - (int)(k2[qb] >> 22);
+                        thr[qb] = dot_threshold(qpop, k2[qb]);
                     }
                 }
             }

@@ -4,8 +4,11 @@
  *
  * A (query, entry) pair is the packed key (distance << 23 | index): the
  * minimum key is the nearest entry with ties to the lowest index, and a
- * running top-2 is (min(k1, key), med3(k1, key, k2)).  For the INT8 MFMA path
- * the bits are expanded to +/-1 bytes and distance = (256 - dot) / 2.
+ * running top-2 is (min(k1, key), med3(k1, key, k2)).  The INT8 MFMA paths
+ * expand the query's bits to +/-1 bytes and the database entry's either to
+ * 0/1 bytes (dot = (both set) - (entry set, query clear), distance =
+ * popcount(query) - dot: fewer unpack instructions) or to +/-1 bytes
+ * (distance = (256 - dot) / 2: no per-query term, fewer registers).
  */
 #pragma once
 
@@ -25,6 +28,17 @@ __device__ __forceinline__ void top2_push(uint32_t& k1, uint32_t& k2, uint32_t k
 {
     k2 = min(k2, max(k1, key)); /* med3(k1, key, k2) since k1 <= k2 */
     k1 = min(k1, key);
+}
+
+/* 16 bits -> 16 bytes of 1 (bit set) / 0 (bit clear) */
+__device__ __forceinline__ v4i b01_16(uint32_t h)
+{
+    v4i r;
+    r.x = (int)(((h & 15u) * 0x00204081u) & 0x01010101u);
+    r.y = (int)((((h >> 4) & 15u) * 0x00204081u) & 0x01010101u);
+    r.z = (int)((((h >> 8) & 15u) * 0x00204081u) & 0x01010101u);
+    r.w = (int)((((h >> 12) & 15u) * 0x00204081u) & 0x01010101u);
+    return r;
 }
 
 /* 4 bits -> 4 bytes of +1 (bit set) / -1 (bit clear) */
@@ -57,10 +71,30 @@ __device__ __forceinline__ void query_operands(uint4 d0, uint4 d1, int g, v4i* b
     }
 }
 
-/* key of a +/-1 dot product: ((256 - dot) / 2) << 23 | idx = (256 - dot) << 22 | idx */
-__device__ __forceinline__ uint32_t dot_key(int dot, uint32_t idx)
+/* key of a (+/-1 query, 0/1 entry) dot product: (popcount(query) - dot) << 23 | idx,
+   from base = popcount(query) << 23 */
+__device__ __forceinline__ uint32_t dot_key(int dot, uint32_t base, uint32_t idx)
+{
+    return (base + idx) - ((uint32_t)dot << 23);
+}
+
+/* key of a (+/-1 query, +/-1 entry) dot product: ((256 - dot) / 2) << 23 | idx */
+__device__ __forceinline__ uint32_t pm1_dot_key(int dot, uint32_t idx)
 {
     return (uint32_t)((256u << 22) + idx) - ((uint32_t)dot << 22);
+}
+
+/* the smallest dot whose key can still be below k2: the distance must not
+   exceed k2's (k2 = kNoKey admits every entry) */
+__device__ __forceinline__ int dot_threshold(int qpop, uint32_t k2)
+{
+    return qpop - (int)(k2 >> 23);
+}
+
+__device__ __forceinline__ int popc256(uint4 d0, uint4 d1)
+{
+    return __popc(d0.x) + __popc(d0.y) + __popc(d0.z) + __popc(d0.w) + __popc(d1.x) + __popc(d1.y) + __popc(d1.z) +
+           __popc(d1.w);
 }
 
 } // namespace orbk
