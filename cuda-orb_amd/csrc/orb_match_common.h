@@ -41,11 +41,12 @@ __device__ __forceinline__ v4i b01_16(uint32_t h)
     return r;
 }
 
-/* 4 bits -> 4 bytes of +1 (bit set) / -1 (bit clear) */
+/* 4 bits -> 4 bytes of +1 (bit set) / -1 (bit clear): the multiply spreads
+   bit i to bit 2 of byte i, so each selector byte is 4 (a byte of the first
+   source, 0x01) or 0 (a byte of the second, 0xFF) */
 __device__ __forceinline__ int pm1_nibble(uint32_t n)
 {
-    const uint32_t t = (n * 0x00204081u) & 0x01010101u;
-    return (int)~(t * 0xFEu);
+    return (int)__builtin_amdgcn_perm(0x01010101u, 0xFFFFFFFFu, (n * 0x00810204u) & 0x04040404u);
 }
 
 /* 16 bits -> 16 +/-1 bytes */
