@@ -396,8 +396,11 @@ __global__ void k_gather(const uint8_t* __restrict__ q, const int* __restrict__ 
     if (lab && lab_src) lab[i] = lab_src[r];
 }
 
-/* one wavefront per query: the nprobe nearest centroids in Hamming distance
-   (ties to the lower bucket id) -- the probe without the network */
+/* the nprobe nearest centroids of each query in Hamming distance (ties to the
+   lower bucket id) -- the probe without the network; the centroids are staged
+   in LDS once per workgroup and each wavefront takes kCenQ queries in turn */
+constexpr int kCenQ = 8;
+
 __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restrict__ q, int nq,
                                                         const uint8_t* __restrict__ cen, int nb, int k,
                                                         int* __restrict__ probes, float* __restrict__ pmax)
@@ -407,25 +410,28 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
     for (int i = threadIdx.x; i < nb * 2; i += 256) s_c[i] = ((const uint4*)cen)[i];
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + wv;
-    if (r >= nq) return;
-    const uint4 qa = ((const uint4*)(q + (long long)r * 32))[0], qb = ((const uint4*)(q + (long long)r * 32))[1];
-    /* key 0x10000 - distance: nearer centroids rank higher; 0 marks no centroid */
-    uint32_t key[16];
+    for (int t = 0; t < kCenQ; t++) {
+        const int r = (blockIdx.x * 4 + wv) * kCenQ + t;
+        if (r >= nq) return;
+        const uint4 qa = ((const uint4*)(q + (long long)r * 32))[0], qb = ((const uint4*)(q + (long long)r * 32))[1];
+        /* key 0x10000 - distance: nearer centroids rank higher; 0 marks no centroid */
+        uint32_t key[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int c = lane + 64 * t;
-        if (c < nb) {
-            const uint4 a = s_c[2 * c], b = s_c[2 * c + 1];
-            const int d = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
-                          __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
-            key[t] = 0x10000u - (uint32_t)d;
-        } else {
-            key[t] = 0u;
+        for (int u = 0; u < 16; u++) {
+            const int c = lane + 64 * u;
+            if (c < nb) {
+                const uint4 a = s_c[2 * c], b = s_c[2 * c + 1];
+                const int d = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
+                              __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
+                key[u] = 0x10000u - (uint32_t)d;
+            } else {
+                key[u] = 0u;
+            }
         }
+        wave_topk<16>(key, k, s_cand[wv], probes + (long long)r * k);
+        if (lane == 0) pmax[r] = 1.f;
+        wave_sync_lds(); /* the next query reuses s_cand */
     }
-    wave_topk(key, k, 16, s_cand[wv], probes + (long long)r * k);
-    if (lane == 0) pmax[r] = 1.f;
 }
 
 /* ---------------------------------------------------------------- launchers */
@@ -441,7 +447,8 @@ hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, i
                                  float* pmax, hipStream_t s)
 {
     if (nq > 0)
-        hipLaunchKernelGGL(k_centroid_probe, dim3((nq + 3) / 4), dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+        hipLaunchKernelGGL(k_centroid_probe, dim3((nq + 4 * kCenQ - 1) / (4 * kCenQ)), dim3(256), 0, s, q, nq, cen, nb,
+                           k, probes, pmax);
     return hipGetLastError();
 }
 

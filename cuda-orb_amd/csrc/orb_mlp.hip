@@ -420,7 +420,7 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
     uint32_t key[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) key[t] = (lane + 64 * t) < C ? float_order_key(v[t]) : 0u;
-    wave_topk(key, k, 31, s_cand[wv], probes + (long long)r * k);
+    wave_topk<31>(key, k, s_cand[wv], probes + (long long)r * k);
 }
 
 /* ---------------------------------------------------------------- launchers */

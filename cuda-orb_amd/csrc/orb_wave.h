@@ -32,14 +32,15 @@ __device__ __forceinline__ uint32_t float_order_key(float f)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-/* The k-th largest of the wavefront's 64 NS keys (every key below 2^(hibit+1)),
+/* The k-th largest of the wavefront's 64 NS keys (every key below 2^(HIBIT+1)),
    found bit by bit from the top: T grows while at least k keys are >= T.
    Keys 0 are never counted (every probe value is >= 1). */
-template <int NS>
-__device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k, int hibit)
+template <int NS, int HIBIT>
+__device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
 {
     uint32_t T = 0;
-    for (int b = hibit; b >= 0; b--) {
+#pragma unroll
+    for (int b = HIBIT; b >= 0; b--) {
         const uint32_t c = T | (1u << b);
         int cnt = 0;
 #pragma unroll
@@ -50,7 +51,7 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k, int
 }
 
 /* Top-k of up to 1024 entries, entry lane + 64 t holding key[t] (key 0 marks
-   an absent entry, every key below 2^(hibit+1); at least k entries present,
+   an absent entry, every key below 2^(HIBIT+1); at least k entries present,
    1 <= k <= 1024): writes the
    entry numbers to out[0 .. k) by descending key, equal keys by ascending
    entry number.  cand is this wavefront's LDS buffer of 1024 entries.
@@ -60,8 +61,8 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k, int
       order.  2. The exact k-th largest key T of those.  3. Entries above T
       (fewer than k) are ranked against each other; entries equal to T take
       the remaining ranks in entry order. */
-__device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, int hibit, uint2* cand,
-                                          int* __restrict__ out)
+template <int HIBIT>
+__device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint2* cand, int* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     uint32_t L = 1;
@@ -69,7 +70,7 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, int 
         uint32_t lm[1] = {0};
 #pragma unroll
         for (int t = 0; t < 16; t++) lm[0] = max(lm[0], key[t]);
-        L = wave_kth_key<1>(lm, k, hibit);
+        L = wave_kth_key<1, HIBIT>(lm, k);
     }
     int nc = 0;
 #pragma unroll
@@ -91,11 +92,11 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, int 
         ci[s] = (int)e.y;
     }
     /* slots at and above S hold key 0: only the occupied ones are counted */
-    const uint32_t T = S <= 1   ? wave_kth_key<1>(ck, k, hibit)
-                       : S <= 2 ? wave_kth_key<2>(ck, k, hibit)
-                       : S <= 4 ? wave_kth_key<4>(ck, k, hibit)
-                       : S <= 8 ? wave_kth_key<8>(ck, k, hibit)
-                                : wave_kth_key<16>(ck, k, hibit);
+    const uint32_t T = S <= 1   ? wave_kth_key<1, HIBIT>(ck, k)
+                       : S <= 2 ? wave_kth_key<2, HIBIT>(ck, k)
+                       : S <= 4 ? wave_kth_key<4, HIBIT>(ck, k)
+                       : S <= 8 ? wave_kth_key<8, HIBIT>(ck, k)
+                                : wave_kth_key<16, HIBIT>(ck, k);
     wave_sync_lds();
     int g = 0;
 #pragma unroll
@@ -111,6 +112,7 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, int 
     for (int i = lane; i < g; i += 64) {
         const uint2 e = cand[i];
         int rank = 0;
+#pragma unroll 4
         for (int j = 0; j < g; j++) {
             const uint2 o = cand[j];
             rank += (o.x > e.x) | ((o.x == e.x) & (j < i));
