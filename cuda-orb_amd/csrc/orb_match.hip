@@ -274,23 +274,24 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
             k1[qb] = k2[qb] = kNoKey;
         }
         const int j0 = c * P.chunk, j1 = min(ndb, j0 + P.chunk);
+        /* unpack 64 entries per tile: thread -> entry tid/4, bits 64*(tid%4) .. +63,
+           the bytes loaded a tile ahead */
+        const int e = tid >> 2, qq = tid & 3;
+        uint2 nv = make_uint2(0u, 0u);
+        if (j0 + e < j1) nv = *(const uint2*)(db + (long long)(j0 + e) * 32 + qq * 8);
         for (int jt = j0; jt < j1; jt += kGranule) {
             __syncthreads();
             {
-                /* unpack 64 entries: thread -> entry tid/4, bits 64*(tid%4) .. +63 */
-                const int e = tid >> 2, qq = tid & 3, j = jt + e;
-                uint32_t lo = 0, hi = 0;
-                if (j < j1) {
-                    const uint2 v = *(const uint2*)(db + (long long)j * 32 + qq * 8);
-                    lo = v.x;
-                    hi = v.y;
-                }
+                const uint32_t lo = nv.x, hi = nv.y;
                 int8_t* dst = s_db + e * kMfmaRow + qq * 64;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t hword = k < 2 ? (lo >> (16 * k)) & 0xffffu : (hi >> (16 * (k - 2))) & 0xffffu;
                     *(v4i*)(dst + 16 * k) = b01_16(hword);
                 }
+                const int jn = jt + kGranule + e;
+                nv = make_uint2(0u, 0u);
+                if (jn < j1) nv = *(const uint2*)(db + (long long)jn * 32 + qq * 8);
             }
             __syncthreads();
 #pragma unroll
