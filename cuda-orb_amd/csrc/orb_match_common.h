@@ -26,8 +26,19 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void top2_push(uint32_t& k1, uint32_t& k2, uint32_t key)
 {
-    k2 = min(k2, max(k1, key)); /* med3(k1, key, k2) since k1 <= k2 */
+    /* k2 = med3(k1, key, k2) since k1 <= k2: one instruction */
+    uint32_t m;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(m) : "v"(k1), "v"(key), "v"(k2));
+    k2 = m;
     k1 = min(k1, key);
+}
+
+/* a * b + c for a, b in the signed 24-bit range: one instruction */
+__device__ __forceinline__ uint32_t mad_i24(int a, int b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+    return r;
 }
 
 /* 16 bits -> 16 bytes of 1 (bit set) / 0 (bit clear) */
@@ -76,13 +87,13 @@ __device__ __forceinline__ void query_operands(uint4 d0, uint4 d1, int g, v4i* b
    from base = popcount(query) << 23 */
 __device__ __forceinline__ uint32_t dot_key(int dot, uint32_t base, uint32_t idx)
 {
-    return (base + idx) - ((uint32_t)dot << 23);
+    return mad_i24(dot, -(1 << 23), base + idx); /* |dot| <= 256 */
 }
 
 /* key of a (+/-1 query, +/-1 entry) dot product: ((256 - dot) / 2) << 23 | idx */
 __device__ __forceinline__ uint32_t pm1_dot_key(int dot, uint32_t idx)
 {
-    return (uint32_t)((256u << 22) + idx) - ((uint32_t)dot << 22);
+    return mad_i24(dot, -(1 << 22), (256u << 22) + idx); /* |dot| <= 256 */
 }
 
 /* the smallest dot whose key can still be below k2: the distance must not
