@@ -161,16 +161,6 @@ __device__ __forceinline__ int wave_sum(int v)
 
 /* ---------------------------------------------------------------- pyramid */
 
-/* Workgroup b of n runs logical unit xcd_remap(b): the hardware deals
-   workgroups round-robin to the 8 XCDs, so this gives every XCD (and its L2)
-   one contiguous range of units, and neighbouring units, which share source
-   rows and halo rows, land on the same L2. */
-__device__ __forceinline__ int xcd_remap(int b, int n)
-{
-    const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
-    return x * q + min(x, r) + k;
-}
-
 /* value of the lane whose byte address (4 * lane) is given */
 __device__ __forceinline__ uint32_t lane_get(uint32_t v, int addr)
 {

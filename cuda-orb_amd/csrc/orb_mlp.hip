@@ -157,13 +157,91 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     }
 }
 
+constexpr int kNT = 128; /* workgroup tile of the NT products (M and N) */
+
+/* Epilogue of the NT products for one 128 x 128 tile: acc as the MFMA left it
+   (lane holds C[4*grp + r][row] of each 16 x 16 block); st is LDS of at least
+   128 x 132 floats, free once the K loop has ended on a barrier. */
+__device__ __forceinline__ void nt_epilogue(const GemmArgs& g, f32x4 (&acc)[4][4], float* st, int m0, int n0)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
+    const int row = lane & 15, grp = lane >> 4;
+    if (!g.addin && !g.zmul) {
+        /* z = acc + bias staged as a 128 x 128 fp32 tile in LDS (the loop ended
+           on a barrier), then written along rows: 16 B per lane */
+        constexpr int kOLd = kNT + 4;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int nl = wn + 16 * b + row;
+            const float bias = g.bias && n0 + nl < g.N ? g.bias[n0 + nl] : 0.f;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) st[(wm + 16 * a + 4 * grp + r) * kOLd + nl] = acc[a][b][r] + bias;
+        }
+        __syncthreads();
+        if (g.out32) {
+#pragma unroll 4
+            for (int i = 0; i < 16; i++) {
+                const int c = tid + 256 * i, rl = c >> 5, cl = (c & 31) * 4;
+                const int m = m0 + rl, n = n0 + cl;
+                if (m < g.M && n < g.N)
+                    *(float4*)(g.out32 + (long long)m * g.ldc + n) = *(const float4*)(st + rl * kOLd + cl);
+            }
+        }
+        if (g.out16) {
+#pragma unroll 2
+            for (int i = 0; i < 8; i++) {
+                const int c = tid + 256 * i, rl = c >> 4, cl = (c & 15) * 8;
+                const int m = m0 + rl, n = n0 + cl;
+                if (m < g.M && n < g.N) {
+                    const float4 lo = *(const float4*)(st + rl * kOLd + cl);
+                    const float4 hi = *(const float4*)(st + rl * kOLd + cl + 4);
+                    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    uint32_t w[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
+                        const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
+                        w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
+                    }
+                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            }
+        }
+        return;
+    }
+    /* lane holds C[4*grp + r][row] of each 16 x 16 block */
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int n = n0 + wn + 16 * b + row;
+            if (n >= g.N) continue;
+            const float bias = g.bias ? g.bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * a + 4 * grp + r;
+                if (m >= g.M) continue;
+                const long long o = (long long)m * g.ldc + n;
+                float z = acc[a][b][r];
+                if (g.addin) z = z + g.addin[o];
+                z = z + bias;
+                if (g.zmul) z = z * gelu_grad(g.zmul[o]);
+                if (g.out32) g.out32[o] = z;
+                if (g.out16) g.out16[o] = f2bf(g.gelu ? gelu(z) : z);
+            }
+        }
+    }
+}
+
 /* C = A B with both operands contiguous along K (A(m, k) = A[m*sam + k],
    B(k, n) = B[n*sbn + k]), plus the optional second K segment A2 B2 in the
    same accumulators: 128 x 128 workgroup tiles, four wavefronts of
    64 x 64 (4 x 4 MFMA blocks), 64-deep K steps double-buffered in LDS with the
    next step's global loads issued before the current step's MFMAs.  Rows
    beyond M or N are read clamped and never stored.  Same epilogue as k_gemm. */
-constexpr int kNT = 128;
 constexpr int kNK = 64;
 constexpr int kNLd = kNK + 8; /* LDS row (bf16 elements): 144 B, conflict-free b128 reads */
 
@@ -173,8 +251,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
     __shared__ __align__(16) uint16_t s_lds[4 * kNT * kNLd];
     uint16_t (*sA)[kNT * kNLd] = (uint16_t (*)[kNT * kNLd])s_lds;
     uint16_t (*sB)[kNT * kNLd] = (uint16_t (*)[kNT * kNLd])(s_lds + 2 * kNT * kNLd);
-    const int tiles_n = (g.N + kNT - 1) / kNT;
-    const int m0 = (blockIdx.x / tiles_n) * kNT, n0 = (blockIdx.x % tiles_n) * kNT;
+    /* tiles in M-major order, each XCD a contiguous range: the N tiles of an
+       M block share its rows of A in one L2 */
+    const int tiles_n = (g.N + kNT - 1) / kNT, u = xcd_remap(blockIdx.x, gridDim.x);
+    const int m0 = (u / tiles_n) * kNT, n0 = (u % tiles_n) * kNT;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
     const int row = lane & 15, grp = lane >> 4;
@@ -255,74 +335,95 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
         *(uint4*)(lB + nxt + 3 * kRow32) = b3;
         __syncthreads();
     }
-    if (!g.addin && !g.zmul) {
-        /* z = acc + bias staged as a 128 x 128 fp32 tile in LDS (the loop ended
-           on a barrier), then written along rows: 16 B per lane */
-        constexpr int kOLd = kNT + 4;
-        float* st = (float*)s_lds;
+    nt_epilogue(g, acc, (float*)s_lds, m0, n0);
+}
+
+/* The same product with 128-deep K steps (K and K2 multiples of 128): one
+   LDS tile of A and one of B (69.6 KB: two workgroups per CU), the next
+   step's 16 B loads (buffer loads: rows beyond M or N read as zeros) in
+   registers while the 64 MFMAs of the current step run, two barriers per
+   step.  Twice the MFMA work per exposed load latency of the 64-deep form. */
+constexpr int kNK2 = 128;
+constexpr int kNLd2 = kNK2 + 8; /* LDS row (bf16 elements): 272 B, conflict-free b128 reads */
+
+__global__ __launch_bounds__(256, 2) void k_gemm_nt128(GemmArgs g)
+{
+    __shared__ __align__(16) uint16_t s_lds[2 * kNT * kNLd2];
+    /* tiles in M-major order, each XCD a contiguous range: the N tiles of an
+       M block share its rows of A in one L2 */
+    const int tiles_n = (g.N + kNT - 1) / kNT, u = xcd_remap(blockIdx.x, gridDim.x);
+    const int m0 = (u / tiles_n) * kNT, n0 = (u % tiles_n) * kNT;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
+    const int row = lane & 15, grp = lane >> 4;
+    /* loader: thread -> tile rows lr + 16 j (j < 8), k offset lk (8 elements) */
+    const int lr = tid >> 4, lk = (tid & 15) * 8;
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void*)g.A, 0, (int)(g.M * g.sam * 2), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc((void*)g.B, 0, (int)(g.N * g.sbn * 2), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rA2 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(g.K2 ? g.A2 : g.A), 0, (int)(g.K2 ? g.M * g.sa2m * 2 : 0), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB2 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(g.K2 ? g.B2 : g.B), 0, (int)(g.K2 ? g.N * g.sb2n * 2 : 0), 0x00020000);
+    const uint32_t vA = (uint32_t)((lr * g.sam + lk) * 2), vB = (uint32_t)((lr * g.sbn + lk) * 2);
+    const uint32_t vA2 = (uint32_t)((lr * g.sa2m + lk) * 2), vB2 = (uint32_t)((lr * g.sb2n + lk) * 2);
+    char* const lA = (char*)s_lds + (lr * kNLd2 + lk) * 2;
+    char* const lB = lA + kNT * kNLd2 * 2;
+    constexpr int kRow16 = 16 * kNLd2 * 2;
+    const char* rdA = (const char*)s_lds + ((wm + row) * kNLd2 + 8 * grp) * 2;
+    const char* rdB = (const char*)s_lds + (kNT * kNLd2 + (wn + row) * kNLd2 + 8 * grp) * 2;
+    const int nk1 = g.K / kNK2, nk = nk1 + g.K2 / kNK2;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 x[16];
+    auto gload = [&](int kt) {
+        const bool seg2 = kt >= nk1;
+        const uint32_t kb = (uint32_t)((seg2 ? kt - nk1 : kt) * kNK2 * 2);
+        const __amdgpu_buffer_rsrc_t ra = seg2 ? rA2 : rA, rb = seg2 ? rB2 : rB;
+        const uint32_t va = seg2 ? vA2 : vA, vb = seg2 ? vB2 : vB;
+        const long long sa = seg2 ? g.sa2m : g.sam, sb = seg2 ? g.sb2n : g.sbn;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int nl = wn + 16 * b + row;
-            const float bias = g.bias && n0 + nl < g.N ? g.bias[n0 + nl] : 0.f;
+        for (int j = 0; j < 8; j++) {
+            x[j] = __builtin_amdgcn_raw_buffer_load_b128(ra, va, (uint32_t)((m0 + 16 * j) * sa * 2) + kb, 0);
+            x[8 + j] = __builtin_amdgcn_raw_buffer_load_b128(rb, vb, (uint32_t)((n0 + 16 * j) * sb * 2) + kb, 0);
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            *(u32x4*)(lA + j * kRow16) = x[j];
+            *(u32x4*)(lB + j * kRow16) = x[8 + j];
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        /* the next step in flight (the last step reloads itself: never stored) */
+        gload(min(kt + 1, nk - 1));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            bf16x8 av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) av[a] = *(const bf16x8*)(rdA + (16 * a * kNLd2 + 32 * ks) * 2);
+#pragma unroll
+            for (int b = 0; b < 4; b++) bv[b] = *(const bf16x8*)(rdB + (16 * b * kNLd2 + 32 * ks) * 2);
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) st[(wm + 16 * a + 4 * grp + r) * kOLd + nl] = acc[a][b][r] + bias;
+                for (int b = 0; b < 4; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
+        __builtin_amdgcn_sched_barrier(0);
         __syncthreads();
-        if (g.out32) {
-#pragma unroll 4
-            for (int i = 0; i < 16; i++) {
-                const int c = tid + 256 * i, rl = c >> 5, cl = (c & 31) * 4;
-                const int m = m0 + rl, n = n0 + cl;
-                if (m < g.M && n < g.N)
-                    *(float4*)(g.out32 + (long long)m * g.ldc + n) = *(const float4*)(st + rl * kOLd + cl);
-            }
-        }
-        if (g.out16) {
-#pragma unroll 2
-            for (int i = 0; i < 8; i++) {
-                const int c = tid + 256 * i, rl = c >> 4, cl = (c & 15) * 8;
-                const int m = m0 + rl, n = n0 + cl;
-                if (m < g.M && n < g.N) {
-                    const float4 lo = *(const float4*)(st + rl * kOLd + cl);
-                    const float4 hi = *(const float4*)(st + rl * kOLd + cl + 4);
-                    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                    uint32_t w[4];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
-                        const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
-                        w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
-                    }
-                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = make_uint4(w[0], w[1], w[2], w[3]);
-                }
-            }
-        }
-        return;
+        if (kt + 1 < nk) lstore();
+        __syncthreads();
     }
-    /* lane holds C[4*grp + r][row] of each 16 x 16 block */
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int n = n0 + wn + 16 * b + row;
-            if (n >= g.N) continue;
-            const float bias = g.bias ? g.bias[n] : 0.f;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + 16 * a + 4 * grp + r;
-                if (m >= g.M) continue;
-                const long long o = (long long)m * g.ldc + n;
-                float z = acc[a][b][r];
-                if (g.addin) z = z + g.addin[o];
-                z = z + bias;
-                if (g.zmul) z = z * gelu_grad(g.zmul[o]);
-                if (g.out32) g.out32[o] = z;
-                if (g.out16) g.out16[o] = f2bf(g.gelu ? gelu(z) : z);
-            }
-        }
-    }
+    nt_epilogue(g, acc, (float*)s_lds, m0, n0);
 }
 
 /* dst (C x R) = src (R x C) transposed, bf16 bits; R and C multiples of 64 */
@@ -453,7 +554,12 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
         return hipErrorInvalidValue;
     if (nt) {
         const int tiles = ((g.M + kNT - 1) / kNT) * ((g.N + kNT - 1) / kNT);
-        hipLaunchKernelGGL(k_gemm_nt, dim3(tiles), dim3(256), 0, s, g);
+        /* 128-deep steps when both K segments allow them and the operands fit 32-bit buffer offsets */
+        const bool deep = g.K % kNK2 == 0 && g.K2 % kNK2 == 0 && (long long)g.M * g.sam * 2 < (1ll << 31) &&
+                          (long long)g.N * g.sbn * 2 < (1ll << 31) && (long long)g.M * g.sa2m * 2 < (1ll << 31) &&
+                          (long long)g.N * g.sb2n * 2 < (1ll << 31);
+        if (deep) hipLaunchKernelGGL(k_gemm_nt128, dim3(tiles), dim3(256), 0, s, g);
+        else hipLaunchKernelGGL(k_gemm_nt, dim3(tiles), dim3(256), 0, s, g);
     } else {
         hipLaunchKernelGGL(k_gemm, dim3((g.M / kGT) * (g.N / kGT)), dim3(256), 0, s, g);
     }

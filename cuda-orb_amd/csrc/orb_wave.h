@@ -1,7 +1,8 @@
 /*
  * Wavefront-level helpers shared by the HIP sources (device code only):
- * LDS visibility inside one wavefront, lane prefix counts, and an exact
- * top-k selection over 1024 keys held 16 per lane.
+ * LDS visibility inside one wavefront, lane prefix counts, the XCD-aware
+ * workgroup order, and an exact top-k selection over 1024 keys held 16 per
+ * lane.
  */
 #pragma once
 
@@ -17,6 +18,16 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Workgroup b of n runs logical unit xcd_remap(b): the hardware deals
+   workgroups round-robin to the 8 XCDs, so this gives every XCD (and its L2)
+   one contiguous range of units, and neighbouring units, which share source
+   rows, land on the same L2. */
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
 }
 
 /* number of lanes below this one whose bit is set in m */
