@@ -1,10 +1,11 @@
 /*
- * Bucketed database of the learned matcher (gfx950): bucket histograms and
- * scatter (a counting sort of the database by bucket), the k-modes centroid
- * update, per-bucket probe lists, and the bucketed exact search -- the INT8
- * MFMA contraction of orb_match.hip applied to each (bucket, probing queries)
- * block, with keys carrying the original database index so ties resolve to
- * the lowest index whatever the bucketed order.
+ * Bucketed database of the learned matcher (gfx950): a stable counting sort
+ * of the database by bucket (entries of a bucket keep their index order),
+ * the k-modes centroid update, per-bucket probe lists, the nearest-centroid
+ * probe, and the bucketed exact search -- the INT8 MFMA contraction of
+ * orb_match.hip applied to each (bucket, probing queries) block, with keys
+ * carrying the sorted position (index order inside the bucket), turned into
+ * database indices once per block, so ties resolve to the lowest index.
  */
 #include <type_traits>
 

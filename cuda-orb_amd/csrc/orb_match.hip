@@ -17,11 +17,13 @@
  *   popcount: a wavefront holds 64 queries (one per lane) and streams its
  *     chunk with wave-uniform (scalar) loads: 8 x (xor, popcount) per pair.
  *   MFMA: the distance is an exact integer contraction,
- *     popcount(a ^ b) = (256 - <a', b'>) / 2 with a', b' the +/-1 expansions
- *     of the bits, computed by v_mfma_i32_16x16x64_i8 (INT8 in, INT32
- *     accumulate).  A workgroup keeps 256 queries as +/-1 B operands in
- *     registers and unpacks each 64-entry database tile into LDS once; a
- *     lane builds keys only for results that can still enter its top-2.
+ *     popcount(a ^ b) = popcount(a) - <a', b01> with a' the +/-1 expansion
+ *     of the query's bits and b01 the 0/1 expansion of the entry's, computed
+ *     by v_mfma_i32_16x16x64_i8 (INT8 in, INT32 accumulate).  A workgroup
+ *     keeps 256 queries as +/-1 B operands in registers and unpacks each
+ *     64-entry database tile into LDS once, the next tile's bytes loaded
+ *     during the current one; a lane builds keys only for results that can
+ *     still enter its top-2.
  */
 #include "orb_internal.h"
 #include "orb_match_common.h"

@@ -10,15 +10,18 @@
  *               transposed copies), epilogue fused: + bias, GELU, or the
  *               GELU-derivative product of the backward pass, FP32 and/or
  *               BF16 stores
- *   k_gemm_nt   the same product when both operands are contiguous along K
+ *   k_gemm_nt128, k_gemm_nt
+ *               the same product when both operands are contiguous along K
  *               (forward pass and the data-gradient products): 128 x 128
- *               tiles, 64-deep K steps double-buffered in LDS
+ *               tiles in an XCD-aware order, 128-deep K steps with buffer
+ *               loads (K a multiple of 128) or 64-deep steps double-buffered
+ *               in LDS, an optional second K segment, LDS-staged epilogue
  *   k_transpose16  bf16 transposes (K-contiguous weight copies for k_gemm_nt)
  *   k_bits_pm1  descriptor bits -> +/-1 BF16 inputs
  *   k_softmax_ce  row softmax, cross-entropy and its gradient (scaled 1/n)
  *   k_colsum    bias gradients
  *   k_adam      Adam on FP32 master weights, BF16 copies refreshed
- *   k_probe     softmax max and the top-k buckets of each query
+ *   k_probe     softmax max and the exact top-k buckets of each query
  */
 #include "orb_internal.h"
 #include "orb_mlp.h"
