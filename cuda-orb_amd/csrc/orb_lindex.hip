@@ -258,13 +258,8 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
             query_operands(d0, d1, g, bq[qb]);
         }
         uint32_t k1[4], k2[4];
-        /* a result can enter the top-2 only if its dot reaches thr = 256 - (k2 >> 22) */
-        int thr[4];
 #pragma unroll
-        for (int qb = 0; qb < 4; qb++) {
-            k1[qb] = k2[qb] = kNoKey;
-            thr[qb] = 256 - (int)(kNoKey >> 22);
-        }
+        for (int qb = 0; qb < 4; qb++) k1[qb] = k2[qb] = kNoKey;
         const int j0 = offsets[b], j1 = offsets[b + 1];
         /* thread -> entry e of a tile, bytes 8 qq .. 8 qq + 7; the next tile's
            bytes are loaded while this tile is searched */
@@ -288,13 +283,13 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++)
                         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
-                    /* dots are even: below thr, all 4 keys lie above k2 */
                     const int m = max(max(acc[0], acc[1]), max(acc[2], acc[3]));
-                    if (m >= thr[qb]) {
+                    /* the key of the best dot with index 0 bounds the 4 keys
+                       from below: above k2, none can enter the top-2 */
+                    if (pm1_dot_key(m, 0u) <= k2[qb]) {
 #pragma unroll
                         for (int r = 0; r < 4; r++)
                             if (!PART || jb + r < j1) top2_push(k1[qb], k2[qb], pm1_dot_key(acc[r], (uint32_t)(jb + r)));
-                        thr[qb] = 256 - (int)(k2[qb] >> 22);
                     }
                 }
             }
