@@ -415,7 +415,7 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
 #pragma unroll
         for (int u = 0; u < 16; u++) {
             const int c = lane + 64 * u;
-            if (c < nb) {
+            if (64 * u < nb) { /* nb is a multiple of 64 */
                 const uint4 a = s_c[2 * c], b = s_c[2 * c + 1];
                 const int d = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
                               __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
@@ -442,6 +442,7 @@ hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, i
 hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
                                  float* pmax, hipStream_t s)
 {
+    if (nb <= 0 || nb > 1024 || nb % 64 || k < 1 || k > nb) return hipErrorInvalidValue;
     if (nq > 0)
         hipLaunchKernelGGL(k_centroid_probe, dim3((nq + 4 * kCenQ - 1) / (4 * kCenQ)), dim3(256), 0, s, q, nq, cen, nb,
                            k, probes, pmax);

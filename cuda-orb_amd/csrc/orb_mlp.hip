@@ -524,12 +524,10 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
     const int r = blockIdx.x * 4 + wv;
     if (r >= n) return;
     const float* z = Z + (long long)r * C;
+    /* C is a multiple of 64: slot t is present in every lane or in none */
     float v[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int c = lane + 64 * t;
-        v[t] = c < C ? z[c] : -INFINITY;
-    }
+    for (int t = 0; t < 16; t++) v[t] = 64 * t < C ? z[lane + 64 * t] : -INFINITY;
     float m = -INFINITY;
 #pragma unroll
     for (int t = 0; t < 16; t++) m = fmaxf(m, v[t]);
@@ -537,13 +535,13 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     float s = 0.f;
 #pragma unroll
-    for (int t = 0; t < 16; t++) s += (lane + 64 * t) < C ? __expf(v[t] - m) : 0.f;
+    for (int t = 0; t < 16; t++) s += 64 * t < C ? __expf(v[t] - m) : 0.f;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) pmax[r] = 1.f / s;
     uint32_t key[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) key[t] = (lane + 64 * t) < C ? float_order_key(v[t]) : 0u;
+    for (int t = 0; t < 16; t++) key[t] = 64 * t < C ? float_order_key(v[t]) : 0u;
     wave_topk<31>(key, k, s_cand[wv], probes + (long long)r * k);
 }
 
@@ -618,6 +616,7 @@ hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t 
 
 hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s)
 {
+    if (C <= 0 || C > 1024 || C % 64 || k < 1 || k > C) return hipErrorInvalidValue;
     if (n > 0) hipLaunchKernelGGL(k_probe, dim3((n + 3) / 4), dim3(256), 0, s, Z, n, C, k, probes, pmax);
     return hipGetLastError();
 }
