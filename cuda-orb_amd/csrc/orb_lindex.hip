@@ -393,10 +393,12 @@ __global__ void k_gather(const uint8_t* __restrict__ q, const int* __restrict__ 
 }
 
 /* the nprobe nearest centroids of each query in Hamming distance (ties to the
-   lower bucket id) -- the probe without the network; the centroids are staged
+   lower bucket id; by ascending distance with SORTED, else by bucket id) --
+   the probe without the network; the centroids are staged
    in LDS once per workgroup and each wavefront takes kCenQ queries in turn */
 constexpr int kCenQ = 8;
 
+template <bool SORTED>
 __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restrict__ q, int nq,
                                                         const uint8_t* __restrict__ cen, int nb, int k,
                                                         int* __restrict__ probes, float* __restrict__ pmax)
@@ -424,7 +426,7 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
                 key[u] = 0u;
             }
         }
-        wave_topk<16>(key, k, s_cand[wv], probes + (long long)r * k);
+        wave_topk<16, SORTED>(key, k, s_cand[wv], probes + (long long)r * k);
         if (lane == 0) pmax[r] = 1.f;
         wave_sync_lds(); /* the next query reuses s_cand */
     }
@@ -440,12 +442,14 @@ hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, i
 }
 
 hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
-                                 float* pmax, hipStream_t s)
+                                 float* pmax, bool sorted, hipStream_t s)
 {
     if (nb <= 0 || nb > 1024 || nb % 64 || k < 1 || k > nb) return hipErrorInvalidValue;
-    if (nq > 0)
-        hipLaunchKernelGGL(k_centroid_probe, dim3((nq + 4 * kCenQ - 1) / (4 * kCenQ)), dim3(256), 0, s, q, nq, cen, nb,
-                           k, probes, pmax);
+    const dim3 grid((nq + 4 * kCenQ - 1) / (4 * kCenQ));
+    if (nq > 0 && sorted)
+        hipLaunchKernelGGL(k_centroid_probe<true>, grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+    else if (nq > 0)
+        hipLaunchKernelGGL(k_centroid_probe<false>, grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
     return hipGetLastError();
 }
 

@@ -257,9 +257,10 @@ int backward(orb_lindex* L, int B, hipStream_t s)
     return ORB_OK;
 }
 
-/* the nprobe buckets of each of nq queries and their max softmax probability
-   (1 without the network) */
-int compute_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* probes, float* pmax, hipStream_t s)
+/* the nprobe buckets of each of nq queries (best first when sorted, else in
+   bucket-id order) and their max softmax probability (1 without the network) */
+int compute_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* probes, float* pmax, bool sorted,
+                   hipStream_t s)
 {
     const int np = L->prm.nprobe, nb = L->C;
     if (mode == ORB_LINDEX_MLP) {
@@ -272,10 +273,10 @@ int compute_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* pro
             LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
             rc = forward(L, B, s, false);
             if (rc != ORB_OK) return rc;
-            LTRY(L, launch_probe(L->Z3.p, B, nb, np, probes + (size_t)c0 * np, pmax + c0, s));
+            LTRY(L, launch_probe(L->Z3.p, B, nb, np, probes + (size_t)c0 * np, pmax + c0, sorted, s));
         }
     } else {
-        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, probes, pmax, s));
+        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, probes, pmax, sorted, s));
     }
     return ORB_OK;
 }
@@ -582,7 +583,8 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     LALLOC(L, L->ex_d1, nq);
     LALLOC(L, L->ex_idx2, nq);
     LALLOC(L, L->ex_d2, nq);
-    int rc = compute_probes(L, d_q, nq, mode, L->probes.p, L->pmax.p, s);
+    /* the search needs the probed set only: its order is left as it comes */
+    int rc = compute_probes(L, d_q, nq, mode, L->probes.p, L->pmax.p, false, s);
     if (rc != ORB_OK) return rc;
     LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->border.p, L->list_count.p, L->list_offsets.p,
                                L->list_cursor.p, L->lists.p, L->unit_offsets.p, L->unit_pos.p, s));
@@ -613,7 +615,7 @@ int orb_lindex_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* 
         return lfail(L, ORB_E_ARG, "nq must be a positive multiple of 64");
     if (L->ndb <= 0) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
     if (mode != ORB_LINDEX_MLP && mode != ORB_LINDEX_CENTROIDS) return lfail(L, ORB_E_ARG, "unknown mode");
-    return compute_probes(L, d_q, nq, mode, d_probes, d_pmax, stream_of(L));
+    return compute_probes(L, d_q, nq, mode, d_probes, d_pmax, true, stream_of(L));
 }
 
 int orb_lindex_set_probe(orb_lindex* L, int nprobe, float fallback)

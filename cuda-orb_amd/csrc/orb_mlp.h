@@ -50,7 +50,9 @@ hipError_t launch_colsum(const uint16_t* X, int n, int C, float* out, hipStream_
 hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
                        float b2, float eps, int step, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t s);
-hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s);
+/* top-k buckets per row of logits (ties to the lower id): best first when
+   sorted, else in bucket-id order (the same set) */
+hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, bool sorted, hipStream_t s);
 
 /* bucketed database (orb_lindex.hip) */
 /* stable counting sort of n entries by bucket (assign): offsets[0..nb], perm
@@ -85,7 +87,7 @@ hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, i
                          hipStream_t s);
 /* the k nearest centroids of each query (Hamming), ties to the lower id */
 hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
-                                 float* pmax, hipStream_t s);
+                                 float* pmax, bool sorted, hipStream_t s);
 /* compaction of the low-confidence queries for the exact fallback */
 hipError_t launch_fallback_compact(const uint8_t* q, int nq, const float* pmax, float thresh, int* fb_count,
                                    int* fb_index, int* fb_queries_idx, uint8_t* fb_q, hipStream_t s);

@@ -515,7 +515,9 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
 }
 
 /* one wavefront per query: max softmax probability and the top-k logits
-   (ties to the lower bucket id; C <= 1024) */
+   (ties to the lower bucket id; C <= 1024, a multiple of 64), by descending
+   logit with SORTED, else by bucket id */
+template <bool SORTED>
 __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k,
                                                int* __restrict__ probes, float* __restrict__ pmax)
 {
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
     uint32_t key[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) key[t] = 64 * t < C ? float_order_key(v[t]) : 0u;
-    wave_topk<31>(key, k, s_cand[wv], probes + (long long)r * k);
+    wave_topk<31, SORTED>(key, k, s_cand[wv], probes + (long long)r * k);
 }
 
 /* ---------------------------------------------------------------- launchers */
@@ -614,10 +616,13 @@ hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s)
+hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, bool sorted, hipStream_t s)
 {
     if (C <= 0 || C > 1024 || C % 64 || k < 1 || k > C) return hipErrorInvalidValue;
-    if (n > 0) hipLaunchKernelGGL(k_probe, dim3((n + 3) / 4), dim3(256), 0, s, Z, n, C, k, probes, pmax);
+    if (n > 0) {
+        if (sorted) hipLaunchKernelGGL(k_probe<true>, dim3((n + 3) / 4), dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        else hipLaunchKernelGGL(k_probe<false>, dim3((n + 3) / 4), dim3(256), 0, s, Z, n, C, k, probes, pmax);
+    }
     return hipGetLastError();
 }
 

@@ -63,16 +63,18 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
 
 /* Top-k of up to 1024 entries, entry lane + 64 t holding key[t] (key 0 marks
    an absent entry, every key below 2^(HIBIT+1); at least k entries present,
-   1 <= k <= 1024): writes the
-   entry numbers to out[0 .. k) by descending key, equal keys by ascending
-   entry number.  cand is this wavefront's LDS buffer of 1024 entries.
+   1 <= k <= 1024): writes the k entry numbers of the largest keys, equal keys
+   taken by ascending entry number, to out[0 .. k) -- with SORTED by
+   descending key (equal keys by ascending entry number), otherwise in entry
+   order.  cand is this wavefront's LDS buffer of 1024 entries.
 
    1. For k <= 64 the k-th largest lane maximum L bounds the answer from below
       (k lanes hold a key >= L); entries >= L are compacted into cand in entry
       order.  2. The exact k-th largest key T of those.  3. Entries above T
-      (fewer than k) are ranked against each other; entries equal to T take
-      the remaining ranks in entry order. */
-template <int HIBIT>
+      (fewer than k): SORTED ranks them against each other, otherwise they
+      keep entry order; entries equal to T take the remaining places in entry
+      order. */
+template <int HIBIT, bool SORTED>
 __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint2* cand, int* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
@@ -115,12 +117,15 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint
         if (s < S) {
             const bool p = ck[s] > T;
             const uint64_t m = __ballot(p);
-            if (p) cand[g + lanes_below(m)] = make_uint2(ck[s], (uint32_t)ci[s]);
+            if (p) {
+                if (SORTED) cand[g + lanes_below(m)] = make_uint2(ck[s], (uint32_t)ci[s]);
+                else out[g + lanes_below(m)] = ci[s];
+            }
             g += __popcll(m);
         }
     }
-    wave_sync_lds();
-    for (int i = lane; i < g; i += 64) {
+    if (SORTED) wave_sync_lds();
+    for (int i = lane; SORTED && i < g; i += 64) {
         const uint2 e = cand[i];
         int rank = 0;
 #pragma unroll 4
