@@ -444,16 +444,21 @@ __global__ __launch_bounds__(256) void k_transpose16(const uint16_t* __restrict_
 
 /* ------------------------------------------------------------ elementwise */
 
-/* descriptor bits -> +/-1 bf16 (bit k of the descriptor -> column k) */
+/* descriptor bits -> +/-1 bf16 (bit k of the descriptor -> column k): one
+   thread per descriptor byte, 8 bf16 values in one 16-byte store */
 __global__ void k_bits_pm1(const uint8_t* __restrict__ desc, int n, uint16_t pos, uint16_t neg,
                            uint16_t* __restrict__ X)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)n * 256) return;
-    const long long r = i >> 8;
-    const int k = (int)(i & 255);
-    const int bit = (desc[r * 32 + (k >> 3)] >> (k & 7)) & 1;
-    X[i] = bit ? pos : neg;
+    if (i >= (long long)n * 32) return;
+    const uint32_t b = desc[i];
+    uint32_t w[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const uint32_t lo = (b >> (2 * t)) & 1u ? pos : neg, hi = (b >> (2 * t + 1)) & 1u ? pos : neg;
+        w[t] = lo | (hi << 16);
+    }
+    *(uint4*)(X + i * 8) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 /* one wavefront per row of C logits: loss and bf16 gradient (softmax - onehot) / n */
@@ -578,7 +583,7 @@ hipError_t launch_transpose16(const uint16_t* src, int R, int C, uint16_t* dst, 
 
 hipError_t launch_bits_pm(const uint8_t* desc, int n, uint16_t pos, uint16_t neg, uint16_t* X, hipStream_t s)
 {
-    const long long tot = (long long)n * 256;
+    const long long tot = (long long)n * 32;
     if (tot > 0)
         hipLaunchKernelGGL(k_bits_pm1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, desc, n, pos, neg, X);
     return hipGetLastError();
