@@ -181,8 +181,14 @@ __global__ __launch_bounds__(256) void k_probe_count(const int* __restrict__ pro
     __shared__ int h[1024];
     for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
     __syncthreads();
-    const int i0 = blockIdx.x * kPlChunk, i1 = min(n, i0 + kPlChunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) atomicAdd(&h[probes[i]], 1);
+    /* all of a thread's entries loaded first, then counted */
+    const int i0 = blockIdx.x * kPlChunk + threadIdx.x, i1 = min(n, blockIdx.x * kPlChunk + kPlChunk);
+    int v[kPlChunk / 256];
+#pragma unroll
+    for (int t = 0; t < kPlChunk / 256; t++) v[t] = i0 + 256 * t < i1 ? probes[i0 + 256 * t] : -1;
+#pragma unroll
+    for (int t = 0; t < kPlChunk / 256; t++)
+        if (v[t] >= 0) atomicAdd(&h[v[t]], 1);
     __syncthreads();
     for (int b = threadIdx.x; b < nb; b += 256)
         if (h[b]) atomicAdd(&count[b], h[b]);
@@ -196,13 +202,20 @@ __global__ __launch_bounds__(256) void k_probe_fill(const int* __restrict__ prob
     __shared__ int h[1024];
     for (int b = threadIdx.x; b < nb; b += 256) h[b] = 0;
     __syncthreads();
-    const int i0 = blockIdx.x * kPlChunk, i1 = min(n, i0 + kPlChunk);
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) atomicAdd(&h[probes[i]], 1);
+    const int i0 = blockIdx.x * kPlChunk + threadIdx.x, i1 = min(n, blockIdx.x * kPlChunk + kPlChunk);
+    int v[kPlChunk / 256];
+#pragma unroll
+    for (int t = 0; t < kPlChunk / 256; t++) v[t] = i0 + 256 * t < i1 ? probes[i0 + 256 * t] : -1;
+#pragma unroll
+    for (int t = 0; t < kPlChunk / 256; t++)
+        if (v[t] >= 0) atomicAdd(&h[v[t]], 1);
     __syncthreads();
     for (int b = threadIdx.x; b < nb; b += 256)
         if (h[b]) h[b] = atomicAdd(&cursor[b], h[b]);
     __syncthreads();
-    for (int i = i0 + threadIdx.x; i < i1; i += 256) lists[atomicAdd(&h[probes[i]], 1)] = i;
+#pragma unroll
+    for (int t = 0; t < kPlChunk / 256; t++)
+        if (v[t] >= 0) lists[atomicAdd(&h[v[t]], 1)] = i0 + 256 * t;
 }
 
 /* unit -> its position in the bucket order, for every unit of position i */
