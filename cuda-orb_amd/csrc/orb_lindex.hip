@@ -29,10 +29,11 @@ __global__ void k_zero_int(int* p, int n)
 
 /* single workgroup: exclusive scan of nb counts (nb <= 4096) into offsets[0..nb]
    and of the unit counts ceil(count / unit) into units[0..nb] -- taken in the
-   bucket order order[] when given; thread t holds positions 4t .. 4t+3 */
+   bucket order order[] when given, with unit_pos[u] the position of unit u;
+   thread t holds positions 4t .. 4t+3 */
 __global__ __launch_bounds__(1024) void k_scan_offsets(const int* __restrict__ count, int nb, int* __restrict__ offsets,
                                                        int* __restrict__ cursor, int unit, int* __restrict__ units,
-                                                       const int* __restrict__ order)
+                                                       const int* __restrict__ order, int* __restrict__ unit_pos)
 {
     __shared__ int s_wc[16], s_wu[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -72,6 +73,8 @@ __global__ __launch_bounds__(1024) void k_scan_offsets(const int* __restrict__ c
             offsets[i] = ac;
             if (cursor) cursor[i] = ac;
             if (units) units[i] = au;
+            if (unit_pos)
+                for (int v = 0; v < u[j]; v++) unit_pos[au + v] = i; /* unit -> its position */
         }
         ac += c[j];
         au += u[j];
@@ -216,14 +219,6 @@ __global__ __launch_bounds__(256) void k_probe_fill(const int* __restrict__ prob
 #pragma unroll
     for (int t = 0; t < kPlChunk / 256; t++)
         if (v[t] >= 0) lists[atomicAdd(&h[v[t]], 1)] = i0 + 256 * t;
-}
-
-/* unit -> its position in the bucket order, for every unit of position i */
-__global__ void k_unit_pos(const int* __restrict__ unit_offsets, int nb, int* __restrict__ unit_pos)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    for (int u = unit_offsets[i]; u < unit_offsets[i + 1]; u++) unit_pos[u] = i;
 }
 
 /* Each unit: up to 256 list entries (query, slot) of one bucket x all of the
@@ -478,7 +473,7 @@ hipError_t launch_bucket_sort(const int* assign, int n, int nb, const uint8_t* d
     hipLaunchKernelGGL(k_chunk_hist, dim3(nchunks), dim3(256), 0, s, assign, n, nb, nchunks, H);
     hipLaunchKernelGGL(k_chunk_rows, dim3(nb), dim3(256), 0, s, H, nchunks, 0, hist, (const int*)nullptr);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, (const int*)hist, nb, offsets, (int*)nullptr, 0,
-                       (int*)nullptr, (const int*)nullptr);
+                       (int*)nullptr, (const int*)nullptr, (int*)nullptr);
     hipLaunchKernelGGL(k_chunk_rows, dim3(nb), dim3(256), 0, s, H, nchunks, 1, (int*)nullptr, (const int*)offsets);
     hipLaunchKernelGGL(k_chunk_scatter, dim3(nchunks), dim3(256), 0, s, assign, n, nchunks, (const int*)H, db, perm,
                        db_sorted);
@@ -502,8 +497,7 @@ hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, con
     const dim3 chunks((unsigned)((n + kPlChunk - 1) / kPlChunk));
     if (n > 0) hipLaunchKernelGGL(k_probe_count, chunks, dim3(256), 0, s, probes, (int)n, nb, list_count);
     hipLaunchKernelGGL(k_scan_offsets, dim3(1), dim3(1024), 0, s, list_count, nb, list_offsets, list_cursor, kLxQ,
-                       unit_offsets, order);
-    hipLaunchKernelGGL(k_unit_pos, grid1(nb), dim3(256), 0, s, unit_offsets, nb, unit_pos);
+                       unit_offsets, order, unit_pos);
     if (n > 0) hipLaunchKernelGGL(k_probe_fill, chunks, dim3(256), 0, s, probes, (int)n, nb, list_cursor, lists);
     return hipGetLastError();
 }

@@ -592,17 +592,26 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
                                  L->db_sorted.p, L->perm.p, L->offsets.p, L->list_count.p + nb, L->partial.p,
                                  L->ncu, s));
     const float thresh = mode == ORB_LINDEX_MLP ? L->prm.fallback : -1.f;
-    LTRY(L, launch_fallback_compact(d_q, nq, L->pmax.p, thresh, L->fb_count.p, L->fb_index.p, L->fb_qidx.p,
-                                    L->fb_q.p, s));
-    /* exact search of the low-confidence queries (count read on the device) */
-    rc = orb__match_dcount_top2(L->ctx, L->fb_q.p, nq, L->fb_count.p, L->db.p, L->ndb, L->ex_idx1.p, L->ex_d1.p,
+    /* the max softmax probability is above 0: a threshold at or below 0 (and
+       centroid mode) sends no query to the exact search, which is skipped */
+    const bool may_fallback = thresh > 0.f;
+    if (may_fallback) {
+        LTRY(L, launch_fallback_compact(d_q, nq, L->pmax.p, thresh, L->fb_count.p, L->fb_index.p, L->fb_qidx.p,
+                                        L->fb_q.p, s));
+        /* exact search of the low-confidence queries (count read on the device) */
+        rc = orb__match_dcount_top2(L->ctx, L->fb_q.p, nq, L->fb_count.p, L->db.p, L->ndb, L->ex_idx1.p, L->ex_d1.p,
                                     L->ex_idx2.p, L->ex_d2.p);
-    if (rc != ORB_OK) return rc;
-    LTRY(L, launch_probe_merge(L->partial.p, nq, np, L->fb_index.p, L->ex_idx1.p, L->ex_d1.p, L->ex_idx2.p,
-                               L->ex_d2.p, d_idx1, d_d1, d_idx2, d_d2, s));
+        if (rc != ORB_OK) return rc;
+    }
+    LTRY(L, launch_probe_merge(L->partial.p, nq, np, may_fallback ? L->fb_index.p : nullptr, L->ex_idx1.p,
+                               L->ex_d1.p, L->ex_idx2.p, L->ex_d2.p, d_idx1, d_d1, d_idx2, d_d2, s));
     if (h_nfallback) {
-        LTRY(L, hipMemcpyAsync(h_nfallback, L->fb_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        LTRY(L, hipStreamSynchronize(s));
+        if (may_fallback) {
+            LTRY(L, hipMemcpyAsync(h_nfallback, L->fb_count.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            LTRY(L, hipStreamSynchronize(s));
+        } else {
+            *h_nfallback = 0;
+        }
     }
     return ORB_OK;
 }
