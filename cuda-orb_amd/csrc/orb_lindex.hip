@@ -269,6 +269,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) k1[qb] = k2[qb] = kNoKey;
         const int j0 = offsets[b], j1 = offsets[b + 1];
+        const bool wave_busy = e0 + wid * 64 < e1;
         /* thread -> entry e of a tile, bytes 8 qq .. 8 qq + 7; the next tile's
            bytes are loaded while this tile is searched */
         const int e = tid >> 2, qq = tid & 3;
@@ -317,6 +318,9 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                 if (jn < j1) nv = *(const uint2*)(db_sorted + (long long)jn * 32 + qq * 8);
             }
             __syncthreads();
+            /* a wavefront without queries (the tail of a bucket's last unit)
+               only helps to stage the tiles */
+            if (!wave_busy) continue;
             if (jt + kLxTile <= j1) search_tile(jt, std::false_type{});
             else search_tile(jt, std::true_type{});
         }
