@@ -17,7 +17,7 @@
 namespace orbk
 {
 
-constexpr int kLxRow = 256 + 16; /* LDS row of one unpacked entry (padded) */
+constexpr int kLxRow = 256 + 32; /* LDS row of one unpacked entry (padded) */
 constexpr int kLxTile = 64;      /* entries per LDS tile */
 constexpr int kLxQ = 256;        /* queries per unit */
 

@@ -35,7 +35,7 @@ constexpr int kGranule = 64;          /* database entries per chunk granule (pop
 constexpr int kMinChunkGranules = 2;  /* at least 128 entries per chunk: bounds the merge work */
 constexpr int kPopQ = 64;             /* popcount: queries per wavefront */
 constexpr int kMfmaQ = 256;           /* MFMA: queries per workgroup */
-constexpr int kMfmaRow = 256 + 16;    /* LDS row of one unpacked entry (padded) */
+constexpr int kMfmaRow = 256 + 32;    /* LDS row of one unpacked entry (padded) */
 
 /* Split nq queries x ndb entries into qunits x chunks units for `slots`
    parallel workers (identical on host and device). */
