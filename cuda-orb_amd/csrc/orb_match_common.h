@@ -33,7 +33,11 @@ __device__ __forceinline__ void top2_push(uint32_t& k1, uint32_t& k2, uint32_t k
     k1 = min(k1, key);
 }
 
-/* a * b + c for a, b in the signed 24-bit range: one instruction */
+/* a * b + c for a, b in the signed 24-bit range: one instruction.  The
+   compiler does not insert MFMA-result wait states before inline asm, so a
+   must not come straight from an MFMA: in the matchers the block test (plain
+   max instructions, waited on) reads the accumulators first and the keys are
+   built inside the branch it guards. */
 __device__ __forceinline__ uint32_t mad_i24(int a, int b, uint32_t c)
 {
     uint32_t r;
