@@ -115,10 +115,19 @@ def test_probe_selection_matches_oracle(torch, ctx, small_index, nprobe):
         assert np.array_equal(pr.cpu().numpy(), want)
         np.testing.assert_allclose(pm.cpu().numpy(), wpm, rtol=1e-5)
         pr, pm = lx.probes(dq, mode=orb_amd.ORB_LINDEX_CENTROIDS)
-        _, cen, _ = lx.bucket_info()
+        _, cen, bof = lx.bucket_info()
         D = (np.unpackbits(q, axis=1)[:, None, :] != np.unpackbits(cen, axis=1)[None, :, :]).sum(-1)
-        assert np.array_equal(pr.cpu().numpy(), np.argsort(D, axis=1, kind="stable")[:, :nprobe])
+        cprobes = np.argsort(D, axis=1, kind="stable")[:, :nprobe]
+        assert np.array_equal(pr.cpu().numpy(), cprobes)
         assert (pm.cpu().numpy() == 1).all()
+        # the search takes the same probed sets (selected in bucket order on its own path)
+        lx.set_probe(nprobe, 0.0)
+        bof = bof.cpu().numpy()
+        for mode, probes in ((orb_amd.ORB_LINDEX_MLP, want), (orb_amd.ORB_LINDEX_CENTROIDS, cprobes)):
+            (i1, d1, _, _), nfb = lx.query(dq, mode=mode)
+            assert nfb == 0
+            want_i, want_d = mlp.bucket_search(q, db, bof, probes)
+            assert np.array_equal(i1.cpu().numpy(), want_i) and np.array_equal(d1.cpu().numpy(), want_d)
     finally:
         lx.set_probe(4, 0.3)
     tied = orb_amd.LearnedIndex(ctx, nbuckets=256, hidden=64, nprobe=nprobe, fallback=0.3, seed=8,
