@@ -27,6 +27,12 @@ __global__ void k_zero_int(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 /* single workgroup: exclusive scan of nb counts (nb <= 4096) into offsets[0..nb]
    and of the unit counts ceil(count / unit) into units[0..nb] -- taken in the
    bucket order order[] when given, with unit_pos[u] the position of unit u;
@@ -236,7 +242,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                                                           const uint8_t* __restrict__ db_sorted,
                                                           const int* __restrict__ perm,
                                                           const int* __restrict__ offsets, int* __restrict__ next,
-                                                          uint2* __restrict__ partial)
+                                                          uint32_t* __restrict__ qbound, uint2* __restrict__ partial)
 {
     __shared__ __align__(16) int8_t s_db[kLxTile * kLxRow];
     __shared__ int s_u;
@@ -253,21 +259,28 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
         const int b = order[lo];
         const int e0 = list_offsets[b] + (u - unit_offsets[lo]) * kLxQ, e1 = min(list_offsets[b + 1], e0 + kLxQ);
         v4i bq[4][4];
+        /* A query's second distance found so far in any of its buckets
+           (qbound) bounds its final second distance: entries farther than it
+           cannot enter the final top-2 and are kept out by starting the
+           top-2 at the phantom key (bound << 23 | kIdxMask), above every
+           real key of that distance; phantoms are dropped at the end. */
+        uint32_t k1[4], k2[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
             const int e = e0 + wid * 64 + qb * 16 + col;
             const int ent = e < e1 ? lists[e] : -1;
             uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+            uint32_t bd = 0xffffffffu;
             if (ent >= 0) {
-                const uint4* qp = (const uint4*)(q + (long long)(ent / nprobe) * 32);
+                const int qi = ent / nprobe;
+                const uint4* qp = (const uint4*)(q + (long long)qi * 32);
                 d0 = qp[0];
                 d1 = qp[1];
+                bd = __hip_atomic_load(qbound + qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             query_operands(d0, d1, g, bq[qb]);
+            k1[qb] = k2[qb] = bd <= 256u ? (bd << kIdxBits) | kIdxMask : kNoKey;
         }
-        uint32_t k1[4], k2[4];
-#pragma unroll
-        for (int qb = 0; qb < 4; qb++) k1[qb] = k2[qb] = kNoKey;
         const int j0 = offsets[b], j1 = offsets[b + 1];
         const bool wave_busy = e0 + wid * 64 < e1;
         /* thread -> entry e of a tile, bytes 8 qq .. 8 qq + 7; the next tile's
@@ -334,10 +347,14 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
             }
             const int el = e0 + wid * 64 + qb * 16 + col;
             if (g == 0 && el < e1) {
-                /* sorted positions back to database indices */
-                const uint32_t a1 = k1[qb] == kNoKey ? kNoKey : (k1[qb] & ~kIdxMask) | (uint32_t)perm[k1[qb] & kIdxMask];
-                const uint32_t a2 = k2[qb] == kNoKey ? kNoKey : (k2[qb] & ~kIdxMask) | (uint32_t)perm[k2[qb] & kIdxMask];
-                partial[lists[el]] = make_uint2(a1, a2);
+                /* phantoms dropped, sorted positions back to database indices
+                   (a real position is below kIdxMask: fewer than 2^23 - 1 entries) */
+                const bool r1 = (k1[qb] & kIdxMask) != kIdxMask, r2 = (k2[qb] & kIdxMask) != kIdxMask;
+                const uint32_t a1 = r1 ? (k1[qb] & ~kIdxMask) | (uint32_t)perm[k1[qb] & kIdxMask] : kNoKey;
+                const uint32_t a2 = r2 ? (k2[qb] & ~kIdxMask) | (uint32_t)perm[k2[qb] & kIdxMask] : kNoKey;
+                const int ent = lists[el];
+                partial[ent] = make_uint2(a1, a2);
+                if (r2) atomicMin(qbound + ent / nprobe, k2[qb] >> kIdxBits);
             }
         }
     }
@@ -509,14 +526,16 @@ hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, con
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
                                 const int* unit_offsets, const int* unit_pos, const int* order, int nb,
                                 const uint8_t* db_sorted, const int* perm, const int* offsets, int* next,
-                                uint2* partial, int ncu, hipStream_t s)
+                                uint32_t* qbound, int nq, uint2* partial, int ncu, hipStream_t s)
 {
+    /* no bound yet for any query */
+    hipLaunchKernelGGL(k_fill_u32, grid1(nq), dim3(256), 0, s, qbound, nq, 0xffffffffu);
     /* as many workgroups as stay resident: they take units until none are left */
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_bucket_search, 256, 0) != hipSuccess || per < 1) per = 1;
     const int blocks = per * (ncu > 0 ? ncu : 1);
     hipLaunchKernelGGL(k_bucket_search, dim3(blocks), dim3(256), 0, s, q, nprobe, lists, list_offsets, unit_offsets,
-                       unit_pos, order, nb, db_sorted, perm, offsets, next, partial);
+                       unit_pos, order, nb, db_sorted, perm, offsets, next, qbound, partial);
     return hipGetLastError();
 }
 

@@ -100,6 +100,7 @@ struct orb_lindex {
     DBuf<float> pmax;
     DBuf<uint8_t> fb_q;
     DBuf<uint2> partial;
+    DBuf<uint32_t> qbound; /* per query: the smallest second distance found so far (bucket search) */
 };
 
 namespace
@@ -575,6 +576,7 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     LALLOC(L, L->unit_pos, (size_t)nq * np / 256 + nb);
     LALLOC(L, L->lists, (size_t)nq * np);
     LALLOC(L, L->partial, (size_t)nq * np);
+    LALLOC(L, L->qbound, nq);
     LALLOC(L, L->fb_count, 1);
     LALLOC(L, L->fb_index, nq);
     LALLOC(L, L->fb_qidx, nq);
@@ -589,8 +591,8 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->border.p, L->list_count.p, L->list_offsets.p,
                                L->list_cursor.p, L->lists.p, L->unit_offsets.p, L->unit_pos.p, s));
     LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, L->unit_pos.p, L->border.p, nb,
-                                 L->db_sorted.p, L->perm.p, L->offsets.p, L->list_count.p + nb, L->partial.p,
-                                 L->ncu, s));
+                                 L->db_sorted.p, L->perm.p, L->offsets.p, L->list_count.p + nb, L->qbound.p, nq,
+                                 L->partial.p, L->ncu, s));
     const float thresh = mode == ORB_LINDEX_MLP ? L->prm.fallback : -1.f;
     /* the max softmax probability is above 0: a threshold at or below 0 (and
        centroid mode) sends no query to the exact search, which is skipped */

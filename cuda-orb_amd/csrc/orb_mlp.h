@@ -71,12 +71,13 @@ hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, con
                               int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, int* unit_pos,
                               hipStream_t s);
 /* bucketed exact search: for every (query, probed bucket) the top-2 keys
-   (distance << 23 | original index) over that bucket's entries; units taken
-   from the zeroed counter *next */
+   (distance << 23 | original index) over that bucket's entries, leaving out
+   entries farther than the query's second distance in its other buckets
+   (qbound: nq words of scratch); units taken from the zeroed counter *next */
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
                                 const int* unit_offsets, const int* unit_pos, const int* order, int nb,
                                 const uint8_t* db_sorted, const int* perm, const int* offsets, int* next,
-                                uint2* partial, int ncu, hipStream_t s);
+                                uint32_t* qbound, int nq, uint2* partial, int ncu, hipStream_t s);
 /* per query: merge of the probed buckets, or the exact result when the query
    was sent to the fallback (fb_index >= 0) */
 hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,
