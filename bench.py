@@ -277,7 +277,7 @@ def learned_match(ctx, torch, dev, log=None):
     note(f"trained, losses {losses}")
     # the trained network against nearest-centroid probing, same probe counts, no fallback
     versus = []
-    for npb in (4, 8, 16, 32):
+    for npb in (4, 8, 16, 20, 24, 28, 32):
         lx.set_probe(npb, 0.0)
         (o, _), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
         (oc, _), msc = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS), reps=2)
@@ -287,8 +287,8 @@ def learned_match(ctx, torch, dev, log=None):
     lx.set_probe(8, 0.3)
     (lq, nfb), ms_learned = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
     sweep = []
-    for npb in (8, 16, 32):
-        for fb in (0.0, 0.3, 0.5):
+    for npb in (8, 16, 20, 24, 28, 32):
+        for fb in ((0.0, 0.3, 0.5) if npb in (8, 16, 32) else (0.0,)):
             lx.set_probe(npb, fb)
             (o, nf), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
             sweep.append({"nprobe": npb, "fallback": fb, "ms": round(ms, 3), "recall_at_1": round(rec(o), 5),
