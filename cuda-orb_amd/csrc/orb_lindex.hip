@@ -424,22 +424,33 @@ __global__ void k_gather(const uint8_t* __restrict__ q, const int* __restrict__ 
 /* the nprobe nearest centroids of each query in Hamming distance (ties to the
    lower bucket id; by ascending distance with SORTED, else by bucket id) --
    the probe without the network; the centroids are staged
-   in LDS once per workgroup and each wavefront takes kCenQ queries in turn */
+   in LDS once per workgroup and each wavefront takes kCenQ queries in turn.
+   CAP and FIX as for the network's probe: with CAP below 1024 a query whose
+   candidates do not fit gets -1 as its first probe, and the FIX launch
+   (CAP 1024) redoes those queries, its workgroups without one leaving before
+   staging the centroids. */
 constexpr int kCenQ = 8;
+constexpr int kCenCap = 256;
 
-template <bool SORTED>
+template <bool SORTED, int CAP, bool FIX>
 __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restrict__ q, int nq,
                                                         const uint8_t* __restrict__ cen, int nb, int k,
                                                         int* __restrict__ probes, float* __restrict__ pmax)
 {
     __shared__ uint4 s_c[1024 * 2];
-    __shared__ uint2 s_cand[4][1024];
+    __shared__ uint2 s_cand[4][CAP];
+    if (FIX) {
+        const int r = blockIdx.x * 4 * kCenQ + (int)threadIdx.x;
+        const bool redo = threadIdx.x < 4 * kCenQ && r < nq && probes[(long long)r * k] < 0;
+        if (!__syncthreads_or(redo)) return;
+    }
     for (int i = threadIdx.x; i < nb * 2; i += 256) s_c[i] = ((const uint4*)cen)[i];
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int t = 0; t < kCenQ; t++) {
         const int r = (blockIdx.x * 4 + wv) * kCenQ + t;
         if (r >= nq) return;
+        if (FIX && probes[(long long)r * k] >= 0) continue;
         const uint4 qa = ((const uint4*)(q + (long long)r * 32))[0], qb = ((const uint4*)(q + (long long)r * 32))[1];
         /* key 0x10000 - distance: nearer centroids rank higher; 0 marks no centroid */
         uint32_t key[16];
@@ -455,7 +466,8 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
                 key[u] = 0u;
             }
         }
-        wave_topk<16, SORTED>(key, k, s_cand[wv], probes + (long long)r * k);
+        const bool ok = wave_topk<16, SORTED, CAP>(key, k, s_cand[wv], probes + (long long)r * k);
+        if (!ok && lane == 0) probes[(long long)r * k] = -1;
         if (lane == 0) pmax[r] = 1.f;
         wave_sync_lds(); /* the next query reuses s_cand */
     }
@@ -475,10 +487,21 @@ hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, i
 {
     if (nb <= 0 || nb > 1024 || nb % 64 || k < 1 || k > nb) return hipErrorInvalidValue;
     const dim3 grid((nq + 4 * kCenQ - 1) / (4 * kCenQ));
-    if (nq > 0 && sorted)
-        hipLaunchKernelGGL(k_centroid_probe<true>, grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
-    else if (nq > 0)
-        hipLaunchKernelGGL(k_centroid_probe<false>, grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+    if (nq <= 0) return hipGetLastError();
+    const bool split = k <= 64 && nb > kCenCap;
+    if (sorted && split) {
+        hipLaunchKernelGGL((k_centroid_probe<true, kCenCap, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+        hipLaunchKernelGGL((k_centroid_probe<true, 1024, true>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+    } else if (sorted) {
+        hipLaunchKernelGGL((k_centroid_probe<true, 1024, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+    } else if (split) {
+        hipLaunchKernelGGL((k_centroid_probe<false, kCenCap, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes,
+                           pmax);
+        hipLaunchKernelGGL((k_centroid_probe<false, 1024, true>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+    } else {
+        hipLaunchKernelGGL((k_centroid_probe<false, 1024, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes,
+                           pmax);
+    }
     return hipGetLastError();
 }
 

@@ -521,15 +521,19 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
 
 /* one wavefront per query: max softmax probability and the top-k logits
    (ties to the lower bucket id; C <= 1024, a multiple of 64), by descending
-   logit with SORTED, else by bucket id */
-template <bool SORTED>
+   logit with SORTED, else by bucket id.  With CAP below 1024 (LDS for CAP
+   candidates per wavefront, which leaves room for more resident wavefronts)
+   a query whose candidates do not fit gets -1 as its first probe; the FIX
+   launch (CAP 1024) redoes exactly those queries. */
+template <bool SORTED, int CAP, bool FIX>
 __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k,
                                                int* __restrict__ probes, float* __restrict__ pmax)
 {
-    __shared__ uint2 s_cand[4][1024];
+    __shared__ uint2 s_cand[4][CAP];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + wv;
     if (r >= n) return;
+    if (FIX && probes[(long long)r * k] >= 0) return;
     const float* z = Z + (long long)r * C;
     /* C is a multiple of 64: slot t is present in every lane or in none */
     float v[16];
@@ -549,7 +553,23 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
     uint32_t key[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) key[t] = 64 * t < C ? float_order_key(v[t]) : 0u;
-    wave_topk<31, SORTED>(key, k, s_cand[wv], probes + (long long)r * k);
+    if (!wave_topk<31, SORTED, CAP>(key, k, s_cand[wv], probes + (long long)r * k) && lane == 0)
+        probes[(long long)r * k] = -1;
+}
+
+/* candidates one wavefront of the first probe pass holds in LDS (2 KB) */
+constexpr int kProbeCap = 256;
+
+template <bool SORTED>
+static void probe_launch(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s)
+{
+    const dim3 grid((n + 3) / 4);
+    if (k <= 64 && C > kProbeCap) {
+        hipLaunchKernelGGL((k_probe<SORTED, kProbeCap, false>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        hipLaunchKernelGGL((k_probe<SORTED, 1024, true>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+    } else {
+        hipLaunchKernelGGL((k_probe<SORTED, 1024, false>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+    }
 }
 
 /* ---------------------------------------------------------------- launchers */
@@ -625,8 +645,8 @@ hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float*
 {
     if (C <= 0 || C > 1024 || C % 64 || k < 1 || k > C) return hipErrorInvalidValue;
     if (n > 0) {
-        if (sorted) hipLaunchKernelGGL(k_probe<true>, dim3((n + 3) / 4), dim3(256), 0, s, Z, n, C, k, probes, pmax);
-        else hipLaunchKernelGGL(k_probe<false>, dim3((n + 3) / 4), dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        if (sorted) probe_launch<true>(Z, n, C, k, probes, pmax, s);
+        else probe_launch<false>(Z, n, C, k, probes, pmax, s);
     }
     return hipGetLastError();
 }

@@ -66,7 +66,9 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
    1 <= k <= 1024): writes the k entry numbers of the largest keys, equal keys
    taken by ascending entry number, to out[0 .. k) -- with SORTED by
    descending key (equal keys by ascending entry number), otherwise in entry
-   order.  cand is this wavefront's LDS buffer of 1024 entries.
+   order.  cand is this wavefront's LDS buffer of CAP entries; with CAP below
+   1024, more than CAP candidates (ties at the k-th lane maximum, or k > 64)
+   make it return false having written nothing, otherwise it returns true.
 
    1. For k <= 64 the k-th largest lane maximum L bounds the answer from below
       (k lanes hold a key >= L); entries >= L are compacted into cand in entry
@@ -74,9 +76,11 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
       (fewer than k): SORTED ranks them against each other, otherwise they
       keep entry order; entries equal to T take the remaining places in entry
       order. */
-template <int HIBIT, bool SORTED>
-__device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint2* cand, int* __restrict__ out)
+template <int HIBIT, bool SORTED, int CAP = 1024>
+__device__ __forceinline__ bool wave_topk(const uint32_t (&key)[16], int k, uint2* cand, int* __restrict__ out)
 {
+    static_assert(CAP % 64 == 0 && CAP >= 64 && CAP <= 1024, "whole slots of 64 candidates");
+    constexpr int NSL = CAP / 64;
     const int lane = threadIdx.x & 63;
     uint32_t L = 1;
     if (k <= 64) {
@@ -90,30 +94,33 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint
     for (int t = 0; t < 16; t++) {
         const bool p = key[t] >= L;
         const uint64_t m = __ballot(p);
-        if (p) cand[nc + lanes_below(m)] = make_uint2(key[t], (uint32_t)(lane + 64 * t));
+        const int pos = nc + lanes_below(m);
+        if (p && pos < CAP) cand[pos] = make_uint2(key[t], (uint32_t)(lane + 64 * t));
         nc += __popcll(m);
     }
+    if (nc > CAP) return false;
     wave_sync_lds();
     const int S = (nc + 63) >> 6;
-    uint32_t ck[16];
-    int ci[16];
+    uint32_t ck[NSL];
+    int ci[NSL];
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < NSL; s++) {
         const int j = lane + 64 * s;
         const uint2 e = (s < S && j < nc) ? cand[j] : make_uint2(0u, 0u);
         ck[s] = e.x;
         ci[s] = (int)e.y;
     }
     /* slots at and above S hold key 0: only the occupied ones are counted */
+    constexpr int N2 = NSL < 2 ? NSL : 2, N4 = NSL < 4 ? NSL : 4, N8 = NSL < 8 ? NSL : 8;
     const uint32_t T = S <= 1   ? wave_kth_key<1, HIBIT>(ck, k)
-                       : S <= 2 ? wave_kth_key<2, HIBIT>(ck, k)
-                       : S <= 4 ? wave_kth_key<4, HIBIT>(ck, k)
-                       : S <= 8 ? wave_kth_key<8, HIBIT>(ck, k)
-                                : wave_kth_key<16, HIBIT>(ck, k);
+                       : S <= 2 ? wave_kth_key<N2, HIBIT>(ck, k)
+                       : S <= 4 ? wave_kth_key<N4, HIBIT>(ck, k)
+                       : S <= 8 ? wave_kth_key<N8, HIBIT>(ck, k)
+                                : wave_kth_key<NSL, HIBIT>(ck, k);
     wave_sync_lds();
     int g = 0;
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < NSL; s++) {
         if (s < S) {
             const bool p = ck[s] > T;
             const uint64_t m = __ballot(p);
@@ -137,7 +144,7 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint
     }
     int base = g;
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < NSL; s++) {
         if (s < S && base < k) {
             const bool p = ck[s] == T;
             const uint64_t m = __ballot(p);
@@ -146,6 +153,7 @@ __device__ __forceinline__ void wave_topk(const uint32_t (&key)[16], int k, uint
             base += __popcll(m);
         }
     }
+    return true;
 }
 
 } // namespace orbk

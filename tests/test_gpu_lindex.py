@@ -140,6 +140,53 @@ def test_probe_selection_matches_oracle(torch, ctx, small_index, nprobe):
     tied.close()
 
 
+@pytest.mark.parametrize("case", ["random", "ties"])
+def test_probe_selection_with_many_candidates(torch, ctx, case):
+    """1024 buckets: the probe's first pass keeps at most 256 candidates per query and
+    leaves the queries whose candidates do not fit to a second pass.  "random": few such
+    queries; "ties": a database of two distinct descriptors, so the centroid distances of a
+    query, and the logits of the untrained network (centroid term only), take two values:
+    every query goes to the second pass.  Probes (best first) and the search over the
+    unordered probed sets equal the oracle in both modes."""
+    import orb_amd
+    from oracle import mlp
+    rng = np.random.default_rng(21)
+    if case == "ties":
+        two = rng.integers(0, 256, (2, 32), dtype=np.uint8)
+        db = two[rng.integers(0, 2, 20000)]
+    else:
+        db = rng.integers(0, 256, (20000, 32), dtype=np.uint8)
+    print(f"many candidates: {case}", flush=True)
+    ddb = torch.from_numpy(db).cuda()
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=1024, hidden=64, nprobe=32, fallback=0.0, seed=6,
+                              centroid_scale=0.25)
+    try:
+        lx.build(ddb, iters=2)
+        q, _ = flip_copies(rng, db, 640, p=0.1)
+        dq = torch.from_numpy(q).cuda()
+        _, cen, bof = lx.bucket_info()
+        bof = bof.cpu().numpy()
+        D = (np.unpackbits(q, axis=1)[:, None, :] != np.unpackbits(cen, axis=1)[None, :, :]).sum(-1)
+        logits = lx.logits(dq).cpu().numpy().astype(np.float64)
+        if case == "ties":
+            assert max(len(np.unique(row)) for row in logits) <= 2
+        for nprobe in (32, 64):
+            lx.set_probe(nprobe, 0.0)
+            want = mlp.probe(logits, nprobe)[0]
+            pr, _ = lx.probes(dq)
+            assert np.array_equal(pr.cpu().numpy(), want)
+            cprobes = np.argsort(D, axis=1, kind="stable")[:, :nprobe]
+            pr, _ = lx.probes(dq, mode=orb_amd.ORB_LINDEX_CENTROIDS)
+            assert np.array_equal(pr.cpu().numpy(), cprobes)
+            for mode, probes in ((orb_amd.ORB_LINDEX_MLP, want), (orb_amd.ORB_LINDEX_CENTROIDS, cprobes)):
+                (i1, d1, _, _), nfb = lx.query(dq, mode=mode)
+                assert nfb == 0
+                want_i, want_d = mlp.bucket_search(q, db, bof, probes)
+                assert np.array_equal(i1.cpu().numpy(), want_i) and np.array_equal(d1.cpu().numpy(), want_d)
+    finally:
+        lx.close()
+
+
 def test_centroid_probe_search_equals_brute_force(torch, ctx, small_index):
     import orb_amd
     from oracle import mlp
