@@ -519,21 +519,15 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
     if (i < n) y[i] = f2bf(x[i]);
 }
 
-/* one wavefront per query: max softmax probability and the top-k logits
+/* Max softmax probability and the top-k logits of query r, one wavefront
    (ties to the lower bucket id; C <= 1024, a multiple of 64), by descending
-   logit with SORTED, else by bucket id.  With CAP below 1024 (LDS for CAP
-   candidates per wavefront, which leaves room for more resident wavefronts)
-   a query whose candidates do not fit gets -1 as its first probe; the FIX
-   launch (CAP 1024) redoes exactly those queries. */
-template <bool SORTED, int CAP, bool FIX>
-__global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k,
-                                               int* __restrict__ probes, float* __restrict__ pmax)
+   logit with SORTED, else by bucket id; cand holds CAP candidates.  False:
+   more than CAP candidates, nothing written to the probes. */
+template <bool SORTED, int CAP>
+__device__ __forceinline__ bool probe_row(const float* __restrict__ Z, int r, int C, int k, int* __restrict__ probes,
+                                          float* __restrict__ pmax, uint2* cand)
 {
-    __shared__ uint2 s_cand[4][CAP];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + wv;
-    if (r >= n) return;
-    if (FIX && probes[(long long)r * k] >= 0) return;
+    const int lane = threadIdx.x & 63;
     const float* z = Z + (long long)r * C;
     /* C is a multiple of 64: slot t is present in every lane or in none */
     float v[16];
@@ -553,8 +547,38 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
     uint32_t key[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) key[t] = 64 * t < C ? float_order_key(v[t]) : 0u;
-    if (!wave_topk<31, SORTED, CAP>(key, k, s_cand[wv], probes + (long long)r * k) && lane == 0)
-        probes[(long long)r * k] = -1;
+    return wave_topk<31, SORTED, CAP>(key, k, cand, probes + (long long)r * k);
+}
+
+/* one wavefront per query.  With CAP below 1024 (LDS for CAP candidates per
+   wavefront, which leaves room for more resident wavefronts) a query whose
+   candidates do not fit gets -1 as its first probe, for k_probe_fix. */
+template <bool SORTED, int CAP>
+__global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k,
+                                               int* __restrict__ probes, float* __restrict__ pmax)
+{
+    __shared__ uint2 s_cand[4][CAP];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + wv;
+    if (r >= n) return;
+    if (!probe_row<SORTED, CAP>(Z, r, C, k, probes, pmax, s_cand[wv]) && lane == 0) probes[(long long)r * k] = -1;
+}
+
+/* the queries k_probe marked, redone with room for every candidate: each
+   wavefront reads the marks of 64 queries and takes the marked ones in turn */
+template <bool SORTED>
+__global__ __launch_bounds__(256) void k_probe_fix(const float* __restrict__ Z, int n, int C, int k,
+                                                   int* __restrict__ probes, float* __restrict__ pmax)
+{
+    __shared__ uint2 s_cand[4][1024];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int base = (blockIdx.x * 4 + wv) * 64;
+    const bool redo = base + lane < n && probes[(long long)(base + lane) * k] < 0;
+    for (uint64_t m = __ballot(redo); m; m &= m - 1) {
+        const int r = base + __ffsll((unsigned long long)m) - 1;
+        probe_row<SORTED, 1024>(Z, r, C, k, probes, pmax, s_cand[wv]);
+        wave_sync_lds(); /* the next query reuses s_cand */
+    }
 }
 
 /* candidates one wavefront of the first probe pass holds in LDS (2 KB) */
@@ -565,10 +589,10 @@ static void probe_launch(const float* Z, int n, int C, int k, int* probes, float
 {
     const dim3 grid((n + 3) / 4);
     if (k <= 64 && C > kProbeCap) {
-        hipLaunchKernelGGL((k_probe<SORTED, kProbeCap, false>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
-        hipLaunchKernelGGL((k_probe<SORTED, 1024, true>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        hipLaunchKernelGGL((k_probe<SORTED, kProbeCap>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        hipLaunchKernelGGL((k_probe_fix<SORTED>), dim3((n + 255) / 256), dim3(256), 0, s, Z, n, C, k, probes, pmax);
     } else {
-        hipLaunchKernelGGL((k_probe<SORTED, 1024, false>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        hipLaunchKernelGGL((k_probe<SORTED, 1024>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
     }
 }
 
