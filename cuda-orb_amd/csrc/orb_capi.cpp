@@ -171,6 +171,10 @@ struct orb_ctx {
     std::vector<uint32_t*> retired_partial;/* replaced workspaces still named by a graph */
     std::vector<FrameSet*> captured_sets;  /* during capture: sets the recorded calls use */
     bool captured_partial = false;
+    /* device address of the allocation-free matcher's scratch array on this
+       context's device (null if its lookup failed: scratch_err says why) */
+    uint32_t* scratch = nullptr;
+    std::string scratch_err;
     std::string err;
     /* stage timing: events recorded around launches while prof_mask != 0 */
     unsigned prof_mask = 0;
@@ -708,17 +712,37 @@ int ensure_partial(orb_ctx* c, size_t words, int nq)
     return ORB_OK;
 }
 
+/* noalloc: run on the device's scratch array instead of the context's
+   workspace, with no allocation, no memset and no stage events */
 int run_match(orb_ctx* c, const uint8_t* q, int nq, const int* d_nq, const uint8_t* db, int ndb, const int* d_ndb,
-              int method, int* idx1, int* d1, int* idx2, int* d2, orb::OrbPoint* pts)
+              int method, int* idx1, int* d1, int* idx2, int* d2, orb::OrbPoint* pts, bool noalloc = false)
 {
     if (nq < 0 || ndb < 0) return fail(c, ORB_E_ARG, "negative descriptor count");
     if (ndb >= (1 << 23)) return fail(c, ORB_E_CAPACITY, "database of %d entries exceeds 2^23-1", ndb);
+    if (noalloc && nq > kNoAllocMaxQueries)
+        return fail(c, ORB_E_CAPACITY, "%d queries exceed the allocation-free limit of %d", nq, kNoAllocMaxQueries);
     if (nq == 0) return ORB_OK;
     if (!q || (ndb > 0 && !db)) return fail(c, ORB_E_ARG, "null descriptor pointer");
     if (method != ORB_MATCH_POPCOUNT && method != ORB_MATCH_MFMA && method != ORB_MATCH_HAMMING2)
         return fail(c, ORB_E_ARG, "unknown method %d", method);
     MatchLaunch m;
     memset(&m, 0, sizeof(m));
+    if (noalloc) {
+        if (!c->scratch)
+            return fail(c, ORB_E_STATE, "no scratch array on device %d: %s", c->device, c->scratch_err.c_str());
+        m.q = q;
+        m.nq = nq;
+        m.db = db;
+        m.ndb = ndb;
+        m.idx1 = idx1;
+        m.d1 = d1;
+        m.idx2 = idx2;
+        m.d2 = d2;
+        m.hamming2 = method == ORB_MATCH_HAMMING2;
+        const hipError_t e = launch_match_noalloc(m, method, c->scratch, c->stream);
+        if (e != hipSuccess) return hip_fail(c, e, "allocation-free match");
+        return ORB_OK;
+    }
     const size_t words = match_partial_words(nq, ndb, method, &m.nchunks, &m.chunk);
     int rc = ensure_partial(c, words, nq);
     if (rc != ORB_OK) return rc;
@@ -783,6 +807,10 @@ int orb_ctx_create(orb_ctx** out, int device)
     if (e != hipSuccess) return fail(nullptr, ORB_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
     orb_ctx* c = new orb_ctx();
     c->device = device;
+    /* the allocation-free matcher's scratch address on this device, looked
+       up once per device with the device current */
+    e = match_scratch_setup(device, &c->scratch);
+    if (e != hipSuccess) c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->cu_count = cus;
@@ -1020,6 +1048,13 @@ int orb_match_top2(orb_ctx* c, const uint8_t* q, int nq, const uint8_t* db, int 
 {
     if (!c) return fail(nullptr, ORB_E_ARG, "null context");
     return run_match(c, q, nq, nullptr, db, ndb, nullptr, method, idx1, d1, idx2, d2, nullptr);
+}
+
+int orb_match_top2_noalloc(orb_ctx* c, const uint8_t* q, int nq, const uint8_t* db, int ndb, int method, int* idx1,
+                           int* d1, int* idx2, int* d2)
+{
+    if (!c) return fail(nullptr, ORB_E_ARG, "null context");
+    return run_match(c, q, nq, nullptr, db, ndb, nullptr, method, idx1, d1, idx2, d2, nullptr, true);
 }
 
 int orb_export_keypoints(orb_ctx* c, const orb::OrbPoint* pts, const uint8_t* desc, const int* d_n, int cap,

@@ -168,6 +168,19 @@ hipError_t launch_match_pairs(const MatchPair* d_pairs, int npairs, int cap1, in
                               unsigned long long* state, int* done, hipStream_t s, const StageEvents* ev);
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
 
+/* Allocation-free matching (orb_match_top2_noalloc): a file-scope device
+   array in orb_match.hip holds the matcher workspace for up to
+   kNoAllocMaxQueries queries.  match_scratch_setup (device current) looks up
+   that array's address on `device` once and returns it; later calls return
+   the cached address.  launch_match_noalloc runs orb_match_top2's kernels on
+   it (method 0/2 popcount, 1 MFMA), resetting what must start cleared with a
+   kernel, and allocates nothing. */
+constexpr int kNoAllocMaxQueries = ORB_NOALLOC_MAX_QUERIES;
+constexpr int kMaxScratchDevices = 64;
+size_t match_scratch_bytes();
+hipError_t match_scratch_setup(int device, uint32_t** ptr);
+hipError_t launch_match_noalloc(MatchLaunch m, int method, uint32_t* scratch, hipStream_t s);
+
 /* Launch one kernel of stage st.  When the stage is timed, its events go with
    the dispatch itself (hipExtLaunchKernel marks the kernel's own start and
    end), so timing adds no packets to the stream and no bubble between

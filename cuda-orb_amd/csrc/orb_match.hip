@@ -25,6 +25,8 @@
  *     during the current one; a lane builds keys only for results that can
  *     still enter its top-2.
  */
+#include <mutex>
+
 #include "orb_internal.h"
 #include "orb_match_common.h"
 
@@ -456,6 +458,81 @@ hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEve
                  slots, (uint2*)m.partial);
     launch_merge(m, kMfmaQ, slots, s, ev);
     return hipGetLastError();
+}
+
+/* ------------------------------------------- allocation-free matching */
+
+/* Scratch of orb_match_top2_noalloc, for callers that cannot allocate: one
+   array per GPU for the life of the process, laid out as
+     [merge words: 2 per query | tile counters: 1 per 64 queries | MFMA
+      per-chunk top-2 keys],
+   sized for kNoAllocMaxQueries queries.  In host code `scratch` names a
+   host-side stand-in: kernels get the device address that
+   match_scratch_setup obtains from hipGetSymbolAddress, once per device. */
+constexpr size_t kScratchStateWords = 2 * (size_t)kNoAllocMaxQueries;
+constexpr size_t kScratchDoneWords = (size_t)kNoAllocMaxQueries / kPopQ;
+constexpr size_t kScratchPartialOff = (kScratchStateWords + kScratchDoneWords + 63) & ~(size_t)63;
+/* MFMA partials: (slots + qunits) * qtile query entries of 2 words (match_partial_words) */
+constexpr size_t kScratchPartialWords =
+    2 * ((size_t)kMfmaBlocks + (size_t)kNoAllocMaxQueries / kMfmaQ) * (size_t)kMfmaQ;
+constexpr size_t kScratchWords = kScratchPartialOff + kScratchPartialWords;
+static_assert(kNoAllocMaxQueries % kMfmaQ == 0, "whole MFMA query tiles");
+
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+
+namespace
+{
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[kMaxScratchDevices] = {};
+}
+
+size_t match_scratch_bytes()
+{
+    return kScratchWords * sizeof(uint32_t);
+}
+
+hipError_t match_scratch_setup(int device, uint32_t** ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[device] = (uint32_t*)p;
+    }
+    *ptr = g_scratch_dev[device];
+    return hipSuccess;
+}
+
+/* the merge words of nq queries back to "no key" and their tile counters to
+   0 (the state k_match_pop expects; a kernel rather than a memset so the call
+   can be recorded into a graph) */
+__global__ __launch_bounds__(256) void k_scratch_reset(uint32_t* __restrict__ state, int nwords,
+                                                       uint32_t* __restrict__ done, int ndone)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < nwords; i += gridDim.x * 256) state[i] = kNoKey;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < ndone; i += gridDim.x * 256) done[i] = 0u;
+}
+
+hipError_t launch_match_noalloc(MatchLaunch m, int method, uint32_t* scr, hipStream_t s)
+{
+    (void)hipGetLastError(); /* report only errors of these launches */
+    if (m.nq <= 0) return hipSuccess;
+    if (m.nq > kNoAllocMaxQueries || !scr) return hipErrorInvalidValue;
+    m.state = (unsigned long long*)scr;
+    m.done = (int*)(scr + kScratchStateWords);
+    m.partial = scr + kScratchPartialOff;
+    if (method == 1) return launch_match_mfma(m, s, nullptr);
+    const int nwords = 2 * m.nq, ndone = (m.nq + kPopQ - 1) / kPopQ;
+    int blocks = (nwords + 255) / 256;
+    if (blocks > 512) blocks = 512;
+    hipLaunchKernelGGL(k_scratch_reset, dim3(blocks), dim3(256), 0, s, scr, nwords, (uint32_t*)m.done, ndone);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_match_popcount(m, s, nullptr);
 }
 
 } // namespace orbk

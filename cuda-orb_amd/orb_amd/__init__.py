@@ -20,6 +20,7 @@ ORB_OK = 0
 ORB_MATCH_POPCOUNT = 0
 ORB_MATCH_MFMA = 1
 ORB_MATCH_HAMMING2 = 2
+NOALLOC_MAX_QUERIES = 524288   # ORB_NOALLOC_MAX_QUERIES of orb_capi.h
 HARRIS_SCORE = 0
 FAST_SCORE = 1
 
@@ -36,7 +37,7 @@ assert CV_KEYPOINT_DTYPE.itemsize == 28
 EXPORTS = [
     "orb_ctx_create", "orb_ctx_destroy", "orb_set_stream", "orb_init", "orb_num_levels",
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
-    "orb_match_top2", "orb_match_batch", "orb_export_keypoints", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
+    "orb_match_top2", "orb_match_top2_noalloc", "orb_match_batch", "orb_export_keypoints", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
     "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
     "orb_tune_set", "orb_tune_get", "orb_debug_pyramid", "orb_capture_begin", "orb_capture_end",
     "orb_graph_launch", "orb_graph_destroy",
@@ -92,6 +93,7 @@ def lib():
         L.orb_match.argtypes = [P, V, V, I, V, I]
         L.orb_match_dcount.argtypes = [P, V, V, V, I, V, V, I]
         L.orb_match_top2.argtypes = [P, V, I, V, I, I, V, V, V, V]
+        L.orb_match_top2_noalloc.argtypes = [P, V, I, V, I, I, V, V, V, V]
         L.orb_export_keypoints.argtypes = [P, V, V, V, I, V, V, V]
         L.orb_match_batch.argtypes = [P, I, V, V, V, I, V, V, I]
         L.orb_sync.argtypes = [P]
@@ -270,6 +272,14 @@ class Context:
             out = [torch.empty(nq, dtype=torch.int32, device=q.device) for _ in range(4)]
         self._check(lib().orb_match_top2(self.h, _dptr(q), nq, _dptr(db), db.shape[0], method,
                                          *[_dptr(o) for o in out]), "orb_match_top2")
+        return out
+
+    def match_top2_noalloc(self, q, db, out, method=ORB_MATCH_POPCOUNT):
+        """orb_match_top2_noalloc into the four int32 device tensors of out (idx1, d1, idx2, d2):
+        the library's fixed scratch array instead of an allocated workspace."""
+        self._bind_stream()
+        self._check(lib().orb_match_top2_noalloc(self.h, _dptr(q), q.shape[0], _dptr(db), db.shape[0], method,
+                                                 *[_dptr(o) for o in out]), "orb_match_top2_noalloc")
         return out
 
     def export_keypoints(self, pts, desc=None, count=None, kp=None, desc_out=None, nout=None):

@@ -128,6 +128,23 @@ int orb_match_batch(orb_ctx* ctx, int npairs, ORB_POINT* const* d_pts1, const ui
 int orb_match_top2(orb_ctx* ctx, const uint8_t* d_query, int nq, const uint8_t* d_db, int ndb,
                    int method, int* d_idx1, int* d_d1, int* d_idx2, int* d_d2);
 
+/* Largest nq orb_match_top2_noalloc accepts. */
+#define ORB_NOALLOC_MAX_QUERIES 524288
+
+/* orb_match_top2 without any allocation.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+   arguments, results and methods as orb_match_top2, but the matcher
+   workspace is a fixed device array of the library (about 10 MiB per GPU,
+   held until the process ends) instead of the context's growing workspace:
+   nq above ORB_NOALLOC_MAX_QUERIES is rejected with ORB_E_CAPACITY, and no
+   hipMalloc, hipFree or memset is issued (what must start cleared is reset by
+   a kernel).  Every call on one GPU shares that array, from any context:
+   calls that overlap on one GPU (different streams or threads) race and give
+   wrong results; serialise them.  Asynchronous on the context's stream. */
+int orb_match_top2_noalloc(orb_ctx* ctx, const uint8_t* d_query, int nq, const uint8_t* d_db, int ndb,
+                           int method, int* d_idx1, int* d_d1, int* d_idx2, int* d_d2);
+
 /* A keypoint in the memory layout of OpenCV's cv::KeyPoint (28 bytes). */
 typedef struct {
     float x, y;     /* pt: level coordinate x layerScale, layerScale = (float)pow(scale_factor, octave) */
