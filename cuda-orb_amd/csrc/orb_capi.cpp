@@ -238,9 +238,47 @@ void make_gauss(float k[4])
     for (int i = 0; i <= 3; i++) k[i] *= sum;
 }
 
+/* The additive-feedback generator of glibc's random(3) in its default
+   TYPE_3 form, which the reference's srand/rand calls produce
+   (orbd.cu:380-392): 31 words seeded by the Park-Miller step
+   x -> 16807 x mod (2^31 - 1), then w[n] = w[n-31] + w[n-3] (mod 2^32), the
+   first 310 of those discarded, output w[n] >> 1.  The state lives in the
+   object, so callers' own rand() streams and other threads are untouched. */
+class Type3Rand
+{
+public:
+    /* valid for 0 < seed < 2^31 - 1, where the Park-Miller step is the plain
+       modular product (the only seed used here is 0x12345678) */
+    explicit Type3Rand(uint32_t seed)
+    {
+        uint64_t x = seed;
+        hist[0] = (uint32_t)x;
+        for (int i = 1; i < 31; i++) {
+            x = (16807u * x) % 2147483647u;
+            hist[i] = (uint32_t)x;
+        }
+        for (int i = 31; i < 34; i++) hist[i] = hist[i - 31];   /* w[31..33] copy w[0..2] */
+        n = 34;
+        for (int i = 0; i < 310; i++) step();
+    }
+    int next() { return (int)(step() >> 1); }
+
+private:
+    static constexpr int kLag = 34;   /* holds w[n-34 .. n-1] */
+    uint32_t hist[kLag];
+    long long n;
+    uint32_t step()
+    {
+        const uint32_t v = hist[(n - 31) % kLag] + hist[(n - 3) % kLag];
+        hist[n % kLag] = v;
+        n++;
+        return v;
+    }
+};
+
 /* setPattern for patch 31, orbd.cu:95-410: the table verbatim for WTA_K 2;
-   for WTA_K 3/4, 128 tuples of distinct points drawn with glibc
-   srand(0x12345678)/rand(), as the reference does. */
+   for WTA_K 3/4, 128 tuples of distinct table points drawn as the reference
+   draws them with srand(0x12345678)/rand(). */
 void make_pattern(int wta_k, int32_t* xy)
 {
     if (wta_k == 2) {
@@ -248,22 +286,17 @@ void make_pattern(int wta_k, int32_t* xy)
         return;
     }
     memset(xy, 0, 1024 * sizeof(int32_t));
-    srand(0x12345678);
-    for (int i = 0; i < 128; i++) {
+    Type3Rand rng(0x12345678u);
+    for (int t = 0; t < 128; t++) {
+        int2* tuple = (int2*)xy + wta_k * t;
         for (int k = 0; k < wta_k; k++) {
-            for (;;) {
-                const int idx = rand() % 512;
-                const int px = k_pattern31[2 * idx], py = k_pattern31[2 * idx + 1];
-                int k1;
-                for (k1 = 0; k1 < k; k1++) {
-                    const int q = wta_k * i + k1;
-                    if (px == xy[2 * q] && py == xy[2 * q + 1]) break;
-                }
-                if (k1 == k) {
-                    xy[2 * (wta_k * i + k)] = px;
-                    xy[2 * (wta_k * i + k) + 1] = py;
-                    break;
-                }
+            bool fresh = false;
+            while (!fresh) {
+                const int idx = rng.next() % 512;
+                const int2 cand = make_int2(k_pattern31[2 * idx], k_pattern31[2 * idx + 1]);
+                fresh = true;
+                for (int e = 0; e < k && fresh; e++) fresh = !(tuple[e].x == cand.x && tuple[e].y == cand.y);
+                if (fresh) tuple[k] = cand;
             }
         }
     }
@@ -1148,6 +1181,13 @@ int orb_tune_get(orb_ctx* c, int stage, int* variant, int* nvariants, float* ms)
     *variant = c->pyr_forced >= 0 ? c->pyr_forced : (c->fs ? c->fs->pyr_choice : -1);
     if (ms)
         for (int v = 0; v < kPyrVariants; v++) ms[v] = c->fs ? c->fs->pyr_ms[v] : 0.f;
+    return ORB_OK;
+}
+
+int orb_debug_pattern(int wta_k, int32_t* xy)
+{
+    if (!xy || wta_k < 2 || wta_k > 4) return fail(nullptr, ORB_E_ARG, "orb_debug_pattern: wta_k 2..4 and a buffer");
+    make_pattern(wta_k, xy);
     return ORB_OK;
 }
 

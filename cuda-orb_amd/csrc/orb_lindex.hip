@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                 d1 = qp[1];
                 bd = __hip_atomic_load(qbound + qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            query_operands(d0, d1, g, bq[qb]);
+            query_operands_neg(d0, d1, g, bq[qb]);
             k1[qb] = k2[qb] = bd <= 256u ? (bd << kIdxBits) | kIdxMask : kNoKey;
         }
         const int j0 = offsets[b], j1 = offsets[b + 1];
@@ -305,13 +305,15 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++)
                         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
-                    const int m = max(max(acc[0], acc[1]), max(acc[2], acc[3]));
-                    /* the key of the best dot with index 0 bounds the 4 keys
-                       from below: above k2, none can enter the top-2 */
-                    if (pm1_dot_key(m, 0u) <= k2[qb]) {
+                    /* acc holds -dot: the key of the smallest with index 0
+                       bounds the 4 keys from below: above k2, none can enter
+                       the top-2 */
+                    const int m = min(min(acc[0], acc[1]), min(acc[2], acc[3]));
+                    if (pm1_neg_dot_key(m, 0u) <= k2[qb]) {
 #pragma unroll
                         for (int r = 0; r < 4; r++)
-                            if (!PART || jb + r < j1) top2_push(k1[qb], k2[qb], pm1_dot_key(acc[r], (uint32_t)(jb + r)));
+                            if (!PART || jb + r < j1)
+                                top2_push(k1[qb], k2[qb], pm1_neg_dot_key(acc[r], (uint32_t)(jb + r)));
                     }
                 }
             }

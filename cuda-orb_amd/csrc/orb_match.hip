@@ -19,7 +19,8 @@
  *   MFMA: the distance is an exact integer contraction,
  *     popcount(a ^ b) = popcount(a) - <a', b01> with a' the +/-1 expansion
  *     of the query's bits and b01 the 0/1 expansion of the entry's, computed
- *     by v_mfma_i32_16x16x64_i8 (INT8 in, INT32 accumulate).  A workgroup
+ *     by v_mfma_i32_16x16x64_i8 (INT8 in, INT32 accumulate) with -a' as the
+ *     operand, so the accumulator holds -<a', b01> and a key is one shift-add.  A workgroup
  *     keeps 256 queries as +/-1 B operands in registers and unpacks each
  *     64-entry database tile into LDS once, the next tile's bytes loaded
  *     during the current one; a lane builds keys only for results that can
@@ -252,12 +253,12 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
     const int g = lane >> 4, col = lane & 15;
     for (int u = blockIdx.x; u < P.qunits * P.chunks; u += slots) {
         const int qu = u % P.qunits, c = u / P.qunits;
-        /* B operands: query (qb*16 + col) of this wave's 64, k-group g */
+        /* B operands: query (qb*16 + col) of this wave's 64, k-group g, negated */
         v4i bq[4][4];
-        /* thr[qb]: a result can enter the top-2 only if its dot reaches thr =
-           popcount(query) - (k2 >> 23), the current second distance as a dot;
-           "no key" admits every result until two are held */
-        int thr[4];
+        /* bnd[qb]: a result can enter the top-2 only if its -dot is at most
+           bnd = (k2 >> 23) - popcount(query), the current second distance
+           as a -dot; "no key" admits every result until two are held */
+        int bnd[4];
         int qidx[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
@@ -269,8 +270,8 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
                 d0 = qp[0];
                 d1 = qp[1];
             }
-            query_operands(d0, d1, g, bq[qb]);
-            thr[qb] = dot_threshold(popc256(d0, d1), kNoKey);
+            query_operands_neg(d0, d1, g, bq[qb]);
+            bnd[qb] = neg_dot_bound(popc256(d0, d1), kNoKey);
         }
         uint32_t k1[4], k2[4];
 #pragma unroll
@@ -313,19 +314,17 @@ __global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ 
 #pragma unroll
                     for (int ks = 0; ks < 4; ks++)
                         acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
-                    /* a dot below thr means a distance above the second one,
+                    /* a -dot above bnd means a distance above the second one,
                        a key above k2, for all 4 entries */
-                    const int m = max(max(acc[0], acc[1]), max(acc[2], acc[3]));
-                    if (m >= thr[qb]) {
-                        /* thr + (k2 >> 23) is the query's popcount */
-                        const int qpop = thr[qb] + (int)(k2[qb] >> 23);
-                        const uint32_t base = (uint32_t)qpop << 23;
+                    const int m = min(min(acc[0], acc[1]), min(acc[2], acc[3]));
+                    if (m <= bnd[qb]) {
+                        /* (k2 >> 23) - bnd is the query's popcount */
+                        const int qpop = (int)(k2[qb] >> 23) - bnd[qb];
+                        const uint32_t base = ((uint32_t)qpop << 23) + (uint32_t)jb;
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int j = jb + r;
-                            if (j < j1) top2_push(k1[qb], k2[qb], dot_key(acc[r], base, (uint32_t)j));
-                        }
-                        thr[qb] = dot_threshold(qpop, k2[qb]);
+                        for (int r = 0; r < 4; r++)
+                            if (jb + r < j1) top2_push(k1[qb], k2[qb], neg_dot_key(acc[r], base + (uint32_t)r));
+                        bnd[qb] = neg_dot_bound(qpop, k2[qb]);
                     }
                 }
             }

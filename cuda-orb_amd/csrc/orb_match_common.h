@@ -8,7 +8,9 @@
  * expand the query's bits to +/-1 bytes and the database entry's either to
  * 0/1 bytes (dot = (both set) - (entry set, query clear), distance =
  * popcount(query) - dot: fewer unpack instructions) or to +/-1 bytes
- * (distance = (256 - dot) / 2: no per-query term, fewer registers).
+ * (distance = (256 - dot) / 2: no per-query term, fewer registers).  The
+ * query operand is negated, so the accumulators hold -dot and a key is a
+ * single shift-add.
  */
 #pragma once
 
@@ -24,25 +26,14 @@ constexpr uint32_t kNoKey = 0xffffffffu;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
+/* k2 = med3(k1, key, k2) (k1 <= k2 always holds) and k1 = min(k1, key).
+   Written as min/max so the compiler selects v_med3_u32 and v_min_u32
+   itself and places any wait states the MFMA results they read need. */
 __device__ __forceinline__ void top2_push(uint32_t& k1, uint32_t& k2, uint32_t key)
 {
-    /* k2 = med3(k1, key, k2) since k1 <= k2: one instruction */
-    uint32_t m;
-    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(m) : "v"(k1), "v"(key), "v"(k2));
-    k2 = m;
-    k1 = min(k1, key);
-}
-
-/* a * b + c for a, b in the signed 24-bit range: one instruction.  The
-   compiler does not insert MFMA-result wait states before inline asm, so a
-   must not come straight from an MFMA: in the matchers the block test (plain
-   max instructions, waited on) reads the accumulators first and the keys are
-   built inside the branch it guards. */
-__device__ __forceinline__ uint32_t mad_i24(int a, int b, uint32_t c)
-{
-    uint32_t r;
-    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
-    return r;
+    const uint32_t lo = min(k1, key), hi = max(k1, key);
+    k2 = max(lo, min(hi, k2));
+    k1 = lo;
 }
 
 /* 16 bits -> 16 bytes of 1 (bit set) / 0 (bit clear) */
@@ -76,35 +67,41 @@ __device__ __forceinline__ v4i pm1_16(uint32_t h)
 }
 
 /* B operands of one query for the four 64-deep K steps of a 256-bit
-   descriptor: lane k-group g takes bits 64*ks + 16*g .. +15 */
-__device__ __forceinline__ void query_operands(uint4 d0, uint4 d1, int g, v4i* bq)
+   descriptor, negated: lane k-group g takes bits 64*ks + 16*g .. +15 and
+   expands their complement (-1 where the bit is set, +1 where clear), so the
+   MFMA accumulates -dot and a key is one shift-add of the accumulator */
+__device__ __forceinline__ void query_operands_neg(uint4 d0, uint4 d1, int g, v4i* bq)
 {
     const uint32_t w[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
         const int hw = 4 * ks + g;
-        bq[ks] = pm1_16((w[hw >> 1] >> (16 * (hw & 1))) & 0xffffu);
+        bq[ks] = pm1_16(~(w[hw >> 1] >> (16 * (hw & 1))) & 0xffffu);
     }
 }
 
-/* key of a (+/-1 query, 0/1 entry) dot product: (popcount(query) - dot) << 23 | idx,
-   from base = popcount(query) << 23 */
-__device__ __forceinline__ uint32_t dot_key(int dot, uint32_t base, uint32_t idx)
+/* key of a (+/-1 query, 0/1 entry) pair from the accumulated -dot:
+   (popcount(query) - dot) << 23 | idx = (-dot << 23) + base_idx with
+   base_idx = popcount(query) << 23 | idx (one v_lshl_add_u32) */
+__device__ __forceinline__ uint32_t neg_dot_key(int negdot, uint32_t base_idx)
 {
-    return mad_i24(dot, -(1 << 23), base + idx); /* |dot| <= 256 */
+    return ((uint32_t)negdot << kIdxBits) + base_idx;
 }
 
-/* key of a (+/-1 query, +/-1 entry) dot product: ((256 - dot) / 2) << 23 | idx */
-__device__ __forceinline__ uint32_t pm1_dot_key(int dot, uint32_t idx)
+/* key of a (+/-1 query, +/-1 entry) pair from the accumulated -dot: the
+   distance is (256 - dot) / 2 (dot is even), so the key is
+   (-dot << 22) + (128 << 23) + idx modulo 2^32 */
+__device__ __forceinline__ uint32_t pm1_neg_dot_key(int negdot, uint32_t idx)
 {
-    return mad_i24(dot, -(1 << 22), (256u << 22) + idx); /* |dot| <= 256 */
+    return ((uint32_t)negdot << (kIdxBits - 1)) + ((128u << kIdxBits) + idx);
 }
 
-/* the smallest dot whose key can still be below k2: the distance must not
-   exceed k2's (k2 = kNoKey admits every entry) */
-__device__ __forceinline__ int dot_threshold(int qpop, uint32_t k2)
+/* For the 0/1-entry matcher: the largest -dot whose key can still be at or
+   below k2 (the distance must not exceed k2's; k2 = kNoKey admits every
+   entry since -dot <= 256 - popcount(query)) */
+__device__ __forceinline__ int neg_dot_bound(int qpop, uint32_t k2)
 {
-    return qpop - (int)(k2 >> 23);
+    return (int)(k2 >> kIdxBits) - qpop;
 }
 
 __device__ __forceinline__ int popc256(uint4 d0, uint4 d1)

@@ -38,7 +38,7 @@ EXPORTS = [
     "orb_ctx_create", "orb_ctx_destroy", "orb_set_stream", "orb_init", "orb_num_levels",
     "orb_detect_and_compute", "orb_detect_and_compute_batch", "orb_match", "orb_match_dcount",
     "orb_match_top2", "orb_match_top2_noalloc", "orb_match_batch", "orb_export_keypoints", "orb_sync", "orb_last_error", "orb_workspace_bytes", "orb_synth_image",
-    "orb_synth_warp", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
+    "orb_synth_warp", "orb_debug_pattern", "orb_debug_sincos", "orb_debug_atan2", "orb_profile_enable", "orb_profile_read",
     "orb_tune_set", "orb_tune_get", "orb_debug_pyramid", "orb_capture_begin", "orb_capture_end",
     "orb_graph_launch", "orb_graph_destroy",
     "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
@@ -103,6 +103,7 @@ def lib():
         L.orb_workspace_bytes.restype = ctypes.c_size_t
         L.orb_synth_image.argtypes = [ctypes.c_uint64, I, I, I, I, V]
         L.orb_synth_warp.argtypes = [ctypes.c_uint64, V, I, I, I, V, V]
+        L.orb_debug_pattern.argtypes = [I, V]
         L.orb_debug_sincos.argtypes = [P, V, I, V, V]
         L.orb_debug_atan2.argtypes = [P, V, I, V]
         L.orb_profile_enable.argtypes = [P, ctypes.c_uint]

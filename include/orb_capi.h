@@ -307,6 +307,10 @@ int orb_graph_destroy(orb_graph* graph);
 int orb_tune_set(orb_ctx* ctx, int stage, int variant);
 int orb_tune_get(orb_ctx* ctx, int stage, int* variant, int* nvariants, float* ms);
 
+/* For tests, host only (no GPU): the 512 sampling points (x, y pairs) that
+   orb_init uploads for wta_k 2, 3 or 4 (setPattern, orbd.cu:95-410). */
+int orb_debug_pattern(int wta_k, int32_t* xy);
+
 /* Device self-checks of the math helpers, for tests: n angles in d_in give
    sin, cos (own polynomials) and n (y, x) pairs give the centroid atan2. */
 int orb_debug_sincos(orb_ctx* ctx, const float* d_in, int n, float* d_sin, float* d_cos);

@@ -1,99 +1,109 @@
 /*
- * HIP-native counterparts of the helpers users of the reference's orb.h
- * reach through cuda_utils.h (main.cpp:170,200,202,209-216): CHECK /
- * CheckMsg (print + exit, cuda_utils.h:10-37), initDevice (:41-67),
- * GpuTimer (:81-108) and iAlignUp (:170-173).
+ * Host helpers that users of the reference's orb.h reach through its utility
+ * header (cuda_utils.h:10-108,170-173; used by main.cpp:170,200,202,209-216):
+ * CHECK / CheckMsg, initDevice, GpuTimer and iAlignUp.  The names and call
+ * signatures are the reference's, so its demo code compiles unchanged against
+ * HIP; the implementations are this library's.
  */
 #pragma once
 
 #include <hip/hip_runtime.h>
 
-#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
 #define H_PI 1.5707963267948966f
 
-#define CHECK(err) __check(err, __FILE__, __LINE__)
-#define CheckMsg(msg) __checkMsg(msg, __FILE__, __LINE__)
-
-/* Print a HIP runtime error and exit (cuda_utils.h:18-25). */
-inline void __check(hipError_t err, const char* file, const int line)
+namespace orb_util
 {
-	if (err != hipSuccess)
-	{
-		fprintf(stderr, "CHECK() Runtime API error in file <%s>, line %i : %s.\n", file, line, hipGetErrorString(err));
-		exit(-1);
-	}
+
+/* Report a failed HIP call with the source position and terminate, the error
+   behaviour the reference's API promises its callers. */
+[[noreturn]] inline void die(const char* what, hipError_t err, const char* file, int line)
+{
+    std::fprintf(stderr, "orb: %s failed at %s:%d: %s (%s)\n", what, file, line, hipGetErrorName(err),
+                 hipGetErrorString(err));
+    std::exit(-1);
 }
 
-/* Print the last HIP error with a message and exit (cuda_utils.h:29-37). */
-inline void __checkMsg(const char* msg, const char* file, const int line)
+inline void check_call(hipError_t err, const char* expr, const char* file, int line)
 {
-	hipError_t err = hipGetLastError();
-	if (err != hipSuccess)
-	{
-		fprintf(stderr, "CheckMsg() HIP error: %s in file <%s>, line %i : %s.\n", msg, file, line, hipGetErrorString(err));
-		exit(-1);
-	}
+    if (err == hipSuccess) return;
+    die(expr, err, file, line);
 }
 
-/* Select device `dev` (clamped to the available range) and print its name
-   (cuda_utils.h:41-67). */
+inline void check_last(const char* msg, const char* file, int line)
+{
+    const hipError_t err = hipGetLastError();
+    if (err == hipSuccess) return;
+    die(msg, err, file, line);
+}
+
+} // namespace orb_util
+
+/* CHECK(call): exit with a message when a HIP runtime call fails. */
+#define CHECK(err) orb_util::check_call((err), #err, __FILE__, __LINE__)
+/* CheckMsg(msg): exit with `msg` when a launch or call left an error pending. */
+#define CheckMsg(msg) orb_util::check_last((msg), __FILE__, __LINE__)
+
+/* Make device `dev` current, clamped into [0, device count), and print which
+   device that is; false when no HIP device exists. */
 inline bool initDevice(int dev)
 {
-	int device_count = 0;
-	CHECK(hipGetDeviceCount(&device_count));
-	if (device_count == 0)
-	{
-		fprintf(stderr, "HIP error: no devices.\n");
-		return false;
-	}
-	dev = std::max<int>(0, std::min<int>(dev, device_count - 1));
-	hipDeviceProp_t prop;
-	CHECK(hipGetDeviceProperties(&prop, dev));
-	CHECK(hipSetDevice(dev));
-	int driver_version = 0, runtime_version = 0;
-	CHECK(hipDriverGetVersion(&driver_version));
-	CHECK(hipRuntimeGetVersion(&runtime_version));
-	fprintf(stderr, "Using Device %d: %s (%s), driver %d, runtime %d\n", dev, prop.name, prop.gcnArchName,
-		driver_version, runtime_version);
-	return true;
+    int count = 0;
+    CHECK(hipGetDeviceCount(&count));
+    if (count < 1) {
+        std::fprintf(stderr, "orb: no HIP device found\n");
+        return false;
+    }
+    const int chosen = dev < 0 ? 0 : (dev >= count ? count - 1 : dev);
+    CHECK(hipSetDevice(chosen));
+    hipDeviceProp_t props;
+    CHECK(hipGetDeviceProperties(&props, chosen));
+    int rt = 0;
+    CHECK(hipRuntimeGetVersion(&rt));
+    std::fprintf(stderr, "orb: device %d of %d, %s [%s], %d CUs, HIP runtime %d\n", chosen, count, props.name,
+                 props.gcnArchName, props.multiProcessorCount, rt);
+    return true;
 }
 
-/* Event timer on a stream, milliseconds since construction (cuda_utils.h:81-108). */
+/* Milliseconds of stream work since construction: read() marks the end on the
+   stream, waits for it and returns the time between the two marks. */
 class GpuTimer
 {
 public:
-	GpuTimer(hipStream_t stream_ = 0) : stream(stream_)
-	{
-		(void)hipEventCreate(&start);
-		(void)hipEventCreate(&stop);
-		(void)hipEventRecord(start, stream);
-	}
+    explicit GpuTimer(hipStream_t stream_ = 0) : stream(stream_)
+    {
+        CHECK(hipEventCreate(&marks[0]));
+        CHECK(hipEventCreate(&marks[1]));
+        CHECK(hipEventRecord(marks[0], stream));
+    }
+    GpuTimer(const GpuTimer&) = delete;
+    GpuTimer& operator=(const GpuTimer&) = delete;
+    ~GpuTimer()
+    {
+        (void)hipEventDestroy(marks[1]);
+        (void)hipEventDestroy(marks[0]);
+    }
 
-	~GpuTimer()
-	{
-		(void)hipEventDestroy(start);
-		(void)hipEventDestroy(stop);
-	}
-
-	float read()
-	{
-		(void)hipEventRecord(stop, stream);
-		(void)hipEventSynchronize(stop);
-		float time = 0.f;
-		(void)hipEventElapsedTime(&time, start, stop);
-		return time;
-	}
+    float read()
+    {
+        CHECK(hipEventRecord(marks[1], stream));
+        CHECK(hipEventSynchronize(marks[1]));
+        float ms = 0.f;
+        CHECK(hipEventElapsedTime(&ms, marks[0], marks[1]));
+        return ms;
+    }
 
 private:
-	hipEvent_t start, stop;
-	hipStream_t stream;
+    hipEvent_t marks[2] = {nullptr, nullptr};
+    hipStream_t stream;
 };
 
-/* Round a up to a multiple of b (cuda_utils.h:170-173). */
+/* The smallest multiple of b at or above a (for a >= 0; a negative a moves
+   toward zero, as the reference's helper does). */
 inline int iAlignUp(const int a, const int b)
 {
-	return (a % b != 0) ? (a - a % b + b) : a;
+    const int r = a % b;
+    return r == 0 ? a : a + (b - r);
 }

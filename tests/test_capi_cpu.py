@@ -82,3 +82,58 @@ def test_context_creation_without_gpu_fails_cleanly():
     rc = lib.orb_ctx_create(ctypes.byref(h), 0)
     assert rc != 0 and not h.value
     assert lib.orb_last_error(None)
+
+
+def _pattern(lib, wta_k):
+    import orb_amd
+    xy = np.zeros(1024, np.int32)
+    assert lib.orb_debug_pattern(wta_k, orb_amd._np_ptr(xy)) == 0
+    return xy.reshape(512, 2)
+
+
+def test_product_patterns_equal_the_oracle(oracle):
+    """The sampling points orb_init uploads (WTA_K 2: the reference table; 3/4: tuples drawn by the
+    library's own TYPE_3 generator) equal the oracle's, whose generator is pinned against libc."""
+    import orb_amd
+    lib = orb_amd.lib()
+    for wta_k in (2, 3, 4):
+        assert np.array_equal(_pattern(lib, wta_k), oracle.pattern(wta_k)), wta_k
+    assert lib.orb_debug_pattern(5, orb_amd._np_ptr(np.zeros(1024, np.int32))) == -1
+
+
+def test_pattern_leaves_the_callers_rand_stream_alone():
+    import orb_amd
+    lib = orb_amd.lib()
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(4242)
+    want = [libc.rand() for _ in range(6)]
+    libc.srand(4242)
+    got = [libc.rand(), libc.rand()]
+    _pattern(lib, 3)
+    got.append(libc.rand())
+    _pattern(lib, 4)
+    got += [libc.rand() for _ in range(3)]
+    assert got == want
+
+
+def test_concurrent_pattern_builds_agree(oracle):
+    """Eight threads build WTA_K 3 and 4 patterns at once (ctypes releases the GIL during the
+    call): each result equals the single-threaded one, so no generator state is shared."""
+    import threading
+
+    import orb_amd
+    lib = orb_amd.lib()
+    want = {k: oracle.pattern(k) for k in (3, 4)}
+    bad = []
+
+    def work(k):
+        for _ in range(50):
+            if not np.array_equal(_pattern(lib, k), want[k]):
+                bad.append(k)
+
+    threads = [threading.Thread(target=work, args=(3 + i % 2,)) for i in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not bad
