@@ -1,5 +1,11 @@
 """CPU oracle of the learned bucket matcher -- TEST INFRASTRUCTURE.
 
+Pinned by tests/test_mlp_oracle.py: the backward pass against torch autograd
+and central finite differences of the float64 network (rounding off), the
+bf16 helper against torch's bfloat16 conversion, gelu against torch's erf
+GELU, the loss against torch's cross-entropy, and the bucketed top-2 against
+brute force and the exact C matcher.
+
 Only tests/ and bench.py's cpu_baseline leg use this module, as the checker.
 It restates, in float64 numpy, the network and training step the HIP kernels
 compute (cuda-orb_amd/csrc/orb_mlp.hip), with the same bf16 rounding points,
@@ -51,14 +57,20 @@ def centroid_term(centroids, scale):
     return bits_pm1(centroids) * bf16(np.array([scale]))[0]
 
 
-def forward(params, desc, cg=None):
+def _identity(x):
+    return np.asarray(x, np.float64)
+
+
+def forward(params, desc, cg=None, rounding=True):
     """params: dict of float64 arrays W1 (256,H), b1, W2 (H,H), b2, W3 (H,C), b3 (bf16-valued weights);
-    cg: optional centroid term matrix from centroid_term()."""
+    cg: optional centroid term matrix from centroid_term().  rounding=False drops the bf16 rounding
+    of the activations (the exact float64 network, for checking the backward pass)."""
+    rnd = bf16 if rounding else _identity
     X = bits_pm1(desc)
     Z1 = X @ params["W1"] + params["b1"]
-    A1 = bf16(gelu(Z1))
+    A1 = rnd(gelu(Z1))
     Z2 = A1 @ params["W2"] + params["b2"]
-    A2 = bf16(gelu(Z2))
+    A2 = rnd(gelu(Z2))
     Z3 = A2 @ params["W3"]
     if cg is not None:
         Z3 = Z3 + X @ cg.T
@@ -66,8 +78,11 @@ def forward(params, desc, cg=None):
     return {"X": X, "Z1": Z1, "A1": A1, "Z2": Z2, "A2": A2, "Z3": Z3}
 
 
-def loss_and_grads(params, desc, labels, cg=None):
-    f = forward(params, desc, cg)
+def loss_and_grads(params, desc, labels, cg=None, rounding=True):
+    """Mean softmax cross-entropy of the bucket labels and its gradients with respect to every
+    parameter, rounding the backward products where the kernels do (rounding=False: none)."""
+    rnd = bf16 if rounding else _identity
+    f = forward(params, desc, cg, rounding)
     n = len(labels)
     Z3 = f["Z3"]
     m = Z3.max(1, keepdims=True)
@@ -76,14 +91,14 @@ def loss_and_grads(params, desc, labels, cg=None):
     loss = float(-np.log(P[np.arange(n), labels]).mean())
     G = P.copy()
     G[np.arange(n), labels] -= 1.0
-    dZ3 = bf16(G / n)
+    dZ3 = rnd(G / n)
     g = {}
     g["W3"] = f["A2"].T @ dZ3
     g["b3"] = dZ3.sum(0)
-    dZ2 = bf16((dZ3 @ params["W3"].T) * gelu_grad(f["Z2"]))
+    dZ2 = rnd((dZ3 @ params["W3"].T) * gelu_grad(f["Z2"]))
     g["W2"] = f["A1"].T @ dZ2
     g["b2"] = dZ2.sum(0)
-    dZ1 = bf16((dZ2 @ params["W2"].T) * gelu_grad(f["Z1"]))
+    dZ1 = rnd((dZ2 @ params["W2"].T) * gelu_grad(f["Z1"]))
     g["W1"] = f["X"].T @ dZ1
     g["b1"] = dZ1.sum(0)
     return loss, g, f
@@ -99,17 +114,19 @@ def probe(logits, nprobe=8):
 
 
 def bucket_search(q, db, bucket_of, buckets):
-    """Exact top-1 of each query restricted to the database entries whose bucket is in its probe
-    list; ties to the lowest database index.  Returns (idx, dist), -1/257 when the lists are empty."""
+    """Exact top-2 of each query restricted to the database entries whose bucket is in its probe
+    list, ties to the lowest database index.  Returns (idx1, d1, idx2, d2); a missing neighbour is
+    -1 / 257 (orb_match_top2's convention)."""
     qb = np.unpackbits(q, axis=1)
     dbb = np.unpackbits(db, axis=1)
-    idx = np.full(len(q), -1, np.int64)
-    dist = np.full(len(q), 257, np.int64)
+    out = [np.full(len(q), v, np.int64) for v in (-1, 257, -1, 257)]
     for i in range(len(q)):
         cand = np.nonzero(np.isin(bucket_of, buckets[i]))[0]
         if len(cand) == 0:
             continue
         d = (dbb[cand] != qb[i]).sum(1)
-        j = np.lexsort((cand, d))[0]
-        idx[i], dist[i] = cand[j], d[j]
-    return idx, dist
+        order = np.lexsort((cand, d))
+        out[0][i], out[1][i] = cand[order[0]], d[order[0]]
+        if len(cand) > 1:
+            out[2][i], out[3][i] = cand[order[1]], d[order[1]]
+    return tuple(out)

@@ -378,6 +378,83 @@ def test_nms_brute_force_small_images(oracle):
         assert nsurv == len(want)
 
 
+def _nms_brute(V, L, radii, border, w, h):
+    """Survivors of the reference's square NMS (orbd.cu:945-966) by direct enumeration: a point
+    (y, x) with score v > 0 on level l is dropped iff some (dx, dy) != 0 with |dx|, |dy| <= radii[l]
+    holds v' > 0 with v' > v, or v' == v and dx <= 0 and dy <= 0.  Rows outside the map read 0."""
+    hh, p = V.shape
+    out = []
+    ys, xs = np.nonzero(V > 0)
+    for y, x in sorted(zip(ys.tolist(), xs.tolist())):
+        if not (border <= y < h - border and border <= x < w - border):
+            continue
+        v, r = V[y, x], radii[L[y, x]]
+        sup = False
+        for dy in range(-r, r + 1):
+            yy = y + dy
+            if yy < 0 or yy >= hh:
+                continue
+            for dx in range(-r, r + 1):
+                if dx == 0 and dy == 0:
+                    continue
+                nv = V[yy, x + dx]
+                if nv > 0 and (nv > v or (nv == v and dx <= 0 and dy <= 0)):
+                    sup = True
+                    break
+            if sup:
+                break
+        if not sup:
+            out.append((x, y))
+    return out
+
+
+def test_nms_brute_force_scale12_eight_levels(oracle):
+    """Detection maps of 128x128 images over 8 levels at scale 1.2, so corners of every level
+    (radius rn(1.2^l): 1,1,1,2,2,2,3,4) compete in one level-0 map."""
+    rng = np.random.default_rng(16)
+    radii = [max(1, int(math.floor(1.2 ** l + 0.5))) for l in range(8)]
+    assert radii == [1, 1, 1, 2, 2, 2, 3, 4]
+    levels_seen = set()
+    for trial in range(4):
+        h = w = 128
+        blk = [4, 6, 8, 12][trial]
+        img = (rng.integers(0, 2, (h // blk + 1, w // blk + 1)) * 180 + 30).astype(np.uint8)
+        img = np.kron(img, np.ones((blk, blk), np.uint8))[:h, :w] + rng.integers(0, 12, (h, w)).astype(np.uint8)
+        g = oracle.geometry(w, h, 8, 1.2)
+        assert g.nlevels == 8 and list(g.radius[:8]) == radii
+        raw, _ = oracle.pyramid(g, oracle.pitched(img))
+        vmap, lmap = oracle.detect_map(g, raw, 20, 0)
+        V, L = vmap.reshape(h, g.p[0]), lmap.reshape(h, g.p[0])
+        levels_seen |= set(L[V > 0].tolist())
+        pts, nsurv = oracle.nms(g, vmap, lmap, border=8, max_pts=100000)
+        want = _nms_brute(V, L, radii, 8, w, h)
+        assert [(int(a), int(b)) for a, b in zip(pts["x"], pts["y"])] == want
+        assert nsurv == len(want)
+        assert list(pts["octave"]) == [int(L[y, x]) for x, y in want]
+    assert {0, 3, 5, 6, 7} <= levels_seen, levels_seen
+
+
+def test_nms_brute_force_scale2_five_octaves_with_ties(oracle):
+    """The reference's octave radii 2^l up to 16 (orbd.cu:941), on a synthetic level-0 map of a
+    1280x1280 frame (5 octaves) with few distinct scores, so ties at every offset occur."""
+    rng = np.random.default_rng(17)
+    g = oracle.geometry(1280, 1280, 5, 2.0)
+    radii = [1 << l for l in range(5)]
+    assert g.nlevels == 5 and list(g.radius[:5]) == radii
+    h, w, p = g.h[0], g.w[0], g.p[0]
+    for trial in range(3):
+        V = np.zeros((h, p), np.float32)
+        L = np.zeros((h, p), np.int32)
+        n = 6000
+        ys, xs = rng.integers(0, h, n), rng.integers(0, w, n)
+        V[ys, xs] = rng.integers(1, 5, n).astype(np.float32) * np.float32(0.5)
+        L[ys, xs] = rng.integers(0, 5, n)
+        pts, nsurv = oracle.nms(g, V.ravel(), L.ravel(), border=31, max_pts=100000)
+        want = _nms_brute(V, L, radii, 31, w, h)
+        assert [(int(a), int(b)) for a, b in zip(pts["x"], pts["y"])] == want
+        assert nsurv == len(want) > 100
+
+
 def test_nms_cap_keeps_best_scores_in_raster_order(oracle):
     img = oracle_synth(640, 480, 40, 11)
     pts_all, _, ns = oracle.detect_and_compute(img, max_pts=100000)
