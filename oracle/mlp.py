@@ -117,14 +117,19 @@ def bucket_search(q, db, bucket_of, buckets):
     """Exact top-2 of each query restricted to the database entries whose bucket is in its probe
     list, ties to the lowest database index.  Returns (idx1, d1, idx2, d2); a missing neighbour is
     -1 / 257 (orb_match_top2's convention)."""
-    qb = np.unpackbits(q, axis=1)
-    dbb = np.unpackbits(db, axis=1)
+    q64 = np.ascontiguousarray(q, np.uint8).view(np.uint64)
+    db64 = np.ascontiguousarray(db, np.uint8).view(np.uint64)
+    bucket_of = np.asarray(bucket_of)
+    nb = int(max(bucket_of.max() + 1 if len(bucket_of) else 0, np.max(buckets) + 1 if np.size(buckets) else 0))
+    members = np.argsort(bucket_of, kind="stable")          # entries of each bucket in index order
+    starts = np.concatenate([[0], np.cumsum(np.bincount(bucket_of, minlength=nb))])
     out = [np.full(len(q), v, np.int64) for v in (-1, 257, -1, 257)]
     for i in range(len(q)):
-        cand = np.nonzero(np.isin(bucket_of, buckets[i]))[0]
+        probed = np.unique(buckets[i])
+        cand = np.sort(np.concatenate([members[starts[b]:starts[b + 1]] for b in probed]))
         if len(cand) == 0:
             continue
-        d = (dbb[cand] != qb[i]).sum(1)
+        d = np.bitwise_count(db64[cand] ^ q64[i]).sum(1).astype(np.int64)
         order = np.lexsort((cand, d))
         out[0][i], out[1][i] = cand[order[0]], d[order[0]]
         if len(cand) > 1:
