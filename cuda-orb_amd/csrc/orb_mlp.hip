@@ -485,14 +485,36 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ Z,
     if (lane == 0 && loss) loss[r] = -(z[y] - m - __logf(s));
 }
 
-/* column sums of a bf16 (n x C) matrix into fp32 (bias gradients) */
-__global__ void k_colsum(const uint16_t* __restrict__ X, int n, int C, float* __restrict__ out)
+/* Column sums of a bf16 (n x C) matrix into fp32 (bias gradients).  One
+   1024-thread workgroup per 64 columns: thread (g, c) sums rows g, g+16, ...
+   of column c (every row read as one 128-byte segment by a wavefront, eight
+   rows in flight per thread), then the 16 partial sums of a column are added
+   in a fixed order, so the result does not depend on scheduling. */
+__global__ __launch_bounds__(1024) void k_colsum(const uint16_t* __restrict__ X, int n, int C, float* __restrict__ out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
+    __shared__ float s_part[16][64];
+    const int col = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + col;
     float s = 0.f;
-    for (int r = 0; r < n; r++) s += bf2f(X[(long long)r * C + c]);
-    out[c] = s;
+    if (c < C) {
+        int r = g;
+        for (; r + 7 * 16 < n; r += 8 * 16) {
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bf2f(X[(long long)(r + 16 * k) * C + c]);
+#pragma unroll
+            for (int k = 0; k < 8; k++) s += v[k];
+        }
+        for (; r < n; r += 16) s += bf2f(X[(long long)r * C + c]);
+    }
+    s_part[g][col] = s;
+    __syncthreads();
+    if (g == 0 && c < C) {
+        float t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; k++) t += s_part[k][col];
+        out[c] = t;
+    }
 }
 
 /* Adam on fp32 master weights; bf16 copies refreshed */
@@ -646,7 +668,7 @@ hipError_t launch_softmax_ce(const float* Z, const int* labels, int n, int C, ui
 
 hipError_t launch_colsum(const uint16_t* X, int n, int C, float* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_colsum, dim3((C + 255) / 256), dim3(256), 0, s, X, n, C, out);
+    hipLaunchKernelGGL(k_colsum, dim3((C + 63) / 64), dim3(1024), 0, s, X, n, C, out);
     return hipGetLastError();
 }
 
