@@ -281,8 +281,9 @@ def learned_match(ctx, torch, dev, log=None):
         lx.set_probe(npb, 0.0)
         (o, _), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
         (oc, _), msc = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_CENTROIDS), reps=2)
-        versus.append({"nprobe": npb, "network_recall_at_1": round(rec(o), 5), "network_ms": round(ms, 3),
-                       "centroid_recall_at_1": round(rec(oc), 5), "centroid_ms": round(msc, 3)})
+        versus.append({"nprobe": npb, "network_recall_at_1": round(rec(o), 5), "network_recall_idx": round(rec_idx(o), 5),
+                       "network_ms": round(ms, 3), "centroid_recall_at_1": round(rec(oc), 5),
+                       "centroid_recall_idx": round(rec_idx(oc), 5), "centroid_ms": round(msc, 3)})
     # the learned matcher as specified: top-8 probe, exact fallback below max softmax 0.3
     lx.set_probe(8, 0.3)
     (lq, nfb), ms_learned = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP))
@@ -292,11 +293,15 @@ def learned_match(ctx, torch, dev, log=None):
             lx.set_probe(npb, fb)
             (o, nf), ms = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=2)
             sweep.append({"nprobe": npb, "fallback": fb, "ms": round(ms, 3), "recall_at_1": round(rec(o), 5),
-                          "fallback_queries": int(nf), "speedup": round(ms_exact / ms, 3)})
-    ok = [r for r in sweep if r["recall_at_1"] >= 0.99]
+                          "recall_idx": round(rec_idx(o), 5), "fallback_queries": int(nf),
+                          "speedup": round(ms_exact / ms, 3)})
+    # the fastest settings whose nearest-neighbour index (not only its distance) matches exact search
+    # for >= 99% of the queries
+    ok = [r for r in sweep if r["recall_idx"] >= 0.99]
     best = min(ok, key=lambda r: r["ms"]) if ok else None
     ok_c = [{"nprobe": r["nprobe"], "ms": r["centroid_ms"], "recall_at_1": r["centroid_recall_at_1"],
-             "speedup": round(ms_exact / r["centroid_ms"], 3)} for r in versus if r["centroid_recall_at_1"] >= 0.99]
+             "recall_idx": r["centroid_recall_idx"], "speedup": round(ms_exact / r["centroid_ms"], 3)}
+            for r in versus if r["centroid_recall_idx"] >= 0.99]
     best_c = min(ok_c, key=lambda r: r["ms"]) if ok_c else None
     res = {
         "database": int(db.shape[0]), "queries": int(test_q.shape[0]), "train_queries": int(train_q.shape[0]),
@@ -317,23 +322,36 @@ def learned_match(ctx, torch, dev, log=None):
     return res
 
 
-def cpu_baseline(a, b, min_seconds=10.0):
-    """The scalar C oracle on the host: the same step (detect+describe both images, match them),
-    repeated until about min_seconds of CPU work."""
+def cpu_baseline(a, b, min_seconds=8.0):
+    """The C oracle on the host doing the same step (detect+describe both images, reference-rule match):
+    once on one thread and once on every core this process may use (OpenMP over pyramid and
+    detection rows, keypoints and match queries; identical outputs), each repeated for about
+    min_seconds.  `value` is the all-core rate."""
     from oracle import oracle as O
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        pa, da, _ = O.detect_and_compute(a[:, :W].copy(), noctaves=LEVELS, scale=SCALE, max_pts=CAP)
-        pb, db, _ = O.detect_and_compute(b[:, :W].copy(), noctaves=LEVELS, scale=SCALE, max_pts=CAP)
-        O.match(da, db)
-        reps += 1
-        dt = time.perf_counter() - t0
-        if dt >= min_seconds or reps >= 50:
-            break
-    return {"value": round(2.0 * reps / dt, 4), "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} repeats of the 1920x1080 pair step: oracle detect+describe of both images and "
-                      f"the reference-rule match ({len(pa)}x{len(pb)}), single thread, {dt:.1f} s"}
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+    def leg(threads):
+        O.set_threads(threads)
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            pa, da, _ = O.detect_and_compute(a[:, :W].copy(), noctaves=LEVELS, scale=SCALE, max_pts=CAP)
+            pb, db, _ = O.detect_and_compute(b[:, :W].copy(), noctaves=LEVELS, scale=SCALE, max_pts=CAP)
+            O.match(da, db)
+            reps += 1
+            dt = time.perf_counter() - t0
+            if dt >= min_seconds or reps >= 400:
+                return 2.0 * reps / dt, reps, dt, len(pa), len(pb)
+
+    try:
+        v1, r1, t1, na, nb = leg(1)
+        vn, rn, tn, _, _ = leg(cores)
+    finally:
+        O.set_threads(1)
+    return {"value": round(vn, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "single_thread_value": round(v1, 4),
+            "sample": f"the 1920x1080 pair step (oracle detect+describe of both images, reference-rule match "
+                      f"{na}x{nb}): {rn} repeats in {tn:.1f} s on {cores} threads, {r1} repeats in {t1:.1f} s on 1"}
 
 
 def main():

@@ -76,12 +76,18 @@ def lib():
         L.orc_umax.argtypes = [I, P]
         L.orc_gauss.argtypes = [P]
         L.orc_fast_lut.argtypes = [I, P]
+        L.orc_set_threads.argtypes = [I]
         _lib = L
     return _lib
 
 
 def ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (1 = scalar); outputs are identical for any n."""
+    lib().orc_set_threads(int(n))
 
 
 def align_up(a, b=128):

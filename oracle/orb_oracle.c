@@ -266,6 +266,20 @@ int orc_l0_to_level(const orc_geom* g, int level, int v)
     return (int)floor(v / g->sc[level] + 0.5);
 }
 
+/* threads of the parallel loops below (1: the plain scalar oracle).  Every
+   parallel loop writes disjoint outputs, so results do not depend on it. */
+static int g_threads = 1;
+
+void orc_set_threads(int n)
+{
+    g_threads = n < 1 ? 1 : n;
+}
+
+int orc_get_threads(void)
+{
+    return g_threads;
+}
+
 static inline int px(const uint8_t* buf, long total, long i)
 {
     return (i >= 0 && i < total) ? buf[i] : 0;
@@ -301,6 +315,7 @@ void orc_blur_level(const orc_geom* g, int l, const uint8_t* raw, uint8_t* blur)
     const uint8_t* src = raw + g->off[l];
     uint8_t* dst = blur + g->off[l];
     float* rows = (float*)malloc(sizeof(float) * (size_t)w * h);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int y = 0; y < h; y++) {
         const uint8_t* r = src + (long)y * p;
         for (int x = 0; x < w; x++) {
@@ -310,6 +325,7 @@ void orc_blur_level(const orc_geom* g, int l, const uint8_t* raw, uint8_t* blur)
             rows[(long)y * w + x] = s;
         }
     }
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int y = 0; y < h; y++) {
         for (int x = 0; x < w; x++) {
             float s = k[0] * rows[(long)y * w + x];
@@ -333,6 +349,7 @@ void orc_pyramid(const orc_geom* g, const uint8_t* img, uint8_t* raw, uint8_t* b
     for (int l = 1; l < g->nlevels; l++) {
         uint8_t* dst = raw + g->off[l];
         const int w = g->w[l], h = g->h[l], p = g->p[l];
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
         for (int y = 0; y < h; y++) {
             for (int x = 0; x < w; x++) {
                 int v;
@@ -472,8 +489,11 @@ void orc_detect_map(const orc_geom* g, const uint8_t* raw, int threshold, int sc
     const int p0 = g->p[0];
     memset(vmap, 0, sizeof(float) * (size_t)g->h[0] * p0);
     memset(lmap, 0, sizeof(int) * (size_t)g->h[0] * p0);
+    /* levels in ascending order (the lowest level wins ties); the rows of a
+       level map to distinct level-0 rows, so they run in parallel */
     for (int l = 0; l < g->nlevels; l++) {
         const int w = g->w[l], h = g->h[l], p = g->p[l];
+#pragma omp parallel for schedule(dynamic, 8) num_threads(g_threads) if (g_threads > 1)
         for (int iy = 3; iy < h - 3; iy++) {
             for (int ix = 3; ix < w - 3; ix++) {
                 long c = g->off[l] + (long)iy * p + ix;
@@ -578,6 +598,7 @@ void orc_angles(const orc_geom* g, const uint8_t* raw, orc_point* pts, int n, in
     int umax[64];
     orc_umax(patch_size, umax);
     const int half = patch_size / 2;
+#pragma omp parallel for schedule(dynamic, 32) num_threads(g_threads) if (g_threads > 1)
     for (int i = 0; i < n; i++) {
         const long c = center_of(g, &pts[i]);
         const int p = g->p[pts[i].octave];
@@ -616,6 +637,7 @@ void orc_describe(const orc_geom* g, const uint8_t* blur, const orc_point* pts, 
 {
     int32_t pat[1024];
     orc_pattern(wta_k, pat);
+#pragma omp parallel for schedule(dynamic, 32) num_threads(g_threads) if (g_threads > 1)
     for (int i = 0; i < n; i++) {
         float sn, cs;
         orc_sincosf(pts[i].angle, &sn, &cs);
@@ -696,6 +718,7 @@ static inline int ham(const uint8_t* a, const uint8_t* b)
    best is unique, so the reference's lane order never shows. */
 void orc_match(const uint8_t* d1, int n1, const uint8_t* d2, int n2, int* match, int* dist)
 {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(g_threads) if (g_threads > 1)
     for (int i = 0; i < n1; i++) {
         int b1 = 256, b2 = 256, i1 = 0;
         for (int j = 0; j < n2; j++) {

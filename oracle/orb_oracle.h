@@ -54,6 +54,10 @@ int   orc_pattern(int wta_k, int32_t* xy /* 1024 ints */);
 void  orc_glibc_srand(unsigned seed);
 int   orc_glibc_rand(void);
 
+/* threads of the oracle's parallel loops (default 1); results do not depend on it */
+void orc_set_threads(int n);
+int  orc_get_threads(void);
+
 /* math helpers (orbd.cu:986-998; own sin/cos replaces __sinf/__cosf at orbd.cu:1446-1447) */
 float orc_fast_atan2(float y, float x);
 void  orc_sincosf(float a, float* s, float* c);

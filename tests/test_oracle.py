@@ -692,3 +692,26 @@ def test_pair_matches_agree_with_ground_truth_homography(oracle):
     dst = dst[:2] / dst[2]
     err = np.hypot(dst[0] - pb["x"][m[ok]], dst[1] - pb["y"][m[ok]])
     assert (err < 8.0).mean() > 0.5   # a wrong stage gives ~0 geometric inliers
+
+
+def test_threaded_oracle_is_identical_to_single_thread(oracle):
+    """The oracle's OpenMP loops (pyramid rows, detection rows within a level, keypoints, match
+    queries) write disjoint outputs: 8 threads give the same keypoints, descriptors and matches as
+    1, at both pyramid scales."""
+    a = oracle_synth(640, 480, 40, 12)
+    b = oracle_synth(640, 480, 40, 13)
+    try:
+        runs = {}
+        for nt in (1, 8):
+            oracle.set_threads(nt)
+            r = []
+            for scale, nl in ((1.2, 8), (2.0, 5)):
+                pa, da, na = oracle.detect_and_compute(a, noctaves=nl, scale=scale, max_pts=700)
+                pb, db, _ = oracle.detect_and_compute(b, noctaves=nl, scale=scale, max_pts=700)
+                r.append((pa.tobytes(), da.tobytes(), na, oracle.match(da, db)))
+            runs[nt] = r
+        for (x, y) in zip(runs[1], runs[8]):
+            assert x[0] == y[0] and x[1] == y[1] and x[2] == y[2]
+            assert np.array_equal(x[3][0], y[3][0]) and np.array_equal(x[3][1], y[3][1])
+    finally:
+        oracle.set_threads(1)
