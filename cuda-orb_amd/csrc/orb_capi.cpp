@@ -276,13 +276,24 @@ private:
     }
 };
 
-/* setPattern for patch 31, orbd.cu:95-410: the table verbatim for WTA_K 2;
-   for WTA_K 3/4, 128 tuples of distinct table points drawn as the reference
-   draws them with srand(0x12345678)/rand(). */
-void make_pattern(int wta_k, int32_t* xy)
+/* setPattern, orbd.cu:95-410.  The base set of 512 points is the table for
+   patch 31; for another patch size p it is drawn as the reference draws it,
+   srand(0x34985739) and each coordinate rand() % p - -p / 2 (orbd.cu:359-369:
+   subtracting the negated half adds p/2, so the points sit off the patch
+   centre in [p/2, p + p/2 - 1]; kept as the reference computes it).  WTA_K 2
+   uses the base set; WTA_K 3/4 draw 128 tuples of distinct base points with
+   srand(0x12345678). */
+void make_pattern(int patch_size, int wta_k, int32_t* xy)
 {
+    int32_t base[1024];
+    if (patch_size == 31) {
+        memcpy(base, k_pattern31, sizeof(base));
+    } else {
+        Type3Rand pick(0x34985739u);
+        for (int i = 0; i < 1024; i++) base[i] = pick.next() % patch_size - -patch_size / 2;
+    }
     if (wta_k == 2) {
-        memcpy(xy, k_pattern31, sizeof(k_pattern31));
+        memcpy(xy, base, sizeof(base));
         return;
     }
     memset(xy, 0, 1024 * sizeof(int32_t));
@@ -293,7 +304,7 @@ void make_pattern(int wta_k, int32_t* xy)
             bool fresh = false;
             while (!fresh) {
                 const int idx = rng.next() % 512;
-                const int2 cand = make_int2(k_pattern31[2 * idx], k_pattern31[2 * idx + 1]);
+                const int2 cand = make_int2(base[2 * idx], base[2 * idx + 1]);
                 fresh = true;
                 for (int e = 0; e < k && fresh; e++) fresh = !(tuple[e].x == cand.x && tuple[e].y == cand.y);
                 if (fresh) tuple[k] = cand;
@@ -593,6 +604,7 @@ Tables make_tables(const orb_ctx* c)
     t.score_type = c->prm.score_type;
     t.wta_k = c->prm.wta_k;
     t.half_patch = c->prm.patch_size / 2;
+    t.pattern31 = c->prm.patch_size == 31;
     for (int i = 0; i < 17; i++) t.umax[i] = c->umax[i];
     return t;
 }
@@ -970,7 +982,8 @@ int orb_init(orb_ctx* c, const orb_params* p)
     if (p->wta_k < 2 || p->wta_k > 4) return fail(c, ORB_E_ARG, "wta_k must be 2, 3 or 4");
     if (p->score_type != orb::HARRIS_SCORE && p->score_type != orb::FAST_SCORE)
         return fail(c, ORB_E_ARG, "unknown score type %d", p->score_type);
-    if (p->patch_size != 31) return fail(c, ORB_E_ARG, "patch_size must be 31");
+    /* above 31 the reference overflows its umax table (orbd.cu:37) */
+    if (p->patch_size < 3 || p->patch_size > 31) return fail(c, ORB_E_ARG, "patch_size must be in [3, 31]");
     if (p->fast_threshold < 0 || p->fast_threshold > 255) return fail(c, ORB_E_ARG, "fast_threshold out of [0,255]");
     if (p->max_pts <= 0) return fail(c, ORB_E_ARG, "max_pts must be positive");
     if (p->max_pts > (1 << 22)) return fail(c, ORB_E_CAPACITY, "max_pts above 2^22");
@@ -984,7 +997,7 @@ int orb_init(orb_ctx* c, const orb_params* p)
         c->harris_scale = s * s * s * s;
     }
     int32_t pat[1024];
-    make_pattern(p->wta_k, pat);
+    make_pattern(p->patch_size, p->wta_k, pat);
     HIP_TRY(c, hipMemcpyAsync(c->d_pattern, pat, sizeof(pat), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->inited = true;
@@ -1184,10 +1197,11 @@ int orb_tune_get(orb_ctx* c, int stage, int* variant, int* nvariants, float* ms)
     return ORB_OK;
 }
 
-int orb_debug_pattern(int wta_k, int32_t* xy)
+int orb_debug_pattern(int patch_size, int wta_k, int32_t* xy)
 {
-    if (!xy || wta_k < 2 || wta_k > 4) return fail(nullptr, ORB_E_ARG, "orb_debug_pattern: wta_k 2..4 and a buffer");
-    make_pattern(wta_k, xy);
+    if (!xy || wta_k < 2 || wta_k > 4 || patch_size < 3 || patch_size > 31)
+        return fail(nullptr, ORB_E_ARG, "orb_debug_pattern: patch_size 3..31, wta_k 2..4 and a buffer");
+    make_pattern(patch_size, wta_k, xy);
     return ORB_OK;
 }
 

@@ -5,10 +5,15 @@
  * protocol -- N x detectAndCompute of both images, then N x match, timed with
  * GpuTimer -- prints the same report, and writes the last results to
  * <outdir>/{points1,points2}.bin (OrbPoint arrays) and {desc1,desc2}.bin.
+ * With --draw 1 it also writes the pictures the reference's demo draws with
+ * OpenCV (drawKeypoints, drawMatches: main.cpp:27-99) as binary PPM files:
+ * keypoints1.ppm, keypoints2.ppm (a circle per keypoint) and matches.ppm (the
+ * two images side by side, a line per accepted match).
  *
  *   orb_demo <device> <img1.pgm | synth:W:H:SEED> <img2.pgm | warp> <outdir>
- *            [--scale S] [--levels N] [--wta K] [--max-pts N] [--repeats N]
+ *            [--scale S] [--levels N] [--wta K] [--max-pts N] [--repeats N] [--draw 0|1]
  */
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -46,6 +51,94 @@ void write_file(const std::string& path, const void* data, size_t bytes)
 	fclose(f);
 }
 
+/* An RGB picture for the keypoint and match drawings. */
+struct Rgb {
+	int w = 0, h = 0;
+	std::vector<unsigned char> px;
+	Rgb(int w_, int h_) : w(w_), h(h_), px((size_t)w_ * h_ * 3, 0) {}
+	void gray(const unsigned char* img, int iw, int ih, int x0, int y0)
+	{
+		for (int y = 0; y < ih; y++)
+			for (int x = 0; x < iw; x++)
+				for (int c = 0; c < 3; c++) px[((size_t)(y + y0) * w + x + x0) * 3 + c] = img[(size_t)y * iw + x];
+	}
+	void dot(int x, int y, const unsigned char* col)
+	{
+		if (x < 0 || y < 0 || x >= w || y >= h) return;
+		for (int c = 0; c < 3; c++) px[((size_t)y * w + x) * 3 + c] = col[c];
+	}
+	/* midpoint circle outline */
+	void circle(int cx, int cy, int r, const unsigned char* col)
+	{
+		for (int x = r, y = 0, e = 1 - r; x >= y; y++) {
+			const int pts[8][2] = {{x, y}, {y, x}, {-y, x}, {-x, y}, {-x, -y}, {-y, -x}, {y, -x}, {x, -y}};
+			for (auto& q : pts) dot(cx + q[0], cy + q[1], col);
+			if (e < 0) e += 2 * y + 3;
+			else { e += 2 * (y - x) + 5; x--; }
+		}
+	}
+	/* Bresenham segment */
+	void line(int x0, int y0, int x1, int y1, const unsigned char* col)
+	{
+		const int dx = abs(x1 - x0), dy = -abs(y1 - y0), sx = x0 < x1 ? 1 : -1, sy = y0 < y1 ? 1 : -1;
+		for (int e = dx + dy;;) {
+			dot(x0, y0, col);
+			if (x0 == x1 && y0 == y1) break;
+			const int e2 = 2 * e;
+			if (e2 >= dy) { e += dy; x0 += sx; }
+			if (e2 <= dx) { e += dx; y0 += sy; }
+		}
+	}
+	void save(const std::string& path) const
+	{
+		FILE* f = fopen(path.c_str(), "wb");
+		if (!f) { fprintf(stderr, "cannot write %s\n", path.c_str()); exit(-1); }
+		fprintf(f, "P6\n%d %d\n255\n", w, h);
+		fwrite(px.data(), 1, px.size(), f);
+		fclose(f);
+	}
+};
+
+/* a bright colour per item from a fixed sequence (the reference picks rand() colours) */
+void color_of(unsigned i, unsigned char* col)
+{
+	unsigned v = i * 2654435761u + 0x9e3779b9u;
+	for (int c = 0; c < 3; c++, v = v * 1103515245u + 12345u) col[c] = (unsigned char)(64 + (v >> 24) % 192);
+}
+
+/* main.cpp:27-38: a circle of radius max(1, min(5, log10(score))) per keypoint */
+void draw_keypoints(const std::vector<unsigned char>& img, int w, int h, const orb::OrbData& d, const std::string& path)
+{
+	Rgb pic(w, h);
+	pic.gray(img.data(), w, h, 0, 0);
+	for (int i = 0; i < d.num_pts; i++) {
+		const orb::OrbPoint& p = d.h_data[i];
+		const float lg = p.score > 0.f ? log10f(p.score) : 1.f;
+		const int r = lg < 1.f ? 1 : (lg > 5.f ? 5 : (int)lg);
+		unsigned char col[3];
+		color_of((unsigned)i, col);
+		pic.circle(p.x, p.y, r, col);
+	}
+	pic.save(path);
+}
+
+/* main.cpp:41-99 (horizontal layout): a line per accepted match */
+void draw_matches(const std::vector<unsigned char>& img1, int w1, int h1, const std::vector<unsigned char>& img2,
+                  int w2, int h2, const orb::OrbData& a, const orb::OrbData& b, const std::string& path)
+{
+	Rgb pic(w1 + w2, h1 > h2 ? h1 : h2);
+	pic.gray(img1.data(), w1, h1, 0, 0);
+	pic.gray(img2.data(), w2, h2, w1, 0);
+	for (int i = 0; i < a.num_pts; i++) {
+		const int k = a.h_data[i].match;
+		if (k < 0 || k >= b.num_pts) continue;
+		unsigned char col[3];
+		color_of((unsigned)i, col);
+		pic.line(a.h_data[i].x, a.h_data[i].y, b.h_data[k].x + w1, b.h_data[k].y, col);
+	}
+	pic.save(path);
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -58,7 +151,7 @@ int main(int argc, char** argv)
 	}
 	/* the reference's demo parameters (main.cpp:176-197) unless overridden */
 	double scale = 2.0;
-	int noctaves = 5, wta_k = 4, max_pts = 10000, repeats = 100;
+	int noctaves = 5, wta_k = 4, max_pts = 10000, repeats = 100, draw = 0;
 	for (int i = 5; i + 1 < argc; i += 2)
 	{
 		if (!strcmp(argv[i], "--scale")) scale = atof(argv[i + 1]);
@@ -66,6 +159,7 @@ int main(int argc, char** argv)
 		else if (!strcmp(argv[i], "--wta")) wta_k = atoi(argv[i + 1]);
 		else if (!strcmp(argv[i], "--max-pts")) max_pts = atoi(argv[i + 1]);
 		else if (!strcmp(argv[i], "--repeats")) repeats = atoi(argv[i + 1]);
+		else if (!strcmp(argv[i], "--draw")) draw = atoi(argv[i + 1]);
 	}
 	printf("===== Registration by MI355X ORB =====\n");
 	if (!initDevice(atoi(argv[1]))) return 1;
@@ -160,6 +254,13 @@ int main(int argc, char** argv)
 			fprintf(stderr, "host copy mismatch at %d\n", i);
 			return 1;
 		}
+	}
+
+	if (draw)
+	{
+		draw_keypoints(host1, w1, h1, d1, out + "/keypoints1.ppm");
+		draw_keypoints(host2, w2, h2, d2, out + "/keypoints2.ppm");
+		draw_matches(host1, w1, h1, host2, w2, h2, d1, d2, out + "/matches.ppm");
 	}
 
 	detector.freeOrbData(d1);

@@ -98,6 +98,7 @@ struct Tables {
     float k0, k1, k2, k3;     /* Gaussian taps */
     float harris_scale;
     int threshold, score_type, wta_k, half_patch;
+    int pattern31;            /* the sampling points are the patch-31 table (patch size 31) */
     int umax[17];
 };
 

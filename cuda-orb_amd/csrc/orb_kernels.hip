@@ -1110,6 +1110,20 @@ __device__ __forceinline__ int sample_lds(const uint8_t* bp, int2 pt, float sn, 
     return bp[iy * kBlurRow + ix];
 }
 
+/* the same sample read from the flat blurred pyramid: offset c of the
+   keypoint from level 0's start, bytes outside the pyramid read as 0 (the
+   reference's flat addressing), for patterns that reach past the staged
+   37 x 37 region */
+__device__ __forceinline__ int sample_flat(const ORB_GLOBAL uint8_t* base, long long total, long long c, int p, int2 pt,
+                                           float sn, float cs)
+{
+    const float x = (float)pt.x, y = (float)pt.y;
+    const int ix = __float2int_rn(__fmaf_rn(x, cs, -__fmul_rn(y, sn)));
+    const int iy = __float2int_rn(__fmaf_rn(x, sn, __fmul_rn(y, cs)));
+    const long long i = c + (long long)iy * p + ix;
+    return (i >= 0 && i < total) ? base[i] : 0;
+}
+
 __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ frames, int blocks_per_frame, Tables T,
                                                   int compute_desc)
 {
@@ -1215,12 +1229,19 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
             float sn, cs;
             dev_sincos(ang, sn, cs);
             const uint8_t* bp = &s_blur[wid][18 * kBlurRow + bo + 18];
+            /* patch 31: every rotated point lies within 18 of the centre, in the
+               staged patch; the random points of other patch sizes sit off
+               centre (orbd.cu:359-369) and are read from the flat pyramid
+               (patch 30 has the same half patch as 31 but random points) */
+            const bool flat = !T.pattern31;
+            const ORB_GLOBAL uint8_t* fb = as_global(F.blur);
+            auto smp = [&](int2 pt) { return flat ? sample_flat(fb, F.total, co, p, pt, sn, cs) : sample_lds(bp, pt, sn, cs); };
             if (T.wta_k == 2) {
                 /* bit k of the descriptor compares points 2k and 2k+1 (orbd.cu:1351-1361) */
                 ORB_GLOBAL unsigned long long* outd = (ORB_GLOBAL unsigned long long*)(as_global(F.desc) + (long long)pi * 32);
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const bool bit = sample_lds(bp, pat2[2 * j], sn, cs) < sample_lds(bp, pat2[2 * j + 1], sn, cs);
+                    const bool bit = smp(pat2[2 * j]) < smp(pat2[2 * j + 1]);
                     const unsigned long long m = __ballot(bit);
                     if (lane == 0) outd[j] = m;
                 }
@@ -1233,13 +1254,13 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
                         const ORB_GLOBAL int2* pp = as_global(T.pattern) + 12 * byte;
                         const int i0 = slot < 3 ? 3 * slot : 8, i1 = slot < 3 ? 3 * slot + 1 : 9,
                                   i2 = slot < 3 ? 3 * slot + 2 : 11;
-                        const int t0 = sample_lds(bp, pp[i0], sn, cs), t1 = sample_lds(bp, pp[i1], sn, cs),
-                                  t2 = sample_lds(bp, pp[i2], sn, cs);
+                        const int t0 = smp(pp[i0]), t1 = smp(pp[i1]),
+                                  t2 = smp(pp[i2]);
                         code = t2 > t1 ? (t2 > t0 ? 2 : 0) : (t1 > t0);
                     } else {
                         const ORB_GLOBAL int2* pp = as_global(T.pattern) + 16 * byte + 4 * slot;
-                        int t0 = sample_lds(bp, pp[0], sn, cs), t1 = sample_lds(bp, pp[1], sn, cs);
-                        int t2 = sample_lds(bp, pp[2], sn, cs), t3 = sample_lds(bp, pp[3], sn, cs);
+                        int t0 = smp(pp[0]), t1 = smp(pp[1]);
+                        int t2 = smp(pp[2]), t3 = smp(pp[3]);
                         int u = 0, v = 2;
                         if (t1 > t0) { t0 = t1; u = 1; }
                         if (t3 > t2) { t2 = t3; v = 3; }

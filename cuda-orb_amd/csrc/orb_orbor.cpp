@@ -71,7 +71,16 @@ namespace orb
 			fprintf(stderr, "ORB error: detectAndCompute before init\n");
 			exit(-1);
 		}
+		/* the image pitch is iAlignUp(width, 128) (orb.cpp:162, orbd.cu:1584),
+		   which the reference assumes without reading whp0.z; a nonzero whp0.z
+		   that disagrees would make it read the image wrongly, so it is an error */
 		const int pitch = iAlignUp(whp0.x, 128);
+		if (whp0.z != 0 && whp0.z != pitch)
+		{
+			fprintf(stderr, "ORB error: detectAndCompute: whp0.z = %d, but the image pitch must be iAlignUp(%d, 128) = %d\n",
+				whp0.z, whp0.x, pitch);
+			exit(-1);
+		}
 		const int want_desc = compute_desc && desc_addr != NULL;
 		check(orb_detect_and_compute(ctx, image, whp0.x, whp0.y, pitch, result.d_data, want_desc ? d_desc_work : NULL,
 			d_count, compute_desc ? 1 : 0), "detectAndCompute");

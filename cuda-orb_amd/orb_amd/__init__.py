@@ -103,7 +103,7 @@ def lib():
         L.orb_workspace_bytes.restype = ctypes.c_size_t
         L.orb_synth_image.argtypes = [ctypes.c_uint64, I, I, I, I, V]
         L.orb_synth_warp.argtypes = [ctypes.c_uint64, V, I, I, I, V, V]
-        L.orb_debug_pattern.argtypes = [I, V]
+        L.orb_debug_pattern.argtypes = [I, I, V]
         L.orb_debug_sincos.argtypes = [P, V, I, V, V]
         L.orb_debug_atan2.argtypes = [P, V, I, V]
         L.orb_profile_enable.argtypes = [P, ctypes.c_uint]

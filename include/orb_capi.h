@@ -41,7 +41,7 @@ typedef struct {
     int edge_threshold;   /* border without keypoints (orb.h:40) */
     int wta_k;            /* 2, 3 or 4 (orb.h:42) */
     int score_type;       /* HARRIS_SCORE or FAST_SCORE (orb.h:47) */
-    int patch_size;       /* 31 (orb.h:49) */
+    int patch_size;       /* 3..31 (orb.h:49); 31 uses the reference's table, others its random points */
     int fast_threshold;   /* FAST intensity threshold (orb.h:51) */
     int retain_topn;      /* unused by detection, as in the reference (orb.cpp:41); orb_export_keypoints honours it */
     int max_pts;          /* keypoints kept per image (orb.h:55) */
@@ -308,8 +308,9 @@ int orb_tune_set(orb_ctx* ctx, int stage, int variant);
 int orb_tune_get(orb_ctx* ctx, int stage, int* variant, int* nvariants, float* ms);
 
 /* For tests, host only (no GPU): the 512 sampling points (x, y pairs) that
-   orb_init uploads for wta_k 2, 3 or 4 (setPattern, orbd.cu:95-410). */
-int orb_debug_pattern(int wta_k, int32_t* xy);
+   orb_init uploads for a patch size (3..31) and wta_k 2, 3 or 4 (setPattern,
+   orbd.cu:95-410). */
+int orb_debug_pattern(int patch_size, int wta_k, int32_t* xy);
 
 /* Device self-checks of the math helpers, for tests: n angles in d_in give
    sin, cos (own polynomials) and n (y, x) pairs give the centroid atan2. */

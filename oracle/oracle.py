@@ -72,6 +72,7 @@ def lib():
         L.orc_match_top2.argtypes = [P, I, P, I, I, P, P, P, P]
         L.orc_match_top2_norm.argtypes = [P, I, P, I, I, I, P, P, P, P]
         L.orc_pattern.argtypes = [I, P]
+        L.orc_pattern_patch.argtypes = [I, I, P]
         L.orc_glibc_srand.argtypes = [ctypes.c_uint]
         L.orc_umax.argtypes = [I, P]
         L.orc_gauss.argtypes = [P]
@@ -241,10 +242,10 @@ def sincosf(a):
     return s.value, c.value
 
 
-def pattern(wta_k=2):
+def pattern(wta_k=2, patch_size=31):
     out = np.zeros(1024, np.int32)
-    if lib().orc_pattern(wta_k, ptr(out)) != 0:
-        raise ValueError("wta_k")
+    if lib().orc_pattern_patch(patch_size, wta_k, ptr(out)) != 0:
+        raise ValueError("wta_k / patch_size")
     return out.reshape(512, 2)
 
 

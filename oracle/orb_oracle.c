@@ -122,15 +122,30 @@ int orc_glibc_rand(void)
     return (int)((uint32_t)v >> 1);
 }
 
-/* orbd.cu:95-410 for patch_size 31: WTA_K 2 uses the table verbatim, WTA_K 3/4
-   resample it into 128 tuples of distinct points with srand(0x12345678). */
-int orc_pattern(int wta_k, int32_t* xy)
+/* orbd.cu:95-410: the base set of 512 points is the table (patch_size 31) or,
+   for another patch size p, 512 points drawn with srand(0x34985739), each
+   coordinate rand() % p - -p / 2 (orbd.cu:359-369: the subtraction of a
+   negated half makes every offset +p/2, so the points lie in
+   [p/2, p + p/2 - 1], off the patch centre, a reference bug kept as it is).
+   WTA_K 2 uses the base set as it is; WTA_K 3/4 resample it into 128 tuples
+   of distinct points with srand(0x12345678). */
+int orc_pattern_patch(int patch_size, int wta_k, int32_t* xy)
 {
+    if (wta_k < 2 || wta_k > 4 || patch_size < 3 || patch_size > 31) return -1;
+    int32_t base[1024];
+    if (patch_size == 31) {
+        memcpy(base, k_pattern31, sizeof(base));
+    } else {
+        orc_glibc_srand(0x34985739u);
+        for (int i = 0; i < 512; i++) {
+            base[2 * i] = orc_glibc_rand() % patch_size - -patch_size / 2;
+            base[2 * i + 1] = orc_glibc_rand() % patch_size - -patch_size / 2;
+        }
+    }
     if (wta_k == 2) {
-        memcpy(xy, k_pattern31, sizeof(k_pattern31));
+        memcpy(xy, base, sizeof(base));
         return 0;
     }
-    if (wta_k != 3 && wta_k != 4) return -1;
     memset(xy, 0, 1024 * sizeof(int32_t));
     orc_glibc_srand(0x12345678u);
     const int ntuples = 128;
@@ -138,21 +153,26 @@ int orc_pattern(int wta_k, int32_t* xy)
         for (int k = 0; k < wta_k; k++) {
             for (;;) {
                 int idx = orc_glibc_rand() % 512;
-                int px = k_pattern31[2 * idx], py = k_pattern31[2 * idx + 1];
+                int px_ = base[2 * idx], py_ = base[2 * idx + 1];
                 int k1;
                 for (k1 = 0; k1 < k; k1++) {
                     int q = wta_k * i + k1;
-                    if (px == xy[2 * q] && py == xy[2 * q + 1]) break;
+                    if (px_ == xy[2 * q] && py_ == xy[2 * q + 1]) break;
                 }
                 if (k1 == k) {
-                    xy[2 * (wta_k * i + k)] = px;
-                    xy[2 * (wta_k * i + k) + 1] = py;
+                    xy[2 * (wta_k * i + k)] = px_;
+                    xy[2 * (wta_k * i + k) + 1] = py_;
                     break;
                 }
             }
         }
     }
     return 0;
+}
+
+int orc_pattern(int wta_k, int32_t* xy)
+{
+    return orc_pattern_patch(31, wta_k, xy);
 }
 
 /* -------------------------------------------------------------------- math */
@@ -632,11 +652,11 @@ static int sample(const orc_geom* g, const uint8_t* b, long c, int p, const int3
 /* orbd.cu:1351-1468: 32 bytes per keypoint; WTA_K 2 compares pairs (bit j of
    byte b from points 16b+2j, 16b+2j+1), WTA_K 3 and 4 write 2-bit argmax codes
    (including the reference's 8,9,11 fourth triple for WTA_K 3). */
-void orc_describe(const orc_geom* g, const uint8_t* blur, const orc_point* pts, int n, int wta_k,
-                  uint8_t* desc)
+void orc_describe_patch(const orc_geom* g, const uint8_t* blur, const orc_point* pts, int n, int patch_size,
+                        int wta_k, uint8_t* desc)
 {
     int32_t pat[1024];
-    orc_pattern(wta_k, pat);
+    orc_pattern_patch(patch_size, wta_k, pat);
 #pragma omp parallel for schedule(dynamic, 32) num_threads(g_threads) if (g_threads > 1)
     for (int i = 0; i < n; i++) {
         float sn, cs;
@@ -679,6 +699,11 @@ void orc_describe(const orc_geom* g, const uint8_t* blur, const orc_point* pts, 
     }
 }
 
+void orc_describe(const orc_geom* g, const uint8_t* blur, const orc_point* pts, int n, int wta_k, uint8_t* desc)
+{
+    orc_describe_patch(g, blur, pts, n, 31, wta_k, desc);
+}
+
 /* ---------------------------------------------------------------- pipeline */
 
 /* orb.cpp:57-101 + orb.cpp:194-201: pyramid, detect map, NMS and cap,
@@ -696,7 +721,7 @@ int orc_detect_and_compute(const orc_params* prm, const uint8_t* img, int w, int
     orc_detect_map(&g, raw, prm->fast_threshold, prm->score_type, vmap, lmap);
     int n = orc_nms(&g, vmap, lmap, prm->edge_threshold, prm->max_pts, pts, n_survivors);
     orc_angles(&g, raw, pts, n, prm->patch_size);
-    orc_describe(&g, blur, pts, n, prm->wta_k, desc);
+    orc_describe_patch(&g, blur, pts, n, prm->patch_size, prm->wta_k, desc);
     *n_out = n;
     free(raw); free(blur); free(vmap); free(lmap);
     return 0;

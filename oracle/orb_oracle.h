@@ -51,6 +51,7 @@ void  orc_umax(int patch_size, int* umax /* patch/2+2 */);
 void  orc_gauss(float k[4]);
 float orc_harris_scale(void);
 int   orc_pattern(int wta_k, int32_t* xy /* 1024 ints */);
+int   orc_pattern_patch(int patch_size, int wta_k, int32_t* xy /* 1024 ints */);
 void  orc_glibc_srand(unsigned seed);
 int   orc_glibc_rand(void);
 
@@ -79,6 +80,8 @@ float orc_fast_score(const uint8_t* raw, long total, long center, int pitch, int
 int  orc_nms(const orc_geom* g, const float* vmap, const int* lmap, int border, int max_pts,
              orc_point* pts, int* n_survivors);
 void orc_angles(const orc_geom* g, const uint8_t* raw, orc_point* pts, int n, int patch_size);
+void orc_describe_patch(const orc_geom* g, const uint8_t* blur, const orc_point* pts, int n, int patch_size,
+                        int wta_k, uint8_t* desc);
 void orc_describe(const orc_geom* g, const uint8_t* blur, const orc_point* pts, int n, int wta_k,
                   uint8_t* desc);
 

@@ -84,10 +84,10 @@ def test_context_creation_without_gpu_fails_cleanly():
     assert lib.orb_last_error(None)
 
 
-def _pattern(lib, wta_k):
+def _pattern(lib, wta_k, patch_size=31):
     import orb_amd
     xy = np.zeros(1024, np.int32)
-    assert lib.orb_debug_pattern(wta_k, orb_amd._np_ptr(xy)) == 0
+    assert lib.orb_debug_pattern(patch_size, wta_k, orb_amd._np_ptr(xy)) == 0
     return xy.reshape(512, 2)
 
 
@@ -96,9 +96,12 @@ def test_product_patterns_equal_the_oracle(oracle):
     library's own TYPE_3 generator) equal the oracle's, whose generator is pinned against libc."""
     import orb_amd
     lib = orb_amd.lib()
-    for wta_k in (2, 3, 4):
-        assert np.array_equal(_pattern(lib, wta_k), oracle.pattern(wta_k)), wta_k
-    assert lib.orb_debug_pattern(5, orb_amd._np_ptr(np.zeros(1024, np.int32))) == -1
+    for patch in (31, 25, 8):
+        for wta_k in (2, 3, 4):
+            assert np.array_equal(_pattern(lib, wta_k, patch), oracle.pattern(wta_k, patch)), (patch, wta_k)
+    buf = orb_amd._np_ptr(np.zeros(1024, np.int32))
+    assert lib.orb_debug_pattern(31, 5, buf) == -1
+    assert lib.orb_debug_pattern(32, 2, buf) == -1 and lib.orb_debug_pattern(2, 2, buf) == -1
 
 
 def test_pattern_leaves_the_callers_rand_stream_alone():

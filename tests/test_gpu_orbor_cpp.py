@@ -63,13 +63,34 @@ def check(oracle, pts, desc, imgs, **kw):
     return m
 
 
+def read_ppm(path):
+    data = path.read_bytes()
+    parts = data.split(maxsplit=4)
+    assert parts[0] == b"P6" and parts[3] == b"255"
+    w, h = int(parts[1]), int(parts[2])
+    return np.frombuffer(parts[4][: w * h * 3], np.uint8).reshape(h, w, 3)
+
+
 def test_reference_images_reference_parameters(tmp_path, oracle):
     left, right = GOLDEN / "left.pgm", GOLDEN / "right.pgm"
-    pts, desc, out = run_demo(tmp_path, str(left), str(right), ["--repeats", "5"])
+    pts, desc, out = run_demo(tmp_path, str(left), str(right), ["--repeats", "5", "--draw", "1"])
     assert "Time of detection and computation" in out
-    m = check(oracle, pts, desc, [read_pgm(left), read_pgm(right)], noctaves=5, scale=2.0, wta_k=4,
-              max_pts=10000)
+    img1, img2 = read_pgm(left), read_pgm(right)
+    m = check(oracle, pts, desc, [img1, img2], noctaves=5, scale=2.0, wta_k=4, max_pts=10000)
     assert (m >= 0).sum() > 200
+    # the drawings (main.cpp:27-99): the images in grey, a coloured ring around each keypoint,
+    # a coloured line from each accepted match to its partner in the right half
+    kp = read_ppm(tmp_path / "keypoints1.ppm")
+    assert kp.shape == img1.shape + (3,)
+    grey = (kp[..., 0] == kp[..., 1]) & (kp[..., 1] == kp[..., 2])
+    assert (kp[grey][:, 0] == img1[grey]).all()
+    p1 = pts[0]
+    ring = [(p1[i, 1], p1[i, 0] + 1) for i in range(len(p1)) if 2 <= p1[i, 0] < img1.shape[1] - 2]
+    assert np.mean([not grey[y, x] for y, x in ring]) > 0.5
+    mt = read_ppm(tmp_path / "matches.ppm")
+    assert mt.shape == (max(img1.shape[0], img2.shape[0]), img1.shape[1] + img2.shape[1], 3)
+    left_half = mt[:, : img1.shape[1]]
+    assert (~((left_half[..., 0] == left_half[..., 1]) & (left_half[..., 1] == left_half[..., 2]))).sum() > 1000
 
 
 def test_synthetic_pair_bilinear_pyramid(tmp_path, oracle):

@@ -88,6 +88,26 @@ def test_detect_and_compute_matches_oracle(torch, ctx, oracle, w, h, nshapes, se
     assert np.array_equal(got_d2, want_d)
 
 
+@pytest.mark.parametrize("patch,wta,scale,nl", [(25, 2, 1.2, 8), (25, 4, 1.2, 8), (9, 3, 2.0, 5), (30, 2, 1.2, 8)])
+def test_patch_sizes_below_31_match_oracle(torch, ctx, oracle, patch, wta, scale, nl):
+    """Patch sizes other than 31: the umax disc of the patch for the angle and the reference's
+    random sampling points (off centre, orbd.cu:359-369), read from the flat blurred pyramid with
+    bytes outside it as 0; keypoints, angles and descriptors equal the oracle."""
+    import orb_amd
+    print(f"patch {patch} wta {wta} scale {scale}", flush=True)
+    w, h = 640, 480
+    img = orb_amd.synth_image(9, w, h, 40)
+    ctx.init(noctaves=nl, wta_k=wta, patch_size=patch, max_pts=1000, scale_factor=scale)
+    got_p, got_d = gpu_detect(torch, ctx, img, w, h, 1000)
+    want_p, want_d, _ = oracle.detect_and_compute(img, noctaves=nl, scale=scale, wta_k=wta, patch_size=patch,
+                                                  max_pts=1000)
+    assert len(want_p) > 100
+    assert_same_points(got_p, want_p)
+    assert np.array_equal(got_d, want_d)
+    with pytest.raises(orb_amd.OrbError):
+        ctx.init(noctaves=nl, patch_size=33, max_pts=1000, scale_factor=scale)
+
+
 @pytest.mark.parametrize("w,h", [(16, 16), (40, 40), (64, 48), (100, 100), (17, 300), (300, 17)])
 def test_tiny_frames_match_oracle(torch, ctx, oracle, w, h):
     """Frames at the 16-pixel level floor and inside the 31-pixel detection border: the same

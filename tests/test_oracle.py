@@ -715,3 +715,22 @@ def test_threaded_oracle_is_identical_to_single_thread(oracle):
             assert np.array_equal(x[3][0], y[3][0]) and np.array_equal(x[3][1], y[3][1])
     finally:
         oracle.set_threads(1)
+
+
+def test_small_patch_pattern_is_the_references_libc_draw(oracle):
+    """Patch sizes other than 31 use 512 random points, srand(0x34985739) and each coordinate
+    rand() % p - -p / 2 in C (orbd.cu:359-369), i.e. rand() % p + p // 2: checked against libc's own
+    generator, and WTA_K 3/4 tuples are made of those points."""
+    libc = ctypes.CDLL("libc.so.6")
+    for p in (25, 8, 30):
+        libc.srand(0x34985739)
+        want = [libc.rand() % p + p // 2 for _ in range(1024)]
+        got = oracle.pattern(2, p)
+        assert got.ravel().tolist() == want
+        assert got.min() >= p // 2 and got.max() <= p - 1 + p // 2
+        base = {tuple(v) for v in got}
+        for k in (3, 4):
+            tup = oracle.pattern(k, p)
+            assert {tuple(v) for v in tup[: 128 * k]} <= base
+    with pytest.raises(ValueError):
+        oracle.pattern(2, 33)
