@@ -165,11 +165,22 @@ def batch_frames(ctx, torch, dev, reps=5):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     kp = counts.cpu().numpy()
+    # each stage timed on its own, after the timed batches (the match stage of a batch is one launch)
+    stage_ms = {}
+    for si, name in enumerate(orb_amd.STAGES):
+        ctx.profile_enable(1 << si)
+        for _ in range(2):
+            step()
+        torch.cuda.synchronize()
+        v = ctx.profile_read()[name]
+        if v[1]:
+            stage_ms[name] = round(v[0] / v[1], 4)
+    ctx.profile_enable(0)
     ctx.init(noctaves=LEVELS, edge_threshold=31, wta_k=2, score_type=0, patch_size=31, fast_threshold=20,
              max_pts=CAP, scale_factor=SCALE)
     return {"frames": 128, "pairs": 64, "ms_per_batch": round(dt * 1e3, 3),
             "frames_per_s": round(128 / dt, 1), "mean_keypoints": round(float(kp.mean()), 1),
-            "megapixels": round(sum(w * h for w, h in zip(ws, hs)) / 1e6, 2)}
+            "megapixels": round(sum(w * h for w, h in zip(ws, hs)) / 1e6, 2), "stage_ms": stage_ms}
 
 
 def synth_frames(seeds, w, h, nshapes, warp_seed=None, threads=16):

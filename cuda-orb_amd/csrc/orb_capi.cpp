@@ -123,6 +123,8 @@ struct FrameSet {
     FrameDev* d_frames = nullptr;
     Job* d_pyr[kPyrVariants] = {};
     Job* d_det = nullptr;
+    int* d_pyr_map[kPyrVariants] = {};   /* job index of every pyramid unit */
+    int* d_det_map = nullptr;            /* job index of every detection tile */
     int pyr_njobs[kPyrVariants] = {}, pyr_units[kPyrVariants] = {}, det_njobs = 0, det_blocks = 0;
     /* autotuner: measured ms per pyramid variant and the choice (-1: not tuned) */
     float pyr_ms[kPyrVariants] = {};
@@ -477,6 +479,22 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     size_t o_pyr_jobs[kPyrVariants];
     for (int v = 0; v < kPyrVariants; v++) o_pyr_jobs[v] = carve(sizeof(Job) * pyr_jobs[v].size());
     const size_t o_det_jobs = carve(sizeof(Job) * det_jobs.size());
+    /* the job of every unit, so a unit finds its job in one lookup */
+    std::vector<int> pyr_map[kPyrVariants], det_map((size_t)det_blocks);
+    for (int v = 0; v < kPyrVariants; v++) {
+        pyr_map[v].resize((size_t)pyr_units[v]);
+        for (size_t j = 0; j < pyr_jobs[v].size(); j++) {
+            const int end = j + 1 < pyr_jobs[v].size() ? pyr_jobs[v][j + 1].first_block : pyr_units[v];
+            for (int u = pyr_jobs[v][j].first_block; u < end; u++) pyr_map[v][u] = (int)j;
+        }
+    }
+    for (size_t j = 0; j < det_jobs.size(); j++) {
+        const int end = j + 1 < det_jobs.size() ? det_jobs[j + 1].first_block : det_blocks;
+        for (int u = det_jobs[j].first_block; u < end; u++) det_map[u] = (int)j;
+    }
+    size_t o_pyr_map[kPyrVariants];
+    for (int v = 0; v < kPyrVariants; v++) o_pyr_map[v] = carve(sizeof(int) * pyr_map[v].size());
+    const size_t o_det_map = carve(sizeof(int) * det_map.size());
     const size_t o_nms_frame = carve(sizeof(int) * nms_frame.size());
     const size_t o_cand = carve(sizeof(Candidate) * (size_t)nms_blocks * kNmsCap);
     const size_t o_cnt = carve(sizeof(int) * nms_blocks);
@@ -497,6 +515,8 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
     fs.d_frames = (FrameDev*)(base + o_frames);
     for (int v = 0; v < kPyrVariants; v++) fs.d_pyr[v] = (Job*)(base + o_pyr_jobs[v]);
     fs.d_det = (Job*)(base + o_det_jobs);
+    for (int v = 0; v < kPyrVariants; v++) fs.d_pyr_map[v] = (int*)(base + o_pyr_map[v]);
+    fs.d_det_map = (int*)(base + o_det_map);
     fs.d_nms_frame = (int*)(base + o_nms_frame);
     fs.d_cand = (Candidate*)(base + o_cand);
     fs.d_blk_count = (int*)(base + o_cnt);
@@ -528,6 +548,13 @@ int build_frameset(orb_ctx* c, const std::vector<int>& ws, const std::vector<int
                                   hipMemcpyHostToDevice, c->stream));
     if (!det_jobs.empty())
         HIP_TRY(c, hipMemcpyAsync(fs.d_det, det_jobs.data(), sizeof(Job) * det_jobs.size(), hipMemcpyHostToDevice, c->stream));
+    for (int v = 0; v < kPyrVariants; v++)
+        if (!pyr_map[v].empty())
+            HIP_TRY(c, hipMemcpyAsync(fs.d_pyr_map[v], pyr_map[v].data(), sizeof(int) * pyr_map[v].size(),
+                                      hipMemcpyHostToDevice, c->stream));
+    if (!det_map.empty())
+        HIP_TRY(c, hipMemcpyAsync(fs.d_det_map, det_map.data(), sizeof(int) * det_map.size(), hipMemcpyHostToDevice,
+                                  c->stream));
     if (!nms_frame.empty())
         HIP_TRY(c, hipMemcpyAsync(fs.d_nms_frame, nms_frame.data(), sizeof(int) * nms_frame.size(), hipMemcpyHostToDevice, c->stream));
     if (tab_ints)
@@ -613,6 +640,7 @@ void set_pyramid_variant(const FrameSet& fs, DetectLaunch& d, int v)
 {
     d.pyr_variant = v;
     d.pyr_jobs = fs.d_pyr[v];
+    d.pyr_map = fs.d_pyr_map[v];
     d.pyr_njobs = fs.pyr_njobs[v];
     d.pyr_units = fs.pyr_units[v];
     d.pyr_blocks = (fs.pyr_units[v] + 3) / 4;
@@ -698,6 +726,7 @@ int run_frames(orb_ctx* c, int nf, const uint8_t* const* imgs, const int* ws, co
     d.nframes = nf;
 
     d.det_jobs = fs.d_det;
+    d.det_map = fs.d_det_map;
     d.det_njobs = fs.det_njobs;
     d.det_blocks = fs.det_blocks;
     d.nms_blocks = fs.nms_blocks;

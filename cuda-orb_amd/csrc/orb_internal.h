@@ -105,8 +105,10 @@ struct Tables {
 struct DetectLaunch {
     const FrameDev* frames;
     int nframes;
-    const Job* pyr_jobs; int pyr_njobs; int pyr_blocks; int pyr_units; int pyr_variant;
-    const Job* det_jobs; int det_njobs; int det_blocks;
+    /* jobs sorted by first_block, and the job index of every unit (pyramid:
+       one wavefront per unit; detection: one workgroup per unit) */
+    const Job* pyr_jobs; const int* pyr_map; int pyr_njobs; int pyr_blocks; int pyr_units; int pyr_variant;
+    const Job* det_jobs; const int* det_map; int det_njobs; int det_blocks;
     int nms_blocks;             /* NMS units (row segments) */
     const int* nms_block_frame; /* frame of each NMS unit */
     Candidate* cand;            /* kNmsCap per NMS unit */

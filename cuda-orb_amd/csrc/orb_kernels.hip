@@ -83,21 +83,12 @@ __device__ __forceinline__ ORB_GLOBAL T* as_global(T* p)
     return (ORB_GLOBAL T*)p;
 }
 
-/* The job of unit b: jobs are sorted by first_block, so it is the last one
-   starting at or before b.  Each lane tests one job (a single coalesced load
-   per 64 jobs instead of a chain of dependent loads); called by whole
-   wavefronts with a wave-uniform b. */
-__device__ __forceinline__ Job find_job(const Job* __restrict__ jobs, int n, int b)
+/* The job of unit b (wave-uniform): job_of holds the job index of every
+   unit, so the lookup is two dependent scalar loads whatever the number of
+   jobs (frames x levels x bands). */
+__device__ __forceinline__ Job find_job(const Job* __restrict__ jobs, const int* __restrict__ job_of, int b)
 {
-    const int lane = threadIdx.x & 63;
-    int count = 0;
-    for (int j0 = 0; j0 < n; j0 += 64) {
-        const int j = j0 + lane;
-        const unsigned long long m = __ballot(j < n && jobs[j].first_block <= b);
-        count += __popcll(m);
-        if (m != ~0ull) break;
-    }
-    return jobs[__builtin_amdgcn_readfirstlane(max(count - 1, 0))];
+    return jobs[__builtin_amdgcn_readfirstlane(job_of[b])];
 }
 
 /* Fixed-point bilinear source position of level index i along an axis of S
@@ -365,14 +356,14 @@ __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int 
 
 template <int TH>
 __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
-                                                 int njobs, int nunits, Tables T)
+                                                 const int* __restrict__ job_of, int nunits, Tables T)
 {
     static_assert(TH + 6 <= 64, "one lane per walked row holds its bilinear row entry");
     static_assert((TH + 6) % 8 == 0, "a full strip walks whole unrolled groups of 8 rows");
     const int lane = threadIdx.x & 63;
     const int u = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     if (u >= nunits) return;
-    const Job jb = find_job(jobs, njobs, u);
+    const Job jb = find_job(jobs, job_of, u);
     const FrameDev& F = frames[jb.frame];
     const int l = jb.level;
     const int t = u - jb.first_block;
@@ -485,29 +476,12 @@ __device__ __forceinline__ int ring_off(int k)
     }
 }
 
-/* Harris response from the staged tile, orbd.cu:651-669: Sobel over the 7x7
-   window, 32-bit wrapping products, then fma(-0.04, B, A) * (1/(4*7*255))^4 */
-__device__ float harris_lds(const uint8_t* c, float scale)
+/* Harris response of orbd.cu:651-669 from the sums of the Sobel products
+   over the 7x7 window: 32-bit wrapping products, then
+   fma(-0.04, B, A) * (1/(4*7*255))^4.  The sums are of 49 integer terms each
+   (every |dx|, |dy| <= 1020), so their order does not change them. */
+__device__ __forceinline__ float harris_from_sums(int dxx, int dyy, int dxy, float scale)
 {
-    int dxx = 0, dyy = 0, dxy = 0;
-#pragma unroll
-    for (int i = -3; i <= 3; i++) {
-        const uint8_t* up = c + (i - 1) * kDetSW;
-        const uint8_t* md = c + i * kDetSW;
-        const uint8_t* dn = c + (i + 1) * kDetSW;
-        int u0 = up[-4], u1 = up[-3], m0 = md[-4], m1 = md[-3], d0 = dn[-4], d1 = dn[-3];
-#pragma unroll
-        for (int j = -3; j <= 3; j++) {
-            const int u2 = up[j + 1], m2 = md[j + 1], d2 = dn[j + 1];
-            /* centre column j: left = (u0,m0,d0), mid = (u1,m1,d1), right = (u2,m2,d2) */
-            const int dx = (m2 - m0) * 2 + (u2 - u0) + (d2 - d0);
-            const int dy = (d1 - u1) * 2 + (d0 - u0) + (d2 - u2);
-            dxx += dx * dx;
-            dyy += dy * dy;
-            dxy += dx * dy;
-            u0 = u1; u1 = u2; m0 = m1; m1 = m2; d0 = d1; d1 = d2;
-        }
-    }
     const int A = (int)((unsigned)dxx * (unsigned)dyy - (unsigned)dxy * (unsigned)dxy);
     const unsigned S = (unsigned)(dxx + dyy);
     const int B = (int)(S * S);
@@ -571,7 +545,8 @@ __device__ __forceinline__ int wave_append(bool pred, int* counter)
    3. the corners, compacted: Harris (or FAST) score and the 64-bit max into
       the level-0 key map (orbd.cu:742-755). */
 __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ frames, const Job* __restrict__ jobs,
-                                                int njobs, Tables T, const uint32_t* __restrict__ epochs)
+                                                const int* __restrict__ job_of, Tables T,
+                                                const uint32_t* __restrict__ epochs)
 {
     __shared__ __align__(16) uint8_t s_tile[kDetSH * kDetSW];
     __shared__ uint16_t s_cand[kDetTW * kDetTH];
@@ -579,7 +554,7 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     __shared__ int s_n[2];
     __shared__ int s_scan[8];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const Job jb = find_job(jobs, njobs, b);
+    const Job jb = find_job(jobs, job_of, b);
     const FrameDev& F = frames[jb.frame];
     const int l = jb.level;
     const int t = b - jb.first_block;
@@ -703,21 +678,8 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     const int n = s_n[1];
     if (n == 0) return;
     const unsigned long long ep = (unsigned long long)epochs[jb.frame] << kEpochShift;
-    for (int i = tid; i < n; i += 256) {
-        const int e = s_corner[i];
-        const int ly = e >> 8, lx2 = e & 255;
-        const uint8_t* c = &s_tile[(ly + 4) * kDetSW + lx2 + 4];
-        float score;
-        if (T.score_type == 0) {
-            score = harris_lds(c, T.harris_scale);
-        } else {
-            int ring[16];
-#pragma unroll
-            for (int k = 0; k < 16; k++) ring[k] = c[ring_off(k)];
-            score = fast_score(ring, c[0], thr);
-        }
-        if (!(score > 0.f)) continue;
-        const int ix = x0 + lx2, iy = y0 + ly;
+    const auto emit = [&](int e, float score) {
+        const int ix = x0 + (e & 255), iy = y0 + (e >> 8);
         const int X0 = to_l0(F, T.tabs, l, ix, false), Y0 = to_l0(F, T.tabs, l, iy, true);
         const unsigned long long key = ep | ((unsigned long long)__float_as_uint(score) << 3) |
                                        (unsigned long long)(7 - l);
@@ -725,6 +687,58 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
                                __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_or(as_global(F.cbits) + (long long)Y0 * (F.p0 >> 5) + (X0 >> 5), 1u << (X0 & 31),
                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    if (T.score_type == 0) {
+        /* Harris, 16 lanes per corner and 4 corners per wavefront pass (a
+           tile holds a few corners, so one lane per corner would leave most
+           lanes idle through the 49-position window): lane `sub` of a corner
+           takes window positions sub, sub + 16, sub + 32 and sub + 48, then
+           the row of 16 lanes sums its products */
+        const int lane = tid & 63, wid = tid >> 6, sub = lane & 15;
+        int woff[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int pos = sub + 16 * j, wy = pos / 7;
+            woff[j] = (wy - 3) * kDetSW + (pos - 7 * wy - 3);
+        }
+        for (int g = 4 * wid; g < n; g += 16) {
+            const int ci = g + (lane >> 4);
+            int sxx = 0, syy = 0, sxy = 0, e = 0;
+            if (ci < n) {
+                e = s_corner[ci];
+                const uint8_t* c = &s_tile[((e >> 8) + 4) * kDetSW + (e & 255) + 4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (j == 3 && sub != 0) break;   /* position 48 is the last */
+                    const uint8_t* m = c + woff[j];
+                    const int u0 = m[-kDetSW - 1], u1 = m[-kDetSW], u2 = m[-kDetSW + 1];
+                    const int m0 = m[-1], m2 = m[1];
+                    const int d0 = m[kDetSW - 1], d1 = m[kDetSW], d2 = m[kDetSW + 1];
+                    const int dx = (m2 - m0) * 2 + (u2 - u0) + (d2 - d0);
+                    const int dy = (d1 - u1) * 2 + (d0 - u0) + (d2 - u2);
+                    sxx += dx * dx;
+                    syy += dy * dy;
+                    sxy += dx * dy;
+                }
+            }
+            sxx = row_sum16(sxx);
+            syy = row_sum16(syy);
+            sxy = row_sum16(sxy);
+            if (ci < n && sub == 0) {
+                const float score = harris_from_sums(sxx, syy, sxy, T.harris_scale);
+                if (score > 0.f) emit(e, score);
+            }
+        }
+    } else {
+        for (int i = tid; i < n; i += 256) {
+            const int e = s_corner[i];
+            const uint8_t* c = &s_tile[((e >> 8) + 4) * kDetSW + (e & 255) + 4];
+            int ring[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) ring[k] = c[ring_off(k)];
+            const float score = fast_score(ring, c[0], thr);
+            if (score > 0.f) emit(e, score);
+        }
     }
 }
 
@@ -1306,7 +1320,7 @@ hipError_t launch_pyramid(const DetectLaunch& d, int variant, hipStream_t s, con
 #define ORB_PYR_CASE(v)                                                                                         \
     case v:                                                                                                     \
         launch_stage(ev, kStPyramid, k_pyramid<kPyrTHs[v]>, dim3(d.pyr_blocks), dim3(256), s, d.frames, d.pyr_jobs, \
-                     d.pyr_njobs, d.pyr_units, d.t);                                                            \
+                     d.pyr_map, d.pyr_units, d.t);                                                              \
         break;
         ORB_PYR_CASE(0)
         ORB_PYR_CASE(1)
@@ -1327,7 +1341,7 @@ hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s,
         if (e != hipSuccess) return e;
     }
     if (d.det_blocks > 0)
-        launch_stage(ev, kStDetect, k_detect, dim3(d.det_blocks), dim3(256), s, d.frames, d.det_jobs, d.det_njobs,
+        launch_stage(ev, kStDetect, k_detect, dim3(d.det_blocks), dim3(256), s, d.frames, d.det_jobs, d.det_map,
                      d.t, d.epochs);
     if (d.nms_blocks > 0)
         launch_stage(ev, kStNms, k_nms, dim3((d.nms_blocks + 3) / 4), dim3(256), s, d.frames, d.nms_block_frame,

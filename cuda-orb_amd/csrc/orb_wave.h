@@ -36,6 +36,18 @@ __device__ __forceinline__ int lanes_below(uint64_t m)
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+/* Sum over the 16 lanes of this lane's row (lanes 16r .. 16r+15), returned
+   to every lane of the row: two quad swaps, then the half-row and row
+   mirrors (DPP).  Every lane of the wavefront must be active. */
+__device__ __forceinline__ int row_sum16(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);   /* quad_perm [1,0,3,2] */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);   /* quad_perm [2,3,0,1] */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false);  /* row_half_mirror */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xf, 0xf, false);  /* row_mirror */
+    return v;
+}
+
 /* order-preserving key of a finite float (-0 and +0 share one key, above 0) */
 __device__ __forceinline__ uint32_t float_order_key(float f)
 {
