@@ -1089,7 +1089,9 @@ int orb_match_batch(orb_ctx* c, int npairs, orb::OrbPoint* const* pts1, const ui
         if (!pts1[i] || !desc1[i] || (cap2 > 0 && !desc2[i])) return fail(c, ORB_E_ARG, "pair %d: null pointer", i);
         hp[i] = MatchPair{desc1[i], desc2[i], d_n1 + i, d_n2 + i, pts1[i]};
     }
-    const int qcap = (cap1 + 63) & ~63;
+    const int qcap = match_pairs_qcap(cap1);
+    if ((long long)npairs * qcap > (1LL << 30))
+        return fail(c, ORB_E_CAPACITY, "%d pairs of %d queries exceed the batched matcher's workspace", npairs, cap1);
     int rc = ensure_partial(c, 0, npairs * qcap);
     if (rc != ORB_OK) return rc;
     const bool same = hp.size() == c->h_pairs.size() &&

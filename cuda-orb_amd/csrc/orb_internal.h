@@ -166,9 +166,12 @@ struct MatchPair {
     const int* n2;
     orb::OrbPoint* pts;
 };
-/* npairs popcount matches in one launch; state/done hold qcap queries (a multiple of 64) per pair */
+/* npairs reference-rule matches in one launch (FP4 MFMA); state/done hold
+   qcap = match_pairs_qcap(cap1) queries per pair (merge words all ones,
+   counters zero, as the launch leaves them) */
 hipError_t launch_match_pairs(const MatchPair* d_pairs, int npairs, int cap1, int cap2, int qcap,
                               unsigned long long* state, int* done, hipStream_t s, const StageEvents* ev);
+int match_pairs_qcap(int cap1);
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev);
 
 /* Allocation-free matching (orb_match_top2_noalloc): a file-scope device

@@ -750,30 +750,45 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
    at dx <= 0 && dy <= 0, with the reference's flat indexing (pitch p0).
    Detection marks every scored corner in a bitmap of the level-0 pixels, so
    the NMS only visits marked pixels: one wavefront per 2048-pixel row
-   segment reads its 64 bitmap words (and clears them for the next call),
-   compacts the marked pixels in x order, checks one per lane and writes the
-   survivors of the segment in x order. */
-/* The square of radius r <= R around x (R is the wavefront's largest radius,
-   so every lane loads the same (2R+1)^2 neighbourhood in one batch and
-   compares only inside its own radius). */
+   segment reads its 64 bitmap words (and clears them for the next call) and
+   compacts the marked pixels in x order.  A segment holds a few marked
+   pixels, so 16 lanes share one: lane `sub` of the group checks square row
+   dy = sub - r (and every 16th row after it for r > 7), the 16 lanes'
+   verdicts are combined by a ballot, and the survivors are written in x
+   order.  One neighbourhood row per lane keeps the wavefront small (many
+   wavefronts in flight to cover the scattered key reads). */
+
+/* Row dy of the square around x: suppressed by a neighbour at |dx| <= r?
+   (R >= r is the largest radius among the wavefront's pixels, so the R
+   loads are unrolled and issued together) */
 template <int R>
-__device__ __forceinline__ bool nms_suppressed(const ORB_GLOBAL unsigned long long* __restrict__ rowp, long long p0,
-                                               int x, uint32_t s, uint32_t epoch, int r)
+__device__ __forceinline__ bool nms_row_suppressed(const ORB_GLOBAL unsigned long long* __restrict__ row, int x,
+                                                   int dy, uint32_t s, uint32_t epoch, int r)
 {
+    unsigned long long nk[2 * R + 1];
+#pragma unroll
+    for (int dx = -R; dx <= R; dx++) nk[dx + R] = row[x + dx];
     bool sup = false;
 #pragma unroll
-    for (int dy = -R; dy <= R; dy++) {
-        unsigned long long nk[2 * R + 1];
-#pragma unroll
-        for (int dx = -R; dx <= R; dx++) nk[dx + R] = rowp[(long long)dy * p0 + x + dx];
-#pragma unroll
-        for (int dx = -R; dx <= R; dx++) {
-            if (dx == 0 && dy == 0) continue;
-            const uint32_t ns = (uint32_t)(nk[dx + R] >> 3) & 0x7fffffffu;
-            const bool live = (uint32_t)(nk[dx + R] >> kEpochShift) == epoch;
-            const bool inside = abs(dx) <= r && abs(dy) <= r;
-            sup |= inside && live && (ns > s || (ns == s && dx <= 0 && dy <= 0));
-        }
+    for (int dx = -R; dx <= R; dx++) {
+        const uint32_t ns = (uint32_t)(nk[dx + R] >> 3) & 0x7fffffffu;
+        const bool live = (uint32_t)(nk[dx + R] >> kEpochShift) == epoch;
+        const bool self = dx == 0 && dy == 0;
+        sup |= !self && abs(dx) <= r && live && (ns > s || (ns == s && dx <= 0 && dy <= 0));
+    }
+    return sup;
+}
+
+/* the same for any radius (scale-2 octaves >= 3 have radii above 4) */
+__device__ __forceinline__ bool nms_row_suppressed_any(const ORB_GLOBAL unsigned long long* __restrict__ row, int x,
+                                                       int dy, uint32_t s, uint32_t epoch, int r)
+{
+    bool sup = false;
+    for (int dx = -r; dx <= r; dx++) {
+        if (dx == 0 && dy == 0) continue;
+        const unsigned long long nk = row[x + dx];
+        const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
+        sup |= (uint32_t)(nk >> kEpochShift) == epoch && (ns > s || (ns == s && dx <= 0 && dy <= 0));
     }
     return sup;
 }
@@ -819,55 +834,58 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
     const uint32_t epoch = epochs[f];
     const ORB_GLOBAL unsigned long long* rowp = as_global(F.keys) + (long long)y * F.p0;
     ORB_GLOBAL Candidate* out = as_global(cand) + (long long)u * kNmsCap;
+    const int g = lane >> 4, sub = lane & 15;
     int nout = 0;
-    for (int j0 = 0; j0 < total; j0 += 64) {
-        const int j = j0 + lane;
-        bool surv = false;
-        Candidate cd;
-        cd.pos = 0;
-        cd.score = 0;
-        /* the lane's marked pixel and its key's radius (0: no live key) */
-        int x = 0, r = 0;
-        uint32_t s = 0, lv = 0;
-        if (j < total) {
-            x = s_x[wid][j];
-            const unsigned long long key = rowp[x];
+    for (int c0 = 0; c0 < total; c0 += 64) {
+        /* one lane per marked pixel of the chunk: its key, score and radius
+           (radius 0: no live key) */
+        const int nchunk = min(64, total - c0);
+        int xj = 0, rj = 0;
+        uint32_t sj = 0, posj = 0;
+        if (lane < nchunk) {
+            xj = s_x[wid][c0 + lane];
+            const unsigned long long key = rowp[xj];
             if ((uint32_t)(key >> kEpochShift) == epoch) {
-                s = (uint32_t)(key >> 3) & 0x7fffffffu;
-                lv = 7u - (uint32_t)(key & 7u);
-                r = F.radius[lv];
+                sj = (uint32_t)(key >> 3) & 0x7fffffffu;
+                const uint32_t lv = 7u - (uint32_t)(key & 7u);
+                rj = F.radius[lv];
+                posj = (lv << 28) | ((uint32_t)y << 14) | (uint32_t)xj;
             }
         }
-        int rw = r;
+        int rw = rj;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) rw = max(rw, __shfl_xor(rw, o, 64));
-        bool sup = false;
-        if (r > 0) {
-            switch (rw) {
-            case 1: sup = nms_suppressed<1>(rowp, F.p0, x, s, epoch, r); break;
-            case 2: sup = nms_suppressed<2>(rowp, F.p0, x, s, epoch, r); break;
-            case 3: sup = nms_suppressed<3>(rowp, F.p0, x, s, epoch, r); break;
-            case 4: sup = nms_suppressed<4>(rowp, F.p0, x, s, epoch, r); break;
-            default:   /* radii above 4 (scale-2 octaves >= 3) */
-                for (int dy = -r; dy <= r; dy++)
-                    for (int dx = -r; dx <= r; dx++) {
-                        if (dx == 0 && dy == 0) continue;
-                        const unsigned long long nk = rowp[(long long)dy * F.p0 + x + dx];
-                        const uint32_t ns = (uint32_t)(nk >> 3) & 0x7fffffffu;
-                        if ((uint32_t)(nk >> kEpochShift) == epoch && (ns > s || (ns == s && dx <= 0 && dy <= 0)))
-                            sup = true;
+        /* groups of 16 lanes take the chunk's pixels 4 at a time, in x order */
+        for (int q = 0; q < nchunk; q += 4) {
+            const int src = q + g;
+            const int x = __shfl(xj, src, 64), r = __shfl(rj, src, 64);
+            const uint32_t sc = (uint32_t)__shfl((int)sj, src, 64);
+            const uint32_t pw = (uint32_t)__shfl((int)posj, src, 64);
+            const bool mine = src < nchunk && r > 0;
+            bool sup = false;
+            if (mine) {
+                for (int dy = sub - r; dy <= r; dy += 16) {
+                    const ORB_GLOBAL unsigned long long* rr = rowp + (long long)dy * F.p0;
+                    switch (rw) {
+                    case 1: sup |= nms_row_suppressed<1>(rr, x, dy, sc, epoch, r); break;
+                    case 2: sup |= nms_row_suppressed<2>(rr, x, dy, sc, epoch, r); break;
+                    case 3: sup |= nms_row_suppressed<3>(rr, x, dy, sc, epoch, r); break;
+                    case 4: sup |= nms_row_suppressed<4>(rr, x, dy, sc, epoch, r); break;
+                    default: sup |= nms_row_suppressed_any(rr, x, dy, sc, epoch, r);
                     }
+                }
             }
-            if (!sup) {
-                surv = true;
-                cd.pos = (lv << 28) | ((uint32_t)y << 14) | (uint32_t)x;
-                cd.score = s;
+            const unsigned long long m = __ballot(sup);
+            const bool surv = mine && sub == 0 && ((m >> (16 * g)) & 0xffffull) == 0;
+            const unsigned long long ms = __ballot(surv);
+            if (surv) {
+                Candidate cd;
+                cd.pos = pw;
+                cd.score = sc;
+                out[nout + lanes_below(ms)] = cd;
             }
+            nout += __popcll(ms);
         }
-        const unsigned long long m = __ballot(surv);
-        if (surv)
-            out[nout + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0))] = cd;
-        nout += __popcll(m);
     }
     if (lane == 0) seg_count[u] = nout;
 }

@@ -17,7 +17,7 @@
 namespace orbk
 {
 
-constexpr int kLxRow = 256 + 32; /* LDS row of one unpacked entry (padded) */
+constexpr int kLxRow = kTileRow;  /* LDS row of one unpacked entry */
 constexpr int kLxTile = 64;      /* entries per LDS tile */
 constexpr int kLxQ = 256;        /* queries per unit */
 
@@ -295,9 +295,8 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
 #pragma unroll
             for (int ab = 0; ab < 4; ab++) {
                 v4i a[4];
-                const int8_t* src = s_db + (ab * 16 + col) * kLxRow + 16 * g;
 #pragma unroll
-                for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(src + 64 * ks);
+                for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(s_db + lds_chunk(ab * 16 + col, 4 * ks + g));
                 const int jb = jt + ab * 16 + 4 * g;
 #pragma unroll
                 for (int qb = 0; qb < 4; qb++) {
@@ -322,11 +321,10 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
             __syncthreads();
             {
                 const uint32_t lo2 = nv.x, hi2 = nv.y;
-                int8_t* dst = s_db + e * kLxRow + qq * 64;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t hw = k < 2 ? (lo2 >> (16 * k)) & 0xffffu : (hi2 >> (16 * (k - 2))) & 0xffffu;
-                    *(v4i*)(dst + 16 * k) = pm1_16(hw);
+                    *(v4i*)(s_db + lds_chunk(e, 4 * qq + k)) = pm1_16(hw);
                 }
                 const int jn = jt + kLxTile + e;
                 nv = make_uint2(0u, 0u);

@@ -37,9 +37,8 @@ namespace orbk
 constexpr int kGranule = 64;          /* database entries per chunk granule (popcount: 16 per wavefront) */
 constexpr int kMinChunkGranules = 2;  /* at least 128 entries per chunk: bounds the merge work */
 constexpr int kPopQ = 64;             /* popcount: queries per wavefront */
-constexpr int kMfmaQ = 256;           /* MFMA: queries per workgroup */
-constexpr int kMfmaRow = 256 + 32;    /* LDS row of one unpacked entry (padded) */
-
+constexpr int kMfmaQ = 512;           /* MFMA: queries per workgroup */
+constexpr int kMfmaRow = 128 + 16;    /* LDS row of one unpacked entry: 8 chunks of 32 FP4 codes, padded */
 /* Split nq queries x ndb entries into qunits x chunks units for `slots`
    parallel workers (identical on host and device). */
 struct Plan {
@@ -224,123 +223,259 @@ __global__ __launch_bounds__(256) void k_match_pop(const uint8_t* __restrict__ q
     match_pop_units<H2>(q, nq, db, ndb, slots, blockIdx.x, gridDim.x, state, done, idx1, d1, idx2, d2, pts);
 }
 
-/* Many independent pairs in one launch: grid row y is pair y, with its own
-   device-side plan over the grid row's workgroups and its own merge words
-   and tile counters (qcap queries' worth per pair). */
-__global__ __launch_bounds__(256) void k_match_pop_pairs(const MatchPair* __restrict__ pairs, int cap1, int cap2,
-                                                         int qcap, unsigned long long* __restrict__ state,
-                                                         int* __restrict__ done)
-{
-    const MatchPair pr = pairs[blockIdx.y];
-    const int nq = min(max(*pr.n1, 0), cap1), ndb = min(max(*pr.n2, 0), cap2);
-    match_pop_units<false>(pr.q, nq, (const uint4*)pr.db, ndb, gridDim.x, blockIdx.x, gridDim.x,
-                           state + (long long)blockIdx.y * qcap, done + (long long)blockIdx.y * (qcap / kPopQ),
-                           nullptr, nullptr, nullptr, nullptr, pr.pts);
-}
-
 /* --------------------------------------------------------------- MFMA */
 
-__global__ __launch_bounds__(256) void k_match_mfma(const uint8_t* __restrict__ q, int nq_cap,
-                                                    const int* __restrict__ d_nq, const uint8_t* __restrict__ db,
-                                                    int ndb_cap, const int* __restrict__ d_ndb, int slots,
-                                                    uint2* __restrict__ partial)
+/* One MFMA unit of a workgroup of NW wavefronts: the 64*NW queries of tile
+   qu (query qu*64*NW + wid*64 + qb*16 + col in lane col of k-group g)
+   against database entries [j0, j1).  On return the top-2 keys of each query
+   are merged over its 4 k-groups (every lane of a query holds them) and qidx
+   holds the query indices.  Called by the whole workgroup (it synchronises
+   through s_db, 2 tiles of kGranule * kMfmaRow bytes). */
+template <int NW>
+__device__ __forceinline__ void mfma_unit(const uint8_t* __restrict__ q, int nq, const uint8_t* __restrict__ db,
+                                          int qu, int j0, int j1, int8_t* __restrict__ s_db,
+                                          const uint32_t* __restrict__ s_code, uint32_t (&k1)[4],
+                                          uint32_t (&k2)[4], int (&qidx)[4])
 {
-    __shared__ __align__(16) int8_t s_db[kGranule * kMfmaRow];
-    const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
-    const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
-    const Plan P = make_plan(nq, ndb, kMfmaQ, slots);
+    constexpr int kThreads = 64 * NW, kWordsPerThread = kGranule * 8 / kThreads;
+    static_assert(kWordsPerThread * kThreads == kGranule * 8, "whole descriptor words per thread");
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int g = lane >> 4, col = lane & 15;
-    for (int u = blockIdx.x; u < P.qunits * P.chunks; u += slots) {
-        const int qu = u % P.qunits, c = u / P.qunits;
-        /* B operands: query (qb*16 + col) of this wave's 64, k-group g, negated */
-        v4i bq[4][4];
-        /* bnd[qb]: a result can enter the top-2 only if its -dot is at most
-           bnd = (k2 >> 23) - popcount(query), the current second distance
-           as a -dot; "no key" admits every result until two are held */
-        int bnd[4];
-        int qidx[4];
+    /* B operands: query (qb*16 + col) of this wave's 64, descriptor word
+       4*ks + g (bits 128ks + 32g .. +31), negated (-1 where a bit is set) */
+    v4i bq[4][2];
+    /* bnd[qb]: a result can enter the top-2 only if its -dot is at most
+       bnd = (k2 >> 23) - popcount(query), the current second distance
+       as a -dot; "no key" admits every result until two are held.  The
+       accumulators start at 256, so they hold 256 - dot >= 0, whose float
+       bit patterns order like the values: the test is an unsigned compare
+       with bbits[qb], the bits of 256 + bnd */
+    int bnd[4];
+    uint32_t bbits[4];
 #pragma unroll
-        for (int qb = 0; qb < 4; qb++) {
-            const int qi = qu * kMfmaQ + wid * 64 + qb * 16 + col;
-            qidx[qb] = qi;
-            uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
-            if (qi < nq) {
-                const uint4* qp = (const uint4*)(q + (long long)qi * 32);
-                d0 = qp[0];
-                d1 = qp[1];
-            }
-            query_operands_neg(d0, d1, g, bq[qb]);
-            bnd[qb] = neg_dot_bound(popc256(d0, d1), kNoKey);
+    for (int qb = 0; qb < 4; qb++) {
+        const int qi = qu * kThreads + wid * 64 + qb * 16 + col;
+        qidx[qb] = qi;
+        uint4 d0 = make_uint4(0, 0, 0, 0), d1 = d0;
+        if (qi < nq) {
+            const uint4* qp = (const uint4*)(q + (long long)qi * 32);
+            d0 = qp[0];
+            d1 = qp[1];
         }
-        uint32_t k1[4], k2[4];
+        const uint32_t w[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
 #pragma unroll
-        for (int qb = 0; qb < 4; qb++) {
-            k1[qb] = k2[qb] = kNoKey;
+        for (int ks = 0; ks < 2; ks++) bq[qb][ks] = fp4_word(w[4 * ks + g], fp4_neg_pm1_8);
+        bnd[qb] = neg_dot_bound(popc256(d0, d1), kNoKey);
+        bbits[qb] = __float_as_uint((float)(256 + bnd[qb]));
+        k1[qb] = k2[qb] = kNoKey;
+    }
+    const v4f c256 = {256.f, 256.f, 256.f, 256.f};
+    /* unpack 64 entries (512 descriptor words) per tile: word tid + k*kThreads
+       is word (tid + k*kThreads) % 8 of entry (tid + k*kThreads) / 8, row
+       chunk of the same number, coded through the byte table.  Two LDS tile
+       buffers: tile i+1 is unpacked into one while tile i is read from the
+       other, so one barrier per tile separates them; the words of tile i+2
+       are loaded meanwhile, through a buffer resource ending at entry j1
+       (reads past it return 0) */
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)db, 0, j1 * 32, 0x00020000);
+    auto fetch = [&](int j, uint32_t (&v)[kWordsPerThread]) {
+#pragma unroll
+        for (int k = 0; k < kWordsPerThread; k++) {
+            const int wi = tid + k * kThreads;
+            v[k] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)((j + (wi >> 3)) * 32 + (wi & 7) * 4), 0, 0);
         }
-        const int j0 = c * P.chunk, j1 = min(ndb, j0 + P.chunk);
-        /* unpack 64 entries per tile: thread -> entry tid/4, bits 64*(tid%4) .. +63,
-           the bytes loaded a tile ahead */
-        const int e = tid >> 2, qq = tid & 3;
-        uint2 nv = make_uint2(0u, 0u);
-        if (j0 + e < j1) nv = *(const uint2*)(db + (long long)(j0 + e) * 32 + qq * 8);
-        for (int jt = j0; jt < j1; jt += kGranule) {
-            __syncthreads();
-            {
-                const uint32_t lo = nv.x, hi = nv.y;
-                int8_t* dst = s_db + e * kMfmaRow + qq * 64;
+    };
+    auto unpack = [&](const uint32_t (&v)[kWordsPerThread], int8_t* buf) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t hword = k < 2 ? (lo >> (16 * k)) & 0xffffu : (hi >> (16 * (k - 2))) & 0xffffu;
-                    *(v4i*)(dst + 16 * k) = b01_16(hword);
-                }
-                const int jn = jt + kGranule + e;
-                nv = make_uint2(0u, 0u);
-                if (jn < j1) nv = *(const uint2*)(db + (long long)jn * 32 + qq * 8);
+        for (int k = 0; k < kWordsPerThread; k++) {
+            const int wi = tid + k * kThreads;
+            *(v4i*)(buf + (wi >> 3) * kMfmaRow + 16 * (wi & 7)) = fp4_word_lut(v[k], s_code);
+        }
+    };
+    uint32_t nv[kWordsPerThread];
+    __syncthreads();   /* the buffers' previous readers are done */
+    fetch(j0, nv);
+    unpack(nv, s_db);
+    fetch(j0 + kGranule, nv);
+    __syncthreads();
+    int cur = 0;
+    for (int jt = j0; jt < j1; jt += kGranule, cur ^= 1) {
+        const int8_t* tile = s_db + cur * kGranule * kMfmaRow;
+        if (jt + kGranule < j1) {
+            unpack(nv, s_db + (cur ^ 1) * kGranule * kMfmaRow);
+            fetch(jt + 2 * kGranule, nv);
+        }
+#pragma unroll
+        for (int ab = 0; ab < 4; ab++) {
+            /* A operands: entry (ab*16 + col) of the tile, words 4*ks + g */
+            v4i a[2];
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) a[ks] = *(const v4i*)(tile + (ab * 16 + col) * kMfmaRow + 16 * (4 * ks + g));
+            /* lane holds C[4g + r][col]: entries ab*16 + 4g + r, query qb*16 + col.
+               The four query blocks' chains are interleaved and their results
+               tested after all eight MFMAs */
+            const int jb = jt + ab * 16 + 4 * g;
+            v4f acc[4];
+#pragma unroll
+            for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_fp4(a[0], bq[qb][0], c256);
+#pragma unroll
+            for (int qb = 0; qb < 4; qb++) acc[qb] = mfma_fp4(a[1], bq[qb][1], acc[qb]);
+            /* a -dot above bnd means a distance above the second one, a key
+               above k2, for all 4 entries: one test for the 4 query blocks
+               (the minima are taken by every lane, outside any lane-dependent
+               branch, so each MFMA runs with every lane enabled), and the
+               rare lanes that pass update their top-2 */
+            bool in[4];
+            bool any = false;
+#pragma unroll
+            for (int qb = 0; qb < 4; qb++) {
+                const uint32_t m = min(min(__float_as_uint(acc[qb][0]), __float_as_uint(acc[qb][1])),
+                                       min(__float_as_uint(acc[qb][2]), __float_as_uint(acc[qb][3])));
+                in[qb] = m <= bbits[qb];
+                any |= in[qb];
             }
-            __syncthreads();
-#pragma unroll
-            for (int ab = 0; ab < 4; ab++) {
-                /* A operands: entry (ab*16 + col) of the tile, k-group g */
-                v4i a[4];
-                const int8_t* src = s_db + (ab * 16 + col) * kMfmaRow + 16 * g;
-#pragma unroll
-                for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(src + 64 * ks);
-                /* lane holds C[4g + r][col]: entries ab*16 + 4g + r, query qb*16 + col */
-                const int jb = jt + ab * 16 + 4 * g;
+            if (any) {
 #pragma unroll
                 for (int qb = 0; qb < 4; qb++) {
-                    v4i acc = {0, 0, 0, 0};
-#pragma unroll
-                    for (int ks = 0; ks < 4; ks++)
-                        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
-                    /* a -dot above bnd means a distance above the second one,
-                       a key above k2, for all 4 entries */
-                    const int m = min(min(acc[0], acc[1]), min(acc[2], acc[3]));
-                    if (m <= bnd[qb]) {
+                    if (in[qb]) {
                         /* (k2 >> 23) - bnd is the query's popcount */
                         const int qpop = (int)(k2[qb] >> 23) - bnd[qb];
                         const uint32_t base = ((uint32_t)qpop << 23) + (uint32_t)jb;
 #pragma unroll
                         for (int r = 0; r < 4; r++)
-                            if (jb + r < j1) top2_push(k1[qb], k2[qb], neg_dot_key(acc[r], base + (uint32_t)r));
+                            if (jb + r < j1)
+                                top2_push(k1[qb], k2[qb], neg_dot_key((int)acc[qb][r] - 256, base + (uint32_t)r));
                         bnd[qb] = neg_dot_bound(qpop, k2[qb]);
+                        bbits[qb] = __float_as_uint((float)(256 + bnd[qb]));
                     }
                 }
             }
         }
-        /* merge the 4 k-groups that hold the same query */
+        __syncthreads();   /* tile read, next tile written */
+    }
+    /* merge the 4 k-groups that hold the same query */
 #pragma unroll
-        for (int qb = 0; qb < 4; qb++) {
+    for (int qb = 0; qb < 4; qb++) {
 #pragma unroll
-            for (int m = 16; m <= 32; m <<= 1) {
-                const uint32_t o1 = __shfl_xor(k1[qb], m, 64), o2 = __shfl_xor(k2[qb], m, 64);
-                top2_push(k1[qb], k2[qb], o1);
-                top2_push(k1[qb], k2[qb], o2);
-            }
-            if (g == 0 && qidx[qb] < nq) partial[(long long)c * nq + qidx[qb]] = make_uint2(k1[qb], k2[qb]);
+        for (int m = 16; m <= 32; m <<= 1) {
+            const uint32_t o1 = __shfl_xor(k1[qb], m, 64), o2 = __shfl_xor(k2[qb], m, 64);
+            top2_push(k1[qb], k2[qb], o1);
+            top2_push(k1[qb], k2[qb], o2);
         }
     }
+}
+
+/* dense matching: 512 queries per workgroup (8 wavefronts share each
+   unpacked tile) */
+constexpr int kMfmaWaves = 8;
+static_assert(kMfmaQ == 64 * kMfmaWaves, "queries per MFMA workgroup");
+
+__global__ __launch_bounds__(64 * kMfmaWaves) void k_match_mfma(const uint8_t* __restrict__ q, int nq_cap,
+                                                                const int* __restrict__ d_nq,
+                                                                const uint8_t* __restrict__ db, int ndb_cap,
+                                                                const int* __restrict__ d_ndb, int slots,
+                                                                uint2* __restrict__ partial)
+{
+    __shared__ __align__(16) int8_t s_db[2 * kGranule * kMfmaRow];
+    __shared__ uint32_t s_code[256];
+    fill_code_table(s_code, threadIdx.x, blockDim.x);   /* visible after mfma_unit's first barrier */
+    const int nq = d_nq ? min(*d_nq, nq_cap) : nq_cap;
+    const int ndb = d_ndb ? min(*d_ndb, ndb_cap) : ndb_cap;
+    const Plan P = make_plan(nq, ndb, kMfmaQ, slots);
+    const int g = (threadIdx.x & 63) >> 4;
+    for (int u = blockIdx.x; u < P.qunits * P.chunks; u += slots) {
+        const int qu = u % P.qunits, c = u / P.qunits;
+        uint32_t k1[4], k2[4];
+        int qidx[4];
+        mfma_unit<kMfmaWaves>(q, nq, db, qu, c * P.chunk, min(ndb, c * P.chunk + P.chunk), s_db, s_code, k1, k2,
+                              qidx);
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++)
+            if (g == 0 && qidx[qb] < nq) partial[(long long)c * nq + qidx[qb]] = make_uint2(k1[qb], k2[qb]);
+    }
+}
+
+/* Batched reference-rule matching on the matrix cores: blockIdx.y is the
+   pair and blockIdx.x one (query tile, database chunk) unit of it: kPairQ
+   queries against kPairChunk entries.  Chunking bounds the launch by a
+   chunk rather than by the largest pair's whole database.  The chunks of a
+   tile meet as in the popcount matcher: one 64-bit word per query
+   ((k1 << 32) | k2, merged by compare-and-swap at device scope) and one
+   counter per tile; the unit that finishes the tile's last chunk reads the
+   words (resetting them to "no key"), writes the matches and resets the
+   counter, so the words and counters are clean for the next call.  A pair
+   whose database fits one chunk writes its matches directly. */
+constexpr int kPairWaves = 4, kPairQ = 64 * kPairWaves;
+constexpr int kPairChunk = 16 * kGranule;
+
+/* merge (k1, k2) into a query's merge word */
+__device__ __forceinline__ void merge_word(unsigned long long* wp, uint32_t k1, uint32_t k2)
+{
+    unsigned long long old = __hip_atomic_load(wp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        const uint32_t o1 = (uint32_t)(old >> 32), o2 = (uint32_t)old;
+        uint32_t n1 = o1, n2 = o2;
+        top2_push(n1, n2, k1);
+        top2_push(n1, n2, k2);
+        if (n1 == o1 && n2 == o2) break;
+        if (__hip_atomic_compare_exchange_strong(wp, &old, ((unsigned long long)n1 << 32) | n2, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+    }
+}
+
+__global__ __launch_bounds__(64 * kPairWaves) void k_match_mfma_pairs(const MatchPair* __restrict__ pairs, int cap1,
+                                                                      int cap2, int qcap,
+                                                                      unsigned long long* __restrict__ state,
+                                                                      int* __restrict__ done)
+{
+    __shared__ __align__(16) int8_t s_db[2 * kGranule * kMfmaRow];
+    __shared__ uint32_t s_code[256];
+    __shared__ int s_last;
+    const MatchPair pr = pairs[blockIdx.y];
+    const int nq = min(max(*pr.n1, 0), cap1), ndb = min(max(*pr.n2, 0), cap2);
+    const int qtiles = (cap1 + kPairQ - 1) / kPairQ;
+    const int qu = blockIdx.x % qtiles, c = blockIdx.x / qtiles;
+    const int nch = ndb > 0 ? (ndb + kPairChunk - 1) / kPairChunk : 1;
+    if (qu * kPairQ >= nq || c >= nch) return;
+    if (ndb == 0) {
+        const int qi = qu * kPairQ + threadIdx.x;
+        if (qi < nq) write_match(qi, kNoKey, kNoKey, nullptr, nullptr, nullptr, nullptr, pr.pts);
+        return;
+    }
+    fill_code_table(s_code, threadIdx.x, blockDim.x);   /* visible after mfma_unit's first barrier */
+    uint32_t k1[4], k2[4];
+    int qidx[4];
+    mfma_unit<kPairWaves>(pr.q, nq, pr.db, qu, c * kPairChunk, min(ndb, (c + 1) * kPairChunk), s_db, s_code, k1, k2,
+                          qidx);
+    const int g = (threadIdx.x & 63) >> 4;
+    if (nch == 1) {
+#pragma unroll
+        for (int qb = 0; qb < 4; qb++)
+            if (g == 0 && qidx[qb] < nq)
+                write_match(qidx[qb], k1[qb], k2[qb], nullptr, nullptr, nullptr, nullptr, pr.pts);
+        return;
+    }
+    unsigned long long* words = state + (long long)blockIdx.y * qcap;
+#pragma unroll
+    for (int qb = 0; qb < 4; qb++)
+        if (g == 0 && qidx[qb] < nq) merge_word(words + qidx[qb], k1[qb], k2[qb]);
+    /* every merge of the workgroup has completed before the counter moves */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    int* cnt = done + ((long long)blockIdx.y * qcap + qu * kPairQ) / 64;
+    if (threadIdx.x == 0)
+        s_last = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nch - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int qi = qu * kPairQ + threadIdx.x;
+    if (qi < nq) {
+        const unsigned long long v = __hip_atomic_exchange(words + qi, ~0ull, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+        write_match(qi, (uint32_t)(v >> 32), (uint32_t)v, nullptr, nullptr, nullptr, nullptr, pr.pts);
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* ------------------------------------------------------------- merge */
@@ -432,16 +567,18 @@ hipError_t launch_match_pairs(const MatchPair* d_pairs, int npairs, int cap1, in
 {
     (void)hipGetLastError(); /* report only errors of these launches */
     if (npairs <= 0 || cap1 <= 0) return hipSuccess;
-    /* the workgroups one pair's capacities allow, a share of kPopBlocks each, at least 16 */
-    const Plan P = make_plan(cap1, cap2, kPopQ, kPopBlocks);
-    const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
-    int per = kPopBlocks / npairs;
-    if (per < 16) per = 16;
-    if (units < per) per = (int)units;
-    if (per < 1) per = 1;
-    launch_stage(ev, kStMatch, k_match_pop_pairs, dim3(per, npairs), dim3(256), s, d_pairs, cap1, cap2, qcap, state,
-                 done);
+    /* one workgroup per (query tile, database chunk) the capacities allow;
+       units past a pair's counts return at once */
+    const int qtiles = (cap1 + kPairQ - 1) / kPairQ;
+    const int chunks = cap2 > 0 ? (cap2 + kPairChunk - 1) / kPairChunk : 1;
+    launch_stage(ev, kStMatch, k_match_mfma_pairs, dim3(qtiles * chunks, npairs), dim3(kPairQ), s, d_pairs, cap1,
+                 cap2, qcap, state, done);
     return hipGetLastError();
+}
+
+int match_pairs_qcap(int cap1)
+{
+    return (cap1 + kPairQ - 1) / kPairQ * kPairQ;
 }
 
 hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEvents* ev)
@@ -453,7 +590,7 @@ hipError_t launch_match_mfma(const MatchLaunch& m, hipStream_t s, const StageEve
     const long long units = (long long)P.qunits * (P.chunks > 0 ? P.chunks : 1);
     int blocks = units < kMfmaBlocks ? (int)units : kMfmaBlocks;
     if (blocks < 1) blocks = 1;
-    launch_stage(ev, kStMatch, k_match_mfma, dim3(blocks), dim3(256), s, m.q, m.nq, m.d_nq, m.db, m.ndb, m.d_ndb,
+    launch_stage(ev, kStMatch, k_match_mfma, dim3(blocks), dim3(kMfmaQ), s, m.q, m.nq, m.d_nq, m.db, m.ndb, m.d_ndb,
                  slots, (uint2*)m.partial);
     launch_merge(m, kMfmaQ, slots, s, ev);
     return hipGetLastError();

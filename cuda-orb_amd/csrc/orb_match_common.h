@@ -25,6 +25,94 @@ constexpr uint32_t kIdxMask = (1u << kIdxBits) - 1u;
 constexpr uint32_t kNoKey = 0xffffffffu;
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+/* ---- FP4 operands.  A descriptor bit becomes one E2M1 code (4 bits):
+   0x0 = 0, 0x2 = +1.0, 0xA = -1.0, so one 128-deep FP4 MFMA contracts 128
+   bits.  Products lie in {-1, 0, 1} and a contraction sums at most 256 of
+   them, so the FP32 accumulator holds the exact integer. */
+
+/* bits 0..7 of x to bits 0, 4, ..., 28 (one per nibble) */
+__device__ __forceinline__ uint32_t spread_nibbles(uint32_t x)
+{
+    x &= 0xffu;
+    x = (x | (x << 12)) & 0x000f000fu;
+    x = (x | (x << 6)) & 0x03030303u;
+    x = (x | (x << 3)) & 0x11111111u;
+    return x;
+}
+
+/* 8 bits -> 8 codes of 1 (bit set) / 0 (clear) */
+__device__ __forceinline__ uint32_t fp4_b01_8(uint32_t x)
+{
+    return spread_nibbles(x) << 1;
+}
+
+/* 8 bits -> 8 codes of +1 (bit set) / -1 (clear) */
+__device__ __forceinline__ uint32_t fp4_pm1_8(uint32_t x)
+{
+    return 0x22222222u | (spread_nibbles(~x) << 3);
+}
+
+/* 8 bits -> 8 codes of -1 (bit set) / +1 (clear) */
+__device__ __forceinline__ uint32_t fp4_neg_pm1_8(uint32_t x)
+{
+    return 0x22222222u | (spread_nibbles(x) << 3);
+}
+
+/* one 32-bit descriptor word -> 32 codes (16 bytes): code i (bits 4i..4i+3
+   of the 16 bytes) is bit i of w */
+template <typename F>
+__device__ __forceinline__ v4i fp4_word(uint32_t w, F code8)
+{
+    v4i r;
+    r.x = (int)code8(w);
+    r.y = (int)code8(w >> 8);
+    r.z = (int)code8(w >> 16);
+    r.w = (int)code8(w >> 24);
+    return r;
+}
+
+/* FP4 codes of every byte value: entry b is fp4_b01_8(b) (LDS lookup
+   tables are filled from this at kernel start) */
+__device__ __forceinline__ void fill_code_table(uint32_t* table, int tid, int nthreads)
+{
+    for (int b = tid; b < 256; b += nthreads) table[b] = fp4_b01_8((uint32_t)b);
+}
+
+/* 32 codes of a descriptor word from a 256-entry byte table in LDS */
+__device__ __forceinline__ v4i fp4_word_lut(uint32_t w, const uint32_t* table)
+{
+    v4i r;
+    r.x = (int)table[w & 0xffu];
+    r.y = (int)table[(w >> 8) & 0xffu];
+    r.z = (int)table[(w >> 16) & 0xffu];
+    r.w = (int)table[w >> 24];
+    return r;
+}
+
+/* C + A x B, 16 x 16 x 128 on FP4 operands (unit scales), FP32 accumulate.
+   Lane (g = lane / 16, col = lane % 16) gives A row col and B column col,
+   elements 32g .. 32g + 31 of the 128 (the same positions in A and B). */
+__device__ __forceinline__ v4f mfma_fp4(v4i a, v4i b, v4f c)
+{
+    const v8i a8 = {a.x, a.y, a.z, a.w, 0, 0, 0, 0};
+    const v8i b8 = {b.x, b.y, b.z, b.w, 0, 0, 0, 0};
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
+}
+
+/* An unpacked 64-entry database tile in LDS: one row per entry, 16 chunks
+   of 16 bytes (the expansion of 16 descriptor bits each), rows padded to
+   288 bytes.  (Measured on the dense 65536 x 262144 match: rows of 272, 288
+   or 304 bytes run alike; 320-byte rows and every XOR swizzle of the chunks
+   within 256-byte rows tried ran 7-28 % slower.) */
+constexpr int kTileRow = 256 + 32;
+
+__device__ __forceinline__ int lds_chunk(int r, int c)
+{
+    return r * kTileRow + 16 * c;
+}
 
 /* k2 = med3(k1, key, k2) (k1 <= k2 always holds) and k1 = min(k1, key).
    Written as min/max so the compiler selects v_med3_u32 and v_min_u32
