@@ -1156,12 +1156,13 @@ __device__ __forceinline__ int sample_flat(const ORB_GLOBAL uint8_t* base, long 
     return (i >= 0 && i < total) ? base[i] : 0;
 }
 
+/* PLAIN: WTA_K 2 with the patch-31 table (every sample inside the staged
+   patch), the common case compiled on its own so it keeps fewer registers */
+template <bool PLAIN>
 __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ frames, int blocks_per_frame, Tables T,
                                                   int compute_desc)
 {
-    __shared__ int s_umax[kDescPerBlock][32];
     __shared__ uint8_t s_code[kDescPerBlock][128];
-    __shared__ __align__(16) uint8_t s_raw[kDescPerBlock][31 * kRawRow];
     __shared__ __align__(16) uint8_t s_blur[kDescPerBlock][37 * kBlurRow];
     /* consecutive keypoints (raster neighbours, overlapping patches) on one XCD */
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -1169,42 +1170,44 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
     const int lb = bid - f * blocks_per_frame;
     const FrameDev& F = frames[f];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int hk = T.half_patch, side = 2 * hk + 1;
-    /* the wavefronts work independently (no workgroup barrier): the count,
-       the first point, this lane's pattern points (WTA_K 2) and the wave's
-       copy of umax are all requested at once */
-    int pi = lb * kDescPerBlock + wid;
-    int px = 0, py = 0, pl = 0;
-    if (pi < F.cap) {
-        const ORB_GLOBAL orb::OrbPoint* P = as_global(F.pts) + pi;
-        px = P->x;
-        py = P->y;
-        pl = P->octave;
+    const int hk = T.half_patch;
+    /* The angle's moments come straight from the loaded raw patch: dword
+       i = lane + 64k of the 31 x 36-byte patch is row i / 9, bytes
+       4 (i % 9) .. +3.  Per dword, fixed for every keypoint: the row's
+       offset v from the centre, its umax bound (-1 outside the patch) and the
+       column offset of its first byte (less the keypoint's alignment) */
+    constexpr int kRawDw = kRawRow / 4, kRawN = 31 * kRawDw, kRawK = (kRawN + 63) / 64;
+    int rrow[kRawK], rcol[kRawK], rv[kRawK], rum[kRawK];
+#pragma unroll
+    for (int k = 0; k < kRawK; k++) {
+        const int i = lane + 64 * k, r = i / kRawDw, d = i - r * kRawDw;
+        rrow[k] = r;
+        rcol[k] = 4 * d;
+        rv[k] = r - 15;
+        rum[k] = (i < kRawN && abs(r - 15) <= hk) ? T.umax[abs(r - 15)] : -1;
     }
+    /* the wavefronts work independently (no workgroup barrier): the count,
+       the first point and this lane's pattern points (WTA_K 2) are all
+       requested at once */
+    int pi = lb * kDescPerBlock + wid;
     const int n = *as_global(F.count);
     int2 pat2[8];
-    if (T.wta_k == 2) {
+    if (PLAIN || T.wta_k == 2) {
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             pat2[2 * j] = as_global(T.pattern)[2 * (lane + 64 * j)];
             pat2[2 * j + 1] = as_global(T.pattern)[2 * (lane + 64 * j) + 1];
         }
     }
-    if (lane < 17) s_umax[wid][lane] = T.umax[lane];
-    wave_sync_lds();
     for (; pi < n; pi += blocks_per_frame * kDescPerBlock) {
         ORB_GLOBAL orb::OrbPoint* P = as_global(F.pts) + pi;
-        if (pi >= lb * kDescPerBlock + wid + blocks_per_frame * kDescPerBlock) {
-            px = P->x;
-            py = P->y;
-            pl = P->octave;
-        }
-        const int l = pl;
+        const int px = __builtin_amdgcn_readfirstlane(P->x), py = __builtin_amdgcn_readfirstlane(P->y);
+        const int l = __builtin_amdgcn_readfirstlane(P->octave);
         const int cx = from_l0(F, T.tabs, l, px, false), cy = from_l0(F, T.tabs, l, py, true);
         const int p = F.lp[l];
         const long long co = F.loff[l] + (long long)cy * p + cx;
 
-        /* stage both patches with dword loads (every row shares the keypoint's
+        /* both patches with dword loads (every row shares the keypoint's
            alignment because the pitch is a multiple of 128) */
         const ORB_GLOBAL uint8_t* ra = as_global(F.raw) + co - 15 - 15LL * p;
         const int ro = (int)((uintptr_t)ra & 3);
@@ -1212,51 +1215,47 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
         const ORB_GLOBAL uint8_t* ba = as_global(F.blur) + co - 18 - 18LL * p;
         const int bo = (int)((uintptr_t)ba & 3);
         const ORB_GLOBAL uint8_t* bb = ba - bo;
-        {
-            /* every load of both patches in flight before the LDS writes */
-            constexpr int nraw = 31 * (kRawRow / 4), nblur = 37 * (kBlurRow / 4);
-            constexpr int praw = (nraw + 63) / 64, pblur = (nblur + 63) / 64;
-            uint32_t vr[praw], vb[pblur];
+        constexpr int nblur = 37 * (kBlurRow / 4), pblur = (nblur + 63) / 64;
+        uint32_t vr[kRawK], vb[pblur];
 #pragma unroll
-            for (int k = 0; k < praw; k++) {
-                const int i = lane + 64 * k;
-                const int r = i / (kRawRow / 4), d = i - r * (kRawRow / 4);
-                if (i < nraw) vr[k] = *(const ORB_GLOBAL uint32_t*)(rb + (long long)r * p + 4 * d);
-            }
+        for (int k = 0; k < kRawK; k++)
+            vr[k] = rum[k] >= 0 ? *(const ORB_GLOBAL uint32_t*)(rb + (uint32_t)(rrow[k] * p + rcol[k])) : 0u;
 #pragma unroll
-            for (int k = 0; k < pblur; k++) {
-                const int i = lane + 64 * k;
-                const int r = i / (kBlurRow / 4), d = i - r * (kBlurRow / 4);
-                if (compute_desc && i < nblur) vb[k] = *(const ORB_GLOBAL uint32_t*)(bb + (long long)r * p + 4 * d);
-            }
-#pragma unroll
-            for (int k = 0; k < praw; k++) {
-                const int i = lane + 64 * k;
-                if (i < nraw) *(uint32_t*)&s_raw[wid][4 * i] = vr[k];
-            }
-#pragma unroll
-            for (int k = 0; k < pblur; k++) {
-                const int i = lane + 64 * k;
-                if (compute_desc && i < nblur) *(uint32_t*)&s_blur[wid][4 * i] = vb[k];
-            }
+        for (int k = 0; k < pblur; k++) {
+            const int i = lane + 64 * k;
+            const int r = i / (kBlurRow / 4), d = i - r * (kBlurRow / 4);
+            if (compute_desc && i < nblur) vb[k] = *(const ORB_GLOBAL uint32_t*)(bb + (uint32_t)(r * p + 4 * d));
         }
-        wave_sync_lds();
+#pragma unroll
+        for (int k = 0; k < pblur; k++) {
+            const int i = lane + 64 * k;
+            if (compute_desc && i < nblur) *(uint32_t*)&s_blur[wid][4 * i] = vb[k];
+        }
 
-        /* intensity centroid over the umax disc of the raw level (orbd.cu:1011-1032) */
-        const uint8_t* rc = &s_raw[wid][15 * kRawRow + ro + 15];
+        /* intensity centroid over the umax disc of the raw level
+           (orbd.cu:1011-1032): byte b of dword k is column
+           u = rcol + b - ro - 15 of row v; integer sums, so their order does
+           not change them */
         int m10 = 0, m01 = 0;
-        for (int i = lane; i < side * side; i += 64) {
-            const int v = i / side - hk, u = i % side - hk;
-            if (abs(u) <= s_umax[wid][abs(v)]) {
-                const int I = rc[v * kRawRow + u];
-                m10 += u * I;
-                m01 += v * I;
+#pragma unroll
+        for (int k = 0; k < kRawK; k++) {
+            const int u0 = rcol[k] - ro - 15;
+            int su = 0, sI = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int u = u0 + b;
+                const int I = abs(u) <= rum[k] ? (int)((vr[k] >> (8 * b)) & 255u) : 0;
+                su += u * I;
+                sI += I;
             }
+            m10 += su;
+            m01 += rv[k] * sI;
         }
         m10 = wave_sum(m10);
         m01 = wave_sum(m01);
         const float ang = (m01 == 0 && m10 == 0) ? 0.f : dev_atan2(__int2float_rn(m01), __int2float_rn(m10));
         if (lane == 0) P->angle = ang;
+        wave_sync_lds();
         if (compute_desc) {
             float sn, cs;
             dev_sincos(ang, sn, cs);
@@ -1265,10 +1264,10 @@ __global__ __launch_bounds__(256) void k_describe(const FrameDev* __restrict__ f
                staged patch; the random points of other patch sizes sit off
                centre (orbd.cu:359-369) and are read from the flat pyramid
                (patch 30 has the same half patch as 31 but random points) */
-            const bool flat = !T.pattern31;
+            const bool flat = !PLAIN && !T.pattern31;
             const ORB_GLOBAL uint8_t* fb = as_global(F.blur);
             auto smp = [&](int2 pt) { return flat ? sample_flat(fb, F.total, co, p, pt, sn, cs) : sample_lds(bp, pt, sn, cs); };
-            if (T.wta_k == 2) {
+            if (PLAIN || T.wta_k == 2) {
                 /* bit k of the descriptor compares points 2k and 2k+1 (orbd.cu:1351-1361) */
                 ORB_GLOBAL unsigned long long* outd = (ORB_GLOBAL unsigned long long*)(as_global(F.desc) + (long long)pi * 32);
 #pragma unroll
@@ -1365,9 +1364,14 @@ hipError_t launch_detect(const DetectLaunch& d, int compute_desc, hipStream_t s,
         launch_stage(ev, kStNms, k_nms, dim3((d.nms_blocks + 3) / 4), dim3(256), s, d.frames, d.nms_block_frame,
                      d.nms_blocks, d.epochs, d.cand, d.blk_count);
     launch_stage(ev, kStSelect, k_select, dim3(d.nframes), dim3(1024), s, d.frames, d.cand, d.blk_count, d.epochs);
-    if (compute_desc && d.desc_blocks_per_frame > 0)
-        launch_stage(ev, kStDescribe, k_describe, dim3(d.desc_blocks_per_frame * d.nframes), dim3(256), s, d.frames,
-                     d.desc_blocks_per_frame, d.t, compute_desc);
+    if (compute_desc && d.desc_blocks_per_frame > 0) {
+        if (d.t.wta_k == 2 && d.t.pattern31)
+            launch_stage(ev, kStDescribe, k_describe<true>, dim3(d.desc_blocks_per_frame * d.nframes), dim3(256), s,
+                         d.frames, d.desc_blocks_per_frame, d.t, compute_desc);
+        else
+            launch_stage(ev, kStDescribe, k_describe<false>, dim3(d.desc_blocks_per_frame * d.nframes), dim3(256), s,
+                         d.frames, d.desc_blocks_per_frame, d.t, compute_desc);
+    }
     return hipGetLastError();
 }
 
