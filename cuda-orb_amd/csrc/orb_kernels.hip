@@ -752,11 +752,14 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
    the NMS only visits marked pixels: one wavefront per 2048-pixel row
    segment reads its 64 bitmap words (and clears them for the next call) and
    compacts the marked pixels in x order.  A segment holds a few marked
-   pixels, so 16 lanes share one: lane `sub` of the group checks square row
-   dy = sub - r (and every 16th row after it for r > 7), the 16 lanes'
-   verdicts are combined by a ballot, and the survivors are written in x
-   order.  One neighbourhood row per lane keeps the wavefront small (many
-   wavefronts in flight to cover the scattered key reads). */
+   pixels, so kNmsLanes lanes share one: lane `sub` of the group checks
+   square rows dy = sub - r, sub - r + kNmsLanes, ..., the group's verdicts
+   are combined by a ballot, and the survivors are written in x order.  One
+   neighbourhood row at a time per lane keeps the wavefront small (many
+   wavefronts in flight to cover the scattered key reads); 4 lanes per pixel
+   measured best against 8 and 16 on both the 1080p pair and the 128-frame
+   batch. */
+constexpr int kNmsLanes = 4;
 
 /* Row dy of the square around x: suppressed by a neighbour at |dx| <= r?
    (R >= r is the largest radius among the wavefront's pixels, so the R
@@ -834,7 +837,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
     const uint32_t epoch = epochs[f];
     const ORB_GLOBAL unsigned long long* rowp = as_global(F.keys) + (long long)y * F.p0;
     ORB_GLOBAL Candidate* out = as_global(cand) + (long long)u * kNmsCap;
-    const int g = lane >> 4, sub = lane & 15;
+    const int g = lane / kNmsLanes, sub = lane % kNmsLanes;
     int nout = 0;
     for (int c0 = 0; c0 < total; c0 += 64) {
         /* one lane per marked pixel of the chunk: its key, score and radius
@@ -855,8 +858,9 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
         int rw = rj;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) rw = max(rw, __shfl_xor(rw, o, 64));
-        /* groups of 16 lanes take the chunk's pixels 4 at a time, in x order */
-        for (int q = 0; q < nchunk; q += 4) {
+        /* groups of kNmsLanes lanes take the chunk's pixels 64 / kNmsLanes at a
+           time, in x order */
+        for (int q = 0; q < nchunk; q += 64 / kNmsLanes) {
             const int src = q + g;
             const int x = __shfl(xj, src, 64), r = __shfl(rj, src, 64);
             const uint32_t sc = (uint32_t)__shfl((int)sj, src, 64);
@@ -864,7 +868,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
             const bool mine = src < nchunk && r > 0;
             bool sup = false;
             if (mine) {
-                for (int dy = sub - r; dy <= r; dy += 16) {
+                for (int dy = sub - r; dy <= r; dy += kNmsLanes) {
                     const ORB_GLOBAL unsigned long long* rr = rowp + (long long)dy * F.p0;
                     switch (rw) {
                     case 1: sup |= nms_row_suppressed<1>(rr, x, dy, sc, epoch, r); break;
@@ -876,7 +880,7 @@ __global__ __launch_bounds__(256) void k_nms(const FrameDev* __restrict__ frames
                 }
             }
             const unsigned long long m = __ballot(sup);
-            const bool surv = mine && sub == 0 && ((m >> (16 * g)) & 0xffffull) == 0;
+            const bool surv = mine && sub == 0 && ((m >> (kNmsLanes * g)) & ((1ull << kNmsLanes) - 1)) == 0;
             const unsigned long long ms = __ballot(surv);
             if (surv) {
                 Candidate cd;
