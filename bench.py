@@ -70,7 +70,8 @@ COUNTERS_FILE = ROOT / "profiles" / "orb_1080p_pair_hbm_counters.md"
 def stage_kernel(stage, strip_height):
     """rocprofv3 kernel name of a stage's launch."""
     return {"pyramid": f"void k_pyramid<{strip_height}>", "detect": "k_detect", "nms": "k_nms", "select": "k_select",
-            "describe": "k_describe", "match": "void k_match_pop<false>", "match_merge": "k_match_merge"}.get(stage)
+            "describe": "void k_describe<true>", "match": "void k_match_pop<false>",
+            "match_merge": "k_match_merge"}.get(stage)
 
 
 def measured_traffic(kernel):
@@ -92,8 +93,10 @@ def measured_traffic(kernel):
 
 def dense_match(ctx, torch, dev, reps=3):
     """65536 queries against 262144 random 256-bit descriptors (queries = database rows with
-    Binomial(256, 0.05) bit flips): INT8-MFMA and popcount matchers, timed with events on the
-    library's stream, outputs compared element for element."""
+    Binomial(256, 0.05) bit flips): the matrix-core matcher (FP4 operands, one code per bit, FP32
+    accumulate) and the popcount matcher, timed with events on the library's stream, outputs
+    compared element for element.  int_tops_equiv counts 512 operations per (query, entry) pair
+    (256 multiply-adds), the contraction the matrix cores perform."""
     import orb_amd
     rng = np.random.default_rng(2)
     ndb, nq = 262144, 65536
@@ -124,7 +127,8 @@ def dense_match(ctx, torch, dev, reps=3):
     return {"queries": nq, "database": ndb, "popcount": res["popcount"], "mfma": res["mfma"],
             "mfma_speedup": round(res["popcount"]["ms"] / res["mfma"]["ms"], 3),
             "identical": identical, "idx1_is_source": round(recall_src, 5),
-            "mfma_frac_of_int8_peak": round(res["mfma"]["int_tops_equiv"] / 5000.0, 4)}
+            "mfma_frac_of_int8_peak": round(res["mfma"]["int_tops_equiv"] / 5000.0, 4),
+            "mfma_frac_of_fp4_peak": round(res["mfma"]["int_tops_equiv"] / 10100.0, 4)}
 
 
 def batch_frames(ctx, torch, dev, reps=5):
@@ -376,6 +380,10 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-matching measurement")
     ap.add_argument("--learned-only", action="store_true",
                     help="only the learned-matching measurement (its JSON object), e.g. under a profiler")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 128-frame batch measurement (its JSON object), e.g. under a profiler")
+    ap.add_argument("--dense-only", action="store_true",
+                    help="only the dense-matching measurement (its JSON object), e.g. under a profiler")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 64 MiB elementwise copy kernel first (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -395,9 +403,15 @@ def main():
             torch.bitwise_not(x, out=y)
         torch.cuda.synchronize()
         del x, y
-    if args.learned_only:
+    if args.learned_only or args.batch_only or args.dense_only:
         ctx = orb_amd.Context(0).init(noctaves=LEVELS, max_pts=CAP, scale_factor=SCALE)
-        print(json.dumps({"learned": learned_match(ctx, torch, dev)}), flush=True)
+        if args.learned_only:
+            out = {"learned": learned_match(ctx, torch, dev)}
+        elif args.batch_only:
+            out = {"batch64": batch_frames(ctx, torch, dev, reps=max(args.steps, 1))}
+        else:
+            out = {"dense_match": dense_match(ctx, torch, dev, reps=max(args.steps, 1))}
+        print(json.dumps(out), flush=True)
         ctx.close()
         return
     a = orb_amd.synth_image(SEED, W, H, NSHAPES)
