@@ -33,10 +33,12 @@ constexpr int kRefMaxLevels = 5;     /* MAX_OCTAVE, orbd.cu:18 */
 constexpr int kEpochShift = 34;
 constexpr uint32_t kEpochLimit = 1u << 30;
 
-/* pyramid unit: one wavefront per strip of 248 columns (62 lanes x 4) x TH
+/* pyramid unit: one wavefront per strip of 240 columns (60 lanes x 4, a
+   multiple of 16 so every 16-byte store of a row lies in one strip) x TH
    rows of one level; 4 units per 256-thread workgroup.  The strip heights
-   are the autotuner's variants (TH + 6 walked rows fit one wavefront). */
-constexpr int kPyrStripW = 248;
+   are the autotuner's variants (TH + 6 walked rows fit one wavefront; taller
+   strips, measured up to 122 rows, were slower at batch scale). */
+constexpr int kPyrStripW = 240;
 constexpr int kPyrBands = 8;         /* horizontal bands per frame in the unit order */
 constexpr int kPyrVariants = 5;
 constexpr int kPyrTHs[kPyrVariants] = {18, 26, 34, 42, 58};

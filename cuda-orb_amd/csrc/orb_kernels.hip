@@ -152,20 +152,32 @@ __device__ __forceinline__ int wave_sum(int v)
 
 /* ---------------------------------------------------------------- pyramid */
 
-/* value of the lane whose byte address (4 * lane) is given */
-__device__ __forceinline__ uint32_t lane_get(uint32_t v, int addr)
+/* v of lane i - 1 (lane 0: 0) and of lane i + 1 (lane 63: 0): DPP
+   wavefront shifts by one lane, a VALU move each (no LDS round trip) */
+__device__ __forceinline__ uint32_t from_lane_below(uint32_t v)
 {
-    return (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);   /* wave_shr:1 */
+}
+__device__ __forceinline__ uint32_t from_lane_above(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);   /* wave_shl:1 */
 }
 
 /* Loaded bytes of one level row for the 4 columns of a lane, kept apart from
    their use so the loads of later rows are in flight while a row is computed.
    MODE 0: level 0 from a dword-aligned input (dword loads + byte permute);
    MODE 1: byte gathers (level 0 of an unaligned input, or scale-2
-   decimation); MODE 2: bilinear. */
+   decimation); MODE 2: bilinear, one dword per column and source row;
+   MODE 3: bilinear, one 8-byte window per source row holding the taps of all
+   4 columns (every lane's taps within 8 bytes: scales up to 2); MODE 4:
+   bilinear, two 8-byte windows per source row, for columns 0-1 and 2-3;
+   MODE 5: level 0 where no lane of the wavefront reflects a column or owns
+   row padding: the lane's own dword.  The batch stage is bound by the
+   vector memory instructions per pixel, so a wavefront uses the mode with
+   the fewest loads whose windows hold every lane's taps. */
 template <int MODE>
 struct PyrRow {
-    uint32_t v[MODE == 0 ? 3 : 8];
+    uint32_t v[MODE == 0 ? 3 : (MODE == 3 ? 4 : (MODE == 5 ? 1 : 8))];
     int fy;
 };
 
@@ -179,12 +191,18 @@ typedef unsigned short pyr_u16x2 __attribute__((ext_vector_type(2)));
    and a column offset that are both kOob still sum below 2^32. */
 constexpr uint32_t kOob = 0x40000000u;
 
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+
 struct PyrLane {
     __amdgpu_buffer_rsrc_t src;  /* the input image, h0 * ip bytes */
     int ip, l, h0;
     /* MODE 1: xa = source byte offset in a row, xb = the padding columns' own bytes.
        MODE 2: xa = offset of the source dword holding the column pair (x0, x1),
-       xb = v_perm selector giving x0 | x1 << 16 from it, wx = (256 - fx) | fx << 16 */
+       xb = v_perm selector giving x0 | x1 << 16 from it, wx = (256 - fx) | fx << 16.
+       MODE 3: xa[0] = offset of the lane's 8-byte window, xb = v_perm selectors
+       of each column's pair in it, wx as MODE 2.  MODE 4: xa[0], xa[1] = the
+       windows of columns 0-1 and 2-3, xb and wx as MODE 3 */
     uint32_t xa[4], xb[4], wx[4];
     uint32_t base4, sel, direct; /* MODE 0: reflect window start, byte selector, the lane's own dword */
     uint32_t padmask;            /* bytes of the lane's dword that lie in the row padding */
@@ -196,7 +214,34 @@ struct PyrLane {
 template <int MODE>
 __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, bool live, PyrRow<MODE>& R)
 {
-    if constexpr (MODE == 2) {
+    if constexpr (MODE == 3) {
+        /* one (unaligned) 8-byte window per source row holds every tap */
+        const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
+        const uint32_t sa = live ? (uint32_t)(ya * P.ip) : kOob, sb = live ? (uint32_t)(yb * P.ip) : kOob;
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(P.src, P.xa[0], sa, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(P.src, P.xa[0], sb, 0);
+        R.v[0] = a[0];
+        R.v[1] = a[1];
+        R.v[2] = b[0];
+        R.v[3] = b[1];
+        R.fy = ty & 255;
+    } else if constexpr (MODE == 4) {
+        const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
+        const uint32_t sa = live ? (uint32_t)(ya * P.ip) : kOob, sb = live ? (uint32_t)(yb * P.ip) : kOob;
+        const auto a0 = __builtin_amdgcn_raw_buffer_load_b64(P.src, P.xa[0], sa, 0);
+        const auto a1 = __builtin_amdgcn_raw_buffer_load_b64(P.src, P.xa[1], sa, 0);
+        const auto b0 = __builtin_amdgcn_raw_buffer_load_b64(P.src, P.xa[0], sb, 0);
+        const auto b1 = __builtin_amdgcn_raw_buffer_load_b64(P.src, P.xa[1], sb, 0);
+        R.v[0] = a0[0];
+        R.v[1] = a0[1];
+        R.v[2] = a1[0];
+        R.v[3] = a1[1];
+        R.v[4] = b0[0];
+        R.v[5] = b0[1];
+        R.v[6] = b1[0];
+        R.v[7] = b1[1];
+        R.fy = ty & 255;
+    } else if constexpr (MODE == 2) {
         /* one (unaligned) dword per source row and column holds both taps */
         const int ya = ty >> 8, yb = min(ya + 1, P.h0 - 1);
         const uint32_t sa = live ? (uint32_t)(ya * P.ip) : kOob, sb = live ? (uint32_t)(yb * P.ip) : kOob;
@@ -214,6 +259,10 @@ __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, bool 
             R.v[4 + k] = __builtin_amdgcn_raw_buffer_load_b8(P.src, P.xb[k], so, 0);
         }
         R.fy = 0;
+    } else if constexpr (MODE == 5) {
+        const uint32_t so = live ? (uint32_t)(yr * P.ip) : kOob;
+        R.v[0] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.direct, so, 0);
+        R.fy = 0;
     } else {
         const uint32_t so = live ? (uint32_t)(yr * P.ip) : kOob;
         R.v[0] = __builtin_amdgcn_raw_buffer_load_b32(P.src, P.base4, so, 0);
@@ -228,7 +277,7 @@ __device__ __forceinline__ void pyr_load(const PyrLane& P, int yr, int ty, bool 
 template <int MODE>
 __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>& R, uint32_t& hin, uint32_t& out)
 {
-    if constexpr (MODE == 2) {
+    if constexpr (MODE == 2 || MODE == 3 || MODE == 4) {
         /* ((256-fy)((256-fx)a + fx b) + fy((256-fx)c + fx d) + 32768) >> 16 is
            exact integer arithmetic, so the rows are combined first, both
            columns at once in 16-bit lanes (each sum <= 255 * 256), then the
@@ -237,8 +286,18 @@ __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>&
         uint32_t x[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const uint32_t ab = __builtin_amdgcn_perm(R.v[2 * k], R.v[2 * k], P.xb[k]);
-            const uint32_t cd = __builtin_amdgcn_perm(R.v[2 * k + 1], R.v[2 * k + 1], P.xb[k]);
+            uint32_t ab, cd;
+            if constexpr (MODE == 3) {
+                ab = __builtin_amdgcn_perm(R.v[1], R.v[0], P.xb[k]);
+                cd = __builtin_amdgcn_perm(R.v[3], R.v[2], P.xb[k]);
+            } else if constexpr (MODE == 4) {
+                const int wi = 2 * (k >> 1);
+                ab = __builtin_amdgcn_perm(R.v[wi + 1], R.v[wi], P.xb[k]);
+                cd = __builtin_amdgcn_perm(R.v[4 + wi + 1], R.v[4 + wi], P.xb[k]);
+            } else {
+                ab = __builtin_amdgcn_perm(R.v[2 * k], R.v[2 * k], P.xb[k]);
+                cd = __builtin_amdgcn_perm(R.v[2 * k + 1], R.v[2 * k + 1], P.xb[k]);
+            }
             const pyr_u16x2 col = __builtin_bit_cast(pyr_u16x2, ab) * wa + __builtin_bit_cast(pyr_u16x2, cd) * wb;
             x[k] = __builtin_amdgcn_udot2(col, __builtin_bit_cast(pyr_u16x2, P.wx[k]), 32768u, false);
         }
@@ -248,6 +307,9 @@ __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>&
         hin = R.v[0] | (R.v[1] << 8) | (R.v[2] << 16) | (R.v[3] << 24);
         const uint32_t pad = R.v[4] | (R.v[5] << 8) | (R.v[6] << 16) | (R.v[7] << 24);
         out = (hin & ~P.padmask) | pad;
+    } else if constexpr (MODE == 5) {
+        hin = R.v[0];
+        out = hin;
     } else {
         hin = __builtin_amdgcn_perm(R.v[1], R.v[0], P.sel);
         out = (hin & ~P.padmask) | (R.v[2] & P.padmask & P.ipmask);
@@ -255,24 +317,36 @@ __device__ __forceinline__ void pyr_pixels(const PyrLane& P, const PyrRow<MODE>&
 }
 
 /* One wavefront builds a strip of kPyrStripW columns x TH rows of one level:
-   lane i owns the 4 columns xs - 4 + 4i .. +3 (lanes 0 and 63 only feed the
-   horizontal taps of their neighbours).  The lane walks rows ys-3 .. ys+TH+2
+   lane i owns the 4 columns xs - 4 + 4i .. +3 (lanes 1..60 own the strip;
+   lanes 0 and 61 only feed the horizontal taps of their neighbours).  The lane walks rows ys-3 .. ys+TH+2
    (reflect-101 at the level's top and bottom): per row it forms the 4 level
    pixels (copy of the input, scale-2 decimation, or 8-bit bilinear from the
    input), takes its neighbours' pixels across lanes, runs the horizontal
    7-tap pass and keeps the last 8 results in registers, from which the
    vertical pass emits the blurred row 3 above.  The loads of a row are issued
-   three rows ahead; stores outside the strip's rows or lanes go to kOob.
+   three rows ahead.  The raw and blurred dwords of 4 rows are gathered in the
+   wavefront's LDS rows and leave as one 16-byte store per lane and output
+   (15 lanes per row), a quarter of the store instructions of dword stores;
+   stores outside the strip's rows or columns go to kOob.
    Strips cover the whole pitch: row padding is written as the input's bytes
    at level 0 and as 0 elsewhere, in both the raw and the blurred level
    (orbd.cu:514-535, 1273-1338; blur out of place). */
 template <int MODE, int TH>
 __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int ys, int h, int p,
                                          __amdgpu_buffer_rsrc_t raw, __amdgpu_buffer_rsrc_t blur, int tyv,
-                                         uint32_t cxw)
+                                         int xs, uint32_t* __restrict__ s_rows)
 {
     const int lane = threadIdx.x & 63;
-    const int addr_up = ((lane + 63) & 63) << 2, addr_dn = ((lane + 1) & 63) << 2;
+    /* gather rows: lane i's dword of row q at s_rows[q * 64 + i - 1] (lanes
+       1..60 at 0..59, the halo lanes in 60..63); the 16-byte store of lane m
+       (< 60) takes chunk m % 15 of row m / 15 */
+    uint32_t* s_raw = s_rows;
+    uint32_t* s_blur = s_rows + 256;
+    const int wslot = (lane - 1) & 63;
+    const int gq = lane / 15, gc = lane - 15 * gq;
+    const int gcol = xs + 16 * gc;
+    const bool gok = lane < 60 && gcol < p;
+    const int grd = gok ? 64 * gq + 4 * gc : 0;
     /* rows walked: the strip's rows plus 3 above and below, in steps of 8 */
     const int nwalk = min(TH, h - ys) + 6;
     const int row_end = min(ys + TH, h);
@@ -301,27 +375,32 @@ __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int 
             __builtin_amdgcn_sched_barrier(0);
             uint32_t hin, rawout;
             pyr_pixels<MODE>(P, st[j % 4], hin, rawout);
-            const uint32_t lf = lane_get(hin, addr_up), rt = lane_get(hin, addr_dn);
+            const uint32_t lf = from_lane_below(hin), rt = from_lane_above(hin);
             const int rr = ys - 3 + i;
             outring[j] = rawout;
-            const uint32_t roff = (rr >= ys && rr < row_end) ? (uint32_t)(rr * p) : kOob;
-            __builtin_amdgcn_raw_buffer_store_b32(rawout, raw, roff + cxw, 0, 0);
-            /* horizontal pass: bytes q1..q10 = columns cx-3 .. cx+6 */
-            int q[12];
+            s_raw[64 * (j & 3) + wslot] = rawout;
+            /* horizontal pass: q1..q10 = columns cx-3 .. cx+6, each byte
+               converted to float once; sums of two integers <= 255 are exact
+               in FP32, so they equal the converted integer sums.  The empty
+               asm keeps the compiler from folding the float sums back into
+               integer sums converted one by one (two instructions a sum) */
+            float q[12];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                q[k] = (lf >> (8 * k)) & 255;
-                q[4 + k] = (hin >> (8 * k)) & 255;
-                q[8 + k] = (rt >> (8 * k)) & 255;
+                q[k] = (float)((lf >> (8 * k)) & 255);
+                q[4 + k] = (float)((hin >> (8 * k)) & 255);
+                q[8 + k] = (float)((rt >> (8 * k)) & 255);
             }
+#pragma unroll
+            for (int k = 1; k <= 10; k++) asm("" : "+v"(q[k]));
             float4 hv;
             float* hf = (float*)&hv;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                float v = __fmul_rn(T.k0, (float)q[k + 4]);
-                v = __fmaf_rn(T.k1, (float)(q[k + 3] + q[k + 5]), v);
-                v = __fmaf_rn(T.k2, (float)(q[k + 2] + q[k + 6]), v);
-                v = __fmaf_rn(T.k3, (float)(q[k + 1] + q[k + 7]), v);
+                float v = __fmul_rn(T.k0, q[k + 4]);
+                v = __fmaf_rn(T.k1, __fadd_rn(q[k + 3], q[k + 5]), v);
+                v = __fmaf_rn(T.k2, __fadd_rn(q[k + 2], q[k + 6]), v);
+                v = __fmaf_rn(T.k3, __fadd_rn(q[k + 1], q[k + 7]), v);
                 hf[k] = v;
             }
             ring[j] = hv;
@@ -348,8 +427,17 @@ __device__ __forceinline__ void pyr_walk(const PyrLane& P, const Tables& T, int 
             uint32_t out = __builtin_amdgcn_perm(rb[1], rb[0], 0x0c0c0400u) | __builtin_amdgcn_perm(rb[3], rb[2], 0x04000c0cu);
             /* padding columns of the blurred row keep the raw row's bytes */
             out = (out & ~P.padmask) | (outring[(j + 5) % 8] & P.padmask);
-            const uint32_t boff = (o >= ys && o < row_end) ? (uint32_t)(o * p) : kOob;
-            __builtin_amdgcn_raw_buffer_store_b32(out, blur, boff + cxw, 0, 0);
+            s_blur[64 * (j & 3) + wslot] = out;
+            if ((j & 3) == 3) {
+                /* raw rows rr - 3 .. rr and blurred rows o - 3 .. o */
+                const v4u rv = *(const v4u*)&s_raw[grd];
+                const v4u bv = *(const v4u*)&s_blur[grd];
+                const int gr = rr - 3 + gq, gb = o - 3 + gq;
+                const uint32_t ro = gok && gr >= ys && gr < row_end ? (uint32_t)(gr * p + gcol) : kOob;
+                const uint32_t bo = gok && gb >= ys && gb < row_end ? (uint32_t)(gb * p + gcol) : kOob;
+                __builtin_amdgcn_raw_buffer_store_b128(rv, raw, ro, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(bv, blur, bo, 0, 0);
+            }
         }
     }
 }
@@ -360,6 +448,8 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
 {
     static_assert(TH + 6 <= 64, "one lane per walked row holds its bilinear row entry");
     static_assert((TH + 6) % 8 == 0, "a full strip walks whole unrolled groups of 8 rows");
+    /* per wavefront: 4 gathered rows of raw and of blurred dwords */
+    __shared__ __align__(16) uint32_t s_gather[4 * 512];
     const int lane = threadIdx.x & 63;
     const int u = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
     if (u >= nunits) return;
@@ -376,10 +466,10 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
     P.h0 = F.h0;
     P.src = __builtin_amdgcn_make_buffer_rsrc((void*)F.img, 0, F.h0 * F.ip, 0x00020000);
     const bool aligned = (F.ip & 3) == 0 && (((uintptr_t)F.img) & 3) == 0;
-    const int mode = l == 0 ? (aligned ? 0 : 1) : (F.ref_mode ? 1 : 2);
+    int mode = l == 0 ? (aligned ? 0 : 1) : (F.ref_mode ? 1 : 2);
     P.padmask = 0;
     P.ipmask = 0;
-    int xr[4];
+    int xr[4], x0s[4], fxs[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int x = cx + k;
@@ -392,6 +482,8 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
             const int v = resize_pos(xr[k], F.w0, w);
             const int x0 = v >> 8, fx = x0 == F.w0 - 1 ? 0 : (v & 255);
             const int d = min(x0, F.w0 - 4), sh = x0 - d;
+            x0s[k] = x0;
+            fxs[k] = fx;
             P.xa[k] = (uint32_t)d;
             P.xb[k] = (uint32_t)sh | 0x0c00u | (uint32_t)(sh + 1) << 16 | 0x0c000000u;
             P.wx[k] = (uint32_t)(256 - fx) | (uint32_t)fx << 16;
@@ -402,6 +494,39 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
             P.wx[k] = 0;
         }
     }
+    if (mode == 2 && F.w0 >= 8) {
+        /* MODE 3 when every lane's taps lie in the 8 bytes from
+           min(its lowest tap, w0 - 8) (inside the input row), else MODE 4
+           when those of columns 0-1 and of 2-3 each lie in such a window; a
+           second tap past a window only occurs at x0 = w0 - 1, where fx = 0
+           ignores it */
+        for (int nw = 1; nw <= 2 && mode == 2; nw++) {
+            const int per = 4 / nw;
+            bool fits = true;
+            uint32_t sel[4], st[2];
+#pragma unroll
+            for (int wi = 0; wi < 2; wi++) {
+                int lo = x0s[wi * 2];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k / per == wi) lo = min(lo, x0s[k]);
+                st[wi] = (uint32_t)min(lo, F.w0 - 8);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int o = x0s[k] - (int)st[k / per];
+                fits = fits && o <= 7 && (o <= 6 || fxs[k] == 0);
+                sel[k] = (uint32_t)o | 0x0c00u | (uint32_t)min(o + 1, 7) << 16 | 0x0c000000u;
+            }
+            if (__ballot(!fits) == 0) {
+                mode = nw == 1 ? 3 : 4;
+                P.xa[0] = st[0];
+                P.xa[1] = st[1];
+#pragma unroll
+                for (int k = 0; k < 4; k++) P.xb[k] = sel[k];
+            }
+        }
+    }
     if (mode == 0) {
         const int lo = min(min(xr[0], xr[1]), min(xr[2], xr[3]));
         P.base4 = (uint32_t)(lo & ~3);
@@ -409,17 +534,23 @@ __global__ __launch_bounds__(256) void k_pyramid(const FrameDev* __restrict__ fr
 #pragma unroll
         for (int k = 0; k < 4; k++) P.sel |= (uint32_t)(xr[k] - (lo & ~3)) << (8 * k);
         P.direct = cx >= 0 ? (uint32_t)cx : kOob;
+        /* MODE 5: every lane's 4 columns are its own dword (no reflection,
+           no row padding) */
+        const bool own = xr[0] == cx && xr[3] == cx + 3 && P.padmask == 0;
+        if (__ballot(!own) == 0) mode = 5;
     }
-    /* lanes 1..62 inside the pitch write; the others store to kOob */
-    const uint32_t cxw = (lane >= 1 && lane <= 62 && cx < p) ? (uint32_t)cx : kOob;
     /* lane i holds the bilinear row entry of walked row i */
-    const int tyv = mode == 2 && lane < TH + 6 ? resize_pos(reflect101(ys - 3 + lane, h), F.h0, h) : 0;
+    const int tyv = (mode >= 2 && mode <= 4) && lane < TH + 6 ? resize_pos(reflect101(ys - 3 + lane, h), F.h0, h) : 0;
     const int lbytes = h * p;
     __amdgpu_buffer_rsrc_t raw = __builtin_amdgcn_make_buffer_rsrc((void*)(F.raw + F.loff[l]), 0, lbytes, 0x00020000);
     __amdgpu_buffer_rsrc_t blur = __builtin_amdgcn_make_buffer_rsrc((void*)(F.blur + F.loff[l]), 0, lbytes, 0x00020000);
-    if (mode == 0) pyr_walk<0, TH>(P, T, ys, h, p, raw, blur, tyv, cxw);
-    else if (mode == 1) pyr_walk<1, TH>(P, T, ys, h, p, raw, blur, tyv, cxw);
-    else pyr_walk<2, TH>(P, T, ys, h, p, raw, blur, tyv, cxw);
+    uint32_t* s_rows = s_gather + 512 * (threadIdx.x >> 6);
+    if (mode == 0) pyr_walk<0, TH>(P, T, ys, h, p, raw, blur, tyv, xs, s_rows);
+    else if (mode == 1) pyr_walk<1, TH>(P, T, ys, h, p, raw, blur, tyv, xs, s_rows);
+    else if (mode == 3) pyr_walk<3, TH>(P, T, ys, h, p, raw, blur, tyv, xs, s_rows);
+    else if (mode == 4) pyr_walk<4, TH>(P, T, ys, h, p, raw, blur, tyv, xs, s_rows);
+    else if (mode == 5) pyr_walk<5, TH>(P, T, ys, h, p, raw, blur, tyv, xs, s_rows);
+    else pyr_walk<2, TH>(P, T, ys, h, p, raw, blur, tyv, xs, s_rows);
 }
 
 /* -------------------------------------------------------------- detection */
@@ -563,16 +694,20 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     const int tid = threadIdx.x;
     if (tid < 2) s_n[tid] = 0;
     /* stage kDetSH rows x kDetSW bytes as dwords (every row start is dword
-       aligned): all loads first, then the LDS writes */
+       aligned) through a buffer resource at the tile's first byte, so every
+       address is a 32-bit offset: all loads first, then the LDS writes.  The
+       tile lies inside the flat pyramid and its guard rows. */
     {
-        const ORB_GLOBAL uint8_t* base = as_global(F.raw) + F.loff[l] + (long long)(y0 - 4) * p + (x0 - 4);
+        const uint8_t* base = F.raw + F.loff[l] + (long long)(y0 - 4) * p + (x0 - 4);
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)base, 0, kDetSH * p, 0x00020000);
         constexpr int dw_per_row = kDetSW / 4, ndw = kDetSH * dw_per_row, per = (ndw + 255) / 256;
         uint32_t v[per];
 #pragma unroll
         for (int k = 0; k < per; k++) {
             const int i = tid + 256 * k;
             const int r = i / dw_per_row, c = i - r * dw_per_row;
-            if (i < ndw) v[k] = *(const ORB_GLOBAL uint32_t*)(base + (long long)r * p + 4 * c);
+            if (i < ndw) v[k] = __builtin_amdgcn_raw_buffer_load_b32(rs, (uint32_t)(r * p + 4 * c), 0, 0);
         }
 #pragma unroll
         for (int k = 0; k < per; k++) {

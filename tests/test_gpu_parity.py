@@ -157,9 +157,11 @@ def test_input_padding_bytes_follow_the_flat_layout(torch, ctx, oracle):
 
 
 PYR_CASES = [
-    # w, h, noctaves, scale: bilinear levels, scale-2 decimation, odd sizes and a pitch equal to the width
+    # w, h, noctaves, scale: bilinear levels (scale 1.5 reaches level scales whose taps need one
+    # load per column), scale-2 decimation, odd sizes and a pitch equal to the width
     (640, 480, 8, 1.2),
     (1920, 1080, 8, 1.2),
+    (1920, 1080, 8, 1.5),
     (333, 257, 8, 1.2),
     (1280, 720, 8, 1.25),
     (600, 400, 5, 2.0),
