@@ -683,7 +683,6 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
     __shared__ uint16_t s_cand[kDetTW * kDetTH];
     __shared__ uint16_t s_corner[kDetTW * kDetTH];
     __shared__ int s_n[2];
-    __shared__ int s_scan[8];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const Job jb = find_job(jobs, job_of, b);
     const FrameDev& F = frames[jb.frame];
@@ -770,13 +769,20 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
             const uint32_t nib = (tb | (tb >> 14)) & 15u;
             m |= (iy >= 3 && iy < h - 3 ? nib & colm : 0u) << (4 * rr);
         }
-        int off = block_scan<256>(__popc(m), s_scan, ncand);
+        /* each wavefront reserves its candidates' slots with one LDS add
+           (the order of the list does not matter) */
+        const int cnt = __popc(m);
+        const int incl = wave_incl_scan(cnt);
+        int base = 0;
+        if ((tid & 63) == 63) base = atomicAdd(&s_n[0], incl);
+        int off = __builtin_amdgcn_readlane(base, 63) + incl - cnt;
         for (; m; m &= m - 1) {
             const int bit = __ffs(m) - 1;
             s_cand[off++] = (uint16_t)(((tid / kG + kStep * (bit >> 2)) << 8) | (lx + (bit & 3)));
         }
     }
     __syncthreads();
+    ncand = s_n[0];
     /* pass 2: exact segment test on the candidates */
     for (int i0 = 0; i0 < ncand; i0 += 256) {
         const int i = i0 + tid;
@@ -786,13 +792,23 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
             e = s_cand[i];
             const uint8_t* c = &s_tile[((e >> 8) + 4) * kDetSW + (e & 255) + 4];
             const int v = c[0];
-            unsigned dark = 0, bright = 0;
+            /* ring points k and k + 8 in the two 16-bit lanes of one word;
+               q - (v - t) < 0 (darker) and (v + t) - q < 0 (brighter) are the
+               sign bits of 16-bit differences in [-255, 510], moved to bits
+               k and 16 + k, then the high half down to bits 8..15 */
+            const pyr_u16x2 vm = {(unsigned short)(v - thr), (unsigned short)(v - thr)};
+            const pyr_u16x2 vp = {(unsigned short)(v + thr), (unsigned short)(v + thr)};
+            uint32_t dk = 0, bk = 0;
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const int q = c[ring_off(k)];
-                dark |= (unsigned)(q - v < -thr) << k;
-                bright |= (unsigned)(q - v > thr) << k;
+            for (int k = 0; k < 8; k++) {
+                const pyr_u16x2 q = {(unsigned short)c[ring_off(k)], (unsigned short)c[ring_off(k + 8)]};
+                const uint32_t sd = __builtin_bit_cast(uint32_t, (pyr_u16x2)(q - vm));
+                const uint32_t sb = __builtin_bit_cast(uint32_t, (pyr_u16x2)(vp - q));
+                dk |= (sd >> (15 - k)) & (0x00010001u << k);
+                bk |= (sb >> (15 - k)) & (0x00010001u << k);
             }
+            const unsigned dark = (dk & 0xffu) | ((dk >> 8) & 0xff00u);
+            const unsigned bright = (bk & 0xffu) | ((bk >> 8) & 0xff00u);
             /* quick test of orbd.cu:687-700, then >8 consecutive of the wrapped
                ring (a circular run of 9) on the darker arc first, exclusively
                (orbd.cu:703-740) */
@@ -824,42 +840,53 @@ __global__ __launch_bounds__(256) void k_detect(const FrameDev* __restrict__ fra
                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
     if (T.score_type == 0) {
-        /* Harris, 16 lanes per corner and 4 corners per wavefront pass (a
-           tile holds a few corners, so one lane per corner would leave most
-           lanes idle through the 49-position window): lane `sub` of a corner
-           takes window positions sub, sub + 16, sub + 32 and sub + 48, then
-           the row of 16 lanes sums its products */
-        const int lane = tid & 63, wid = tid >> 6, sub = lane & 15;
-        int woff[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int pos = sub + 16 * j, wy = pos / 7;
-            woff[j] = (wy - 3) * kDetSW + (pos - 7 * wy - 3);
-        }
-        for (int g = 4 * wid; g < n; g += 16) {
-            const int ci = g + (lane >> 4);
+        /* Harris, 8 lanes per corner and 8 corners per wavefront pass (a tile
+           holds a few corners, so one lane per corner would leave most lanes
+           idle through the 7x7 window): lane r < 7 of a corner takes window
+           row r - 3, forms the column sums u + 2m + d and d - u of its 9
+           columns once and the Sobel dx, dy of the row's 7 positions from
+           them, then the 8 lanes sum the products (exact integer sums) */
+        const int lane = tid & 63, wid = tid >> 6, wr = lane & 7;
+        for (int g = 8 * wid; g < n; g += 32) {
+            const int ci = g + (lane >> 3);
             int sxx = 0, syy = 0, sxy = 0, e = 0;
-            if (ci < n) {
-                e = s_corner[ci];
-                const uint8_t* c = &s_tile[((e >> 8) + 4) * kDetSW + (e & 255) + 4];
+            if (ci < n) e = s_corner[ci];
+            if (ci < n && wr < 7) {
+                /* the 9 bytes of columns -4..4 of rows r-1, r, r+1, read as
+                   the 3 aligned dwords around them and shifted into place */
+                const int a = ((e >> 8) + 4 + wr - 3) * kDetSW + (e & 255);
+                const int sh = 8 * (a & 3);
+                uint32_t x[3][3];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (j == 3 && sub != 0) break;   /* position 48 is the last */
-                    const uint8_t* m = c + woff[j];
-                    const int u0 = m[-kDetSW - 1], u1 = m[-kDetSW], u2 = m[-kDetSW + 1];
-                    const int m0 = m[-1], m2 = m[1];
-                    const int d0 = m[kDetSW - 1], d1 = m[kDetSW], d2 = m[kDetSW + 1];
-                    const int dx = (m2 - m0) * 2 + (u2 - u0) + (d2 - d0);
-                    const int dy = (d1 - u1) * 2 + (d0 - u0) + (d2 - u2);
-                    sxx += dx * dx;
-                    syy += dy * dy;
-                    sxy += dx * dy;
+                for (int rr = 0; rr < 3; rr++) {
+                    const uint32_t* w = (const uint32_t*)&s_tile[(a & ~3) + (rr - 1) * kDetSW];
+                    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+                    x[rr][0] = (uint32_t)(((uint64_t)w1 << 32 | w0) >> sh);
+                    x[rr][1] = (uint32_t)(((uint64_t)w2 << 32 | w1) >> sh);
+                    x[rr][2] = w2 >> sh;
+                }
+                int hs[9], vs[9];
+#pragma unroll
+                for (int k = 0; k < 9; k++) {
+                    const int u = (x[0][k >> 2] >> (8 * (k & 3))) & 255;
+                    const int m = (x[1][k >> 2] >> (8 * (k & 3))) & 255;
+                    const int d = (x[2][k >> 2] >> (8 * (k & 3))) & 255;
+                    hs[k] = u + 2 * m + d;
+                    vs[k] = d - u;
+                }
+#pragma unroll
+                for (int x = 0; x < 7; x++) {
+                    const int dx = hs[x + 2] - hs[x];
+                    const int dy = vs[x] + 2 * vs[x + 1] + vs[x + 2];
+                    sxx += __mul24(dx, dx);
+                    syy += __mul24(dy, dy);
+                    sxy += __mul24(dx, dy);
                 }
             }
-            sxx = row_sum16(sxx);
-            syy = row_sum16(syy);
-            sxy = row_sum16(sxy);
-            if (ci < n && sub == 0) {
+            sxx = half_row_sum8(sxx);
+            syy = half_row_sum8(syy);
+            sxy = half_row_sum8(sxy);
+            if (ci < n && wr == 0) {
                 const float score = harris_from_sums(sxx, syy, sxy, T.harris_scale);
                 if (score > 0.f) emit(e, score);
             }

@@ -48,6 +48,32 @@ __device__ __forceinline__ int row_sum16(int v)
     return v;
 }
 
+/* Inclusive prefix sum over the 64 lanes (lane i gets v[0] + .. + v[i]):
+   row shifts by 1, 2, 4, 8 within each 16-lane row, then the last lane of
+   row 0 added to rows 1 and 3 and that of row 1 to rows 2 and 3 (DPP row
+   broadcasts).  Every lane of the wavefront must be active. */
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   /* row_shr:1 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   /* row_shr:2 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   /* row_shr:4 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   /* row_shr:8 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  /* row_bcast:15 */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  /* row_bcast:31 */
+    return v;
+}
+
+/* Sum over the 8 lanes of this lane's half row (lanes 8h .. 8h+7), returned
+   to every lane of it: two quad swaps, then the half-row mirror (DPP).
+   Every lane of the wavefront must be active. */
+__device__ __forceinline__ int half_row_sum8(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0xb1, 0xf, 0xf, false);   /* quad_perm [1,0,3,2] */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4e, 0xf, 0xf, false);   /* quad_perm [2,3,0,1] */
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xf, 0xf, false);  /* row_half_mirror */
+    return v;
+}
+
 /* order-preserving key of a finite float (-0 and +0 share one key, above 0) */
 __device__ __forceinline__ uint32_t float_order_key(float f)
 {
