@@ -380,13 +380,36 @@ __global__ void k_fallback_compact(const uint8_t* __restrict__ q, int nq, const 
     d[1] = s[1];
 }
 
-__global__ void k_probe_merge(const uint2* __restrict__ partial, int nq, int nprobe, const int* __restrict__ fb_index,
-                              const int* __restrict__ ex_idx1, const int* __restrict__ ex_d1,
-                              const int* __restrict__ ex_idx2, const int* __restrict__ ex_d2, int* __restrict__ idx1,
-                              int* __restrict__ d1, int* __restrict__ idx2, int* __restrict__ d2)
+/* top-2 of each query over its probed slots: 8 lanes per query, lane j takes
+   slots j, j + 8, ... (contiguous 64-byte reads across the 8 lanes), then the
+   lanes' top-2 meet by shuffles (the keys are distinct, so the order of the
+   pushes does not matter) */
+__global__ __launch_bounds__(256) void k_probe_merge(const uint2* __restrict__ partial, const int* __restrict__ probes,
+                                                     int nq, int nprobe, const int* __restrict__ fb_index,
+                                                     const int* __restrict__ ex_idx1, const int* __restrict__ ex_d1,
+                                                     const int* __restrict__ ex_idx2, const int* __restrict__ ex_d2,
+                                                     int* __restrict__ idx1, int* __restrict__ d1,
+                                                     int* __restrict__ idx2, int* __restrict__ d2)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3, j = threadIdx.x & 7;
+    const bool live = i < nq;
+    uint32_t k1 = kNoKey, k2 = kNoKey;
+    if (live) {
+        for (int s = j; s < nprobe; s += 8) {
+            /* slots left out by the probability-mass rule (-1) were not searched */
+            if (probes[(long long)i * nprobe + s] < 0) continue;
+            const uint2 v = partial[(long long)i * nprobe + s];
+            top2_push(k1, k2, v.x);
+            top2_push(k1, k2, v.y);
+        }
+    }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        const uint32_t o1 = (uint32_t)__shfl_xor((int)k1, o, 64), o2 = (uint32_t)__shfl_xor((int)k2, o, 64);
+        top2_push(k1, k2, o1);
+        top2_push(k1, k2, o2);
+    }
+    if (!live || j) return;
     const int f = fb_index ? fb_index[i] : -1;
     if (f >= 0) {
         if (idx1) idx1[i] = ex_idx1[f];
@@ -394,12 +417,6 @@ __global__ void k_probe_merge(const uint2* __restrict__ partial, int nq, int npr
         if (idx2) idx2[i] = ex_idx2[f];
         if (d2) d2[i] = ex_d2[f];
         return;
-    }
-    uint32_t k1 = kNoKey, k2 = kNoKey;
-    for (int s = 0; s < nprobe; s++) {
-        const uint2 v = partial[(long long)i * nprobe + s];
-        top2_push(k1, k2, v.x);
-        top2_push(k1, k2, v.y);
     }
     if (idx1) idx1[i] = k1 == kNoKey ? -1 : (int)(k1 & kIdxMask);
     if (d1) d1[i] = k1 == kNoKey ? 257 : (int)(k1 >> kIdxBits);
@@ -423,7 +440,11 @@ __global__ void k_gather(const uint8_t* __restrict__ q, const int* __restrict__ 
 
 /* the nprobe nearest centroids of each query in Hamming distance (ties to the
    lower bucket id; by ascending distance with SORTED, else by bucket id) --
-   the probe without the network; the centroids are staged
+   the probe without the network.  With mass below 1 (k <= 64) only the
+   nearest prefix whose mass reaches `mass` under the weights exp(-d / temp)
+   is probed, by the rule of the network's probe (weights relative to the
+   nearest centroid, wave_mass_keep), best first and -1 after.  The
+   centroids are staged
    in LDS once per workgroup and each wavefront takes kCenQ queries in turn.
    CAP and FIX as for the network's probe: with CAP below 1024 a query whose
    candidates do not fit gets -1 as its first probe, and the FIX launch
@@ -434,11 +455,13 @@ constexpr int kCenCap = 256;
 
 template <bool SORTED, int CAP, bool FIX>
 __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restrict__ q, int nq,
-                                                        const uint8_t* __restrict__ cen, int nb, int k,
-                                                        int* __restrict__ probes, float* __restrict__ pmax)
+                                                        const uint8_t* __restrict__ cen, int nb, int k, float mass,
+                                                        float inv_temp, int* __restrict__ probes,
+                                                        float* __restrict__ pmax)
 {
     __shared__ uint4 s_c[1024 * 2];
     __shared__ uint2 s_cand[4][CAP];
+    __shared__ int s_ids[4][64];
     if (FIX) {
         const int r = blockIdx.x * 4 * kCenQ + (int)threadIdx.x;
         const bool redo = threadIdx.x < 4 * kCenQ && r < nq && probes[(long long)r * k] < 0;
@@ -466,10 +489,36 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
                 key[u] = 0u;
             }
         }
-        const bool ok = wave_topk<16, SORTED, CAP>(key, k, s_cand[wv], probes + (long long)r * k);
-        if (!ok && lane == 0) probes[(long long)r * k] = -1;
+        int* const out = probes + (long long)r * k;
+        if (!(mass < 1.f)) {
+            const bool ok = wave_topk<16, SORTED, CAP>(key, k, s_cand[wv], out);
+            if (!ok && lane == 0) out[0] = -1;
+        } else {
+            uint32_t km = 0;
+#pragma unroll
+            for (int u = 0; u < 16; u++) km = max(km, key[u]);
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) km = max(km, (uint32_t)__shfl_xor((int)km, o, 64));
+            float sw = 0.f;
+#pragma unroll
+            for (int u = 0; u < 16; u++) sw += 64 * u < nb ? __expf((float)((int)key[u] - (int)km) * inv_temp) : 0.f;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) sw += __shfl_xor(sw, o, 64);
+            if (!wave_topk<16, true, CAP>(key, k, s_cand[wv], s_ids[wv])) {
+                if (lane == 0) out[0] = -1;
+            } else {
+                /* lane = rank */
+                wave_sync_lds();
+                const int c = lane < k ? s_ids[wv][lane] : 0;
+                const uint4 a = s_c[2 * c], b = s_c[2 * c + 1];
+                const int d = __popc(qa.x ^ a.x) + __popc(qa.y ^ a.y) + __popc(qa.z ^ a.z) + __popc(qa.w ^ a.w) +
+                              __popc(qb.x ^ b.x) + __popc(qb.y ^ b.y) + __popc(qb.z ^ b.z) + __popc(qb.w ^ b.w);
+                const uint32_t kc = 0x10000u - (uint32_t)d;
+                wave_mass_keep(c, __expf((float)((int)kc - (int)km) * inv_temp), k, mass * sw, out);
+            }
+        }
         if (lane == 0) pmax[r] = 1.f;
-        wave_sync_lds(); /* the next query reuses s_cand */
+        wave_sync_lds(); /* the next query reuses s_cand and s_ids */
     }
 }
 
@@ -482,26 +531,30 @@ hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, i
     return hipGetLastError();
 }
 
-hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
-                                 float* pmax, bool sorted, hipStream_t s)
+hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, float mass, float temp,
+                                 int* probes, float* pmax, bool sorted, hipStream_t s)
 {
     if (nb <= 0 || nb > 1024 || nb % 64 || k < 1 || k > nb) return hipErrorInvalidValue;
+    if (mass < 1.f && (k > 64 || !(mass > 0.f) || !(temp > 0.f))) return hipErrorInvalidValue;
+    const float it = mass < 1.f ? 1.f / temp : 0.f;
     const dim3 grid((nq + 4 * kCenQ - 1) / (4 * kCenQ));
     if (nq <= 0) return hipGetLastError();
     const bool split = k <= 64 && nb > kCenCap;
+#define CENTROID_PROBE(S, CAPV, FIXV)                                                                          \
+    hipLaunchKernelGGL((k_centroid_probe<S, CAPV, FIXV>), grid, dim3(256), 0, s, q, nq, cen, nb, k, mass, it, \
+                       probes, pmax)
     if (sorted && split) {
-        hipLaunchKernelGGL((k_centroid_probe<true, kCenCap, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
-        hipLaunchKernelGGL((k_centroid_probe<true, 1024, true>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+        CENTROID_PROBE(true, kCenCap, false);
+        CENTROID_PROBE(true, 1024, true);
     } else if (sorted) {
-        hipLaunchKernelGGL((k_centroid_probe<true, 1024, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+        CENTROID_PROBE(true, 1024, false);
     } else if (split) {
-        hipLaunchKernelGGL((k_centroid_probe<false, kCenCap, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes,
-                           pmax);
-        hipLaunchKernelGGL((k_centroid_probe<false, 1024, true>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes, pmax);
+        CENTROID_PROBE(false, kCenCap, false);
+        CENTROID_PROBE(false, 1024, true);
     } else {
-        hipLaunchKernelGGL((k_centroid_probe<false, 1024, false>), grid, dim3(256), 0, s, q, nq, cen, nb, k, probes,
-                           pmax);
+        CENTROID_PROBE(false, 1024, false);
     }
+#undef CENTROID_PROBE
     return hipGetLastError();
 }
 
@@ -572,13 +625,13 @@ hipError_t launch_fallback_compact(const uint8_t* q, int nq, const float* pmax, 
     return hipGetLastError();
 }
 
-hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,
-                              const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1, int* d1, int* idx2,
-                              int* d2, hipStream_t s)
+hipError_t launch_probe_merge(const uint2* partial, const int* probes, int nq, int nprobe, const int* fb_index,
+                              const int* ex_idx1, const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1,
+                              int* d1, int* idx2, int* d2, hipStream_t s)
 {
     if (nq > 0)
-        hipLaunchKernelGGL(k_probe_merge, grid1(nq), dim3(256), 0, s, partial, nq, nprobe, fb_index, ex_idx1, ex_d1,
-                           ex_idx2, ex_d2, idx1, d1, idx2, d2);
+        hipLaunchKernelGGL(k_probe_merge, dim3((unsigned)((nq + 31) / 32)), dim3(256), 0, s, partial, probes, nq,
+                           nprobe, fb_index, ex_idx1, ex_d1, ex_idx2, ex_d2, idx1, d1, idx2, d2);
     return hipGetLastError();
 }
 

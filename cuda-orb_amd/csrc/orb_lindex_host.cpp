@@ -82,15 +82,17 @@ struct orb_lindex {
     DBuf<uint16_t> w16t; /* W1^T (H x 256), W2^T (H x H), W3^T (C x H): K-contiguous forward operands */
     bool wt_fresh = false; /* w16t matches w16 */
     int step = 0;
+    /* probability-mass rule of the probes (1: every one of the nprobe buckets)
+       and the distance temperature of its centroid-mode weights */
+    float mass = 1.f, ctemp = 4.f;
     /* batch workspace */
     int cap_batch = 0;
     DBuf<uint16_t> X, A1, A2, dZ1, dZ2, dZ3;
-    DBuf<float> Z1, Z2, Z3, loss;
+    DBuf<float> Z1, Z2, Z3, loss, eloss;
     DBuf<int> labels, order;
     /* database */
     int ndb = 0;
     DBuf<uint8_t> db, db_sorted, centroids;
-    DBuf<uint16_t> cpm;      /* centroids as +/-centroid_scale bf16, nbuckets x 256 */
     uint16_t cscale16 = 0;   /* bf16 bits of centroid_scale (0: no centroid term) */
     DBuf<int> assign, hist, offsets, chunk_hist, perm;
     DBuf<int> border; /* bucket ids by decreasing size (ties by id): the search's unit order */
@@ -137,6 +139,15 @@ uint16_t bf16_bits(float x)
     return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
+/* the float value of bf16 bits */
+float bf16_value(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
 int ensure_batch(orb_lindex* L, int B)
 {
     if (B <= L->cap_batch) return ORB_OK;
@@ -174,10 +185,11 @@ GemmArgs gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B
     return g;
 }
 
-/* forward of B rows already expanded in L->X: Z1, A1, Z2, A2, Z3 */
-/* Network forward of B rows of X into Z3 (logits); keep_z also stores the
-   pre-activations Z1, Z2 the backward pass needs. */
-int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z)
+/* Network forward of B rows of X (the +/-1 expansion of the descriptors
+   qbits) into Z3 (logits); keep_z also stores the pre-activations Z1, Z2 the
+   backward pass needs.  The centroid term is added to the last layer's
+   products in their epilogue. */
+int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbits)
 {
     const int H = L->H, C = L->C;
     const float* w = L->w.p;
@@ -204,14 +216,11 @@ int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z)
     LTRY(L, launch_gemm(g, s));
     g = gemm(L->A2.p, H, 1, wt + t2, 1, H, B, C, H);
     if (L->cscale16) {
-        /* centroid term <x, c_b> * centroid_scale as a second K segment of the
-           same product (each of its terms is exact in fp32) */
-        if (!L->cpm.p) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
-        g.A2 = L->X.p;
-        g.sa2m = kIn;
-        g.B2 = L->cpm.p;
-        g.sb2n = kIn;
-        g.K2 = kIn;
+        /* centroid term centroid_scale * <x, c_b> = centroid_scale * (256 - 2 Hamming) */
+        if (!L->centroids.p) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
+        g.qbits = qbits;
+        g.cbits = L->centroids.p;
+        g.cscale = bf16_value(L->cscale16);
     }
     g.bias = w + L->off[5];
     g.out32 = L->Z3.p;
@@ -219,14 +228,6 @@ int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z)
     return ORB_OK;
 }
 
-/* +/-centroid_scale expansion of the final centroids for the centroid term */
-int make_cpm(orb_lindex* L, hipStream_t s)
-{
-    if (!L->cscale16) return ORB_OK;
-    LALLOC(L, L->cpm, (size_t)L->C * kIn);
-    LTRY(L, launch_bits_pm(L->centroids.p, L->C, L->cscale16, (uint16_t)(L->cscale16 ^ 0x8000u), L->cpm.p, s));
-    return ORB_OK;
-}
 
 /* backward into L->g after forward and labels; loss per row in L->loss */
 int backward(orb_lindex* L, int B, hipStream_t s)
@@ -271,13 +272,14 @@ int compute_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* pro
         if (rc != ORB_OK) return rc;
         for (int c0 = 0; c0 < nq; c0 += chunk) {
             const int B = nq - c0 < chunk ? nq - c0 : chunk;
-            LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
-            rc = forward(L, B, s, false);
+            const uint8_t* qb = d_q + (size_t)c0 * 32;
+            LTRY(L, launch_bits_pm1(qb, B, L->X.p, s));
+            rc = forward(L, B, s, false, qb);
             if (rc != ORB_OK) return rc;
-            LTRY(L, launch_probe(L->Z3.p, B, nb, np, probes + (size_t)c0 * np, pmax + c0, sorted, s));
+            LTRY(L, launch_probe(L->Z3.p, B, nb, np, L->mass, probes + (size_t)c0 * np, pmax + c0, sorted, s));
         }
     } else {
-        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, probes, pmax, sorted, s));
+        LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, L->mass, L->ctemp, probes, pmax, sorted, s));
     }
     return ORB_OK;
 }
@@ -418,8 +420,6 @@ int orb_lindex_build(orb_lindex* L, const uint8_t* d_db, int ndb, int iters)
                                    L->perm.p, L->db_sorted.p, s));
         if (it < iters) LTRY(L, launch_kmodes_update(L->db_sorted.p, L->offsets.p, nb, L->centroids.p, s));
     }
-    int rc = make_cpm(L, s);
-    if (rc != ORB_OK) return rc;
     {
         std::vector<int> off(nb + 1), ord(nb);
         LTRY(L, hipMemcpyAsync(off.data(), L->offsets.p, sizeof(int) * (nb + 1), hipMemcpyDeviceToHost, s));
@@ -457,7 +457,7 @@ int orb_lindex_grads(orb_lindex* L, const uint8_t* d_q, const int* d_labels, int
     LALLOC(L, L->fb_q, (size_t)n * 32);
     LTRY(L, launch_gather(d_q, nullptr, d_labels, n, L->fb_q.p, L->labels.p, s));
     LTRY(L, launch_bits_pm1(L->fb_q.p, n, L->X.p, s));
-    rc = forward(L, n, s, true);
+    rc = forward(L, n, s, true, L->fb_q.p);
     if (rc != ORB_OK) return rc;
     rc = backward(L, n, s);
     if (rc != ORB_OK) return rc;
@@ -504,7 +504,9 @@ int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int
     /* cosine decay of the step size from lr to lr / 20 over the whole run */
     const double total_steps = (double)epochs * nbatch;
     long long t_step = 0;
-    std::vector<float> lv(batch);
+    /* per epoch: the sum of every training row's loss, accumulated on the device */
+    LALLOC(L, L->eloss, epochs);
+    LTRY(L, hipMemsetAsync(L->eloss.p, 0, sizeof(float) * epochs, s));
     for (int e = 0; e < epochs; e++) {
         for (int i = 0; i < n; i++) ord[i] = i;
         for (int i = n - 1; i > 0; i--) {
@@ -514,28 +516,28 @@ int orb_lindex_train(orb_lindex* L, const uint8_t* d_q, const int* d_target, int
             ord[j] = t;
         }
         LTRY(L, hipMemcpyAsync(L->order.p, ord.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-        double loss_sum = 0;
         for (int bi = 0; bi < nbatch; bi++) {
             LTRY(L, launch_gather(d_q, L->order.p + (size_t)bi * batch, L->tmp_idx.p, batch, L->fb_q.p,
                                   L->labels.p, s));
             LTRY(L, launch_bits_pm1(L->fb_q.p, batch, L->X.p, s));
-            rc = forward(L, batch, s, true);
+            rc = forward(L, batch, s, true, L->fb_q.p);
             if (rc != ORB_OK) return rc;
             rc = backward(L, batch, s);
             if (rc != ORB_OK) return rc;
+            if (h_epoch_loss) LTRY(L, launch_loss_sum(L->loss.p, batch, L->eloss.p + e, s));
             L->step++;
             const double frac = (double)(t_step++) / total_steps;
             const float lr_t = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(3.14159265358979323846 * frac))));
             LTRY(L, launch_adam(L->w.p, L->w16.p, L->g.p, L->m.p, L->v.p, (long long)nparams(L), lr_t, 0.9f, 0.999f,
                                 1e-8f, L->step, s));
             L->wt_fresh = false;
-            if (h_epoch_loss && (bi == nbatch - 1)) {
-                LTRY(L, hipMemcpyAsync(lv.data(), L->loss.p, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
-                LTRY(L, hipStreamSynchronize(s));
-                for (float x : lv) loss_sum += x;
-            }
         }
-        if (h_epoch_loss) h_epoch_loss[e] = (float)(loss_sum / batch);
+    }
+    if (h_epoch_loss) {
+        std::vector<float> el(epochs);
+        LTRY(L, hipMemcpyAsync(el.data(), L->eloss.p, sizeof(float) * epochs, hipMemcpyDeviceToHost, s));
+        LTRY(L, hipStreamSynchronize(s));
+        for (int e = 0; e < epochs; e++) h_epoch_loss[e] = (float)((double)el[e] / ((double)nbatch * batch));
     }
     LTRY(L, hipStreamSynchronize(s));
     return ORB_OK;
@@ -551,7 +553,7 @@ int orb_lindex_logits(orb_lindex* L, const uint8_t* d_q, int n, float* d_logits)
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int B = n - c0 < chunk ? n - c0 : chunk;
         LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
-        rc = forward(L, B, s, false);
+        rc = forward(L, B, s, false, d_q + (size_t)c0 * 32);
         if (rc != ORB_OK) return rc;
         LTRY(L, hipMemcpyAsync(d_logits + (size_t)c0 * L->C, L->Z3.p, sizeof(float) * B * L->C,
                                hipMemcpyDeviceToDevice, s));
@@ -605,7 +607,7 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
                                     L->ex_idx2.p, L->ex_d2.p);
         if (rc != ORB_OK) return rc;
     }
-    LTRY(L, launch_probe_merge(L->partial.p, nq, np, may_fallback ? L->fb_index.p : nullptr, L->ex_idx1.p,
+    LTRY(L, launch_probe_merge(L->partial.p, L->probes.p, nq, np, may_fallback ? L->fb_index.p : nullptr, L->ex_idx1.p,
                                L->ex_d1.p, L->ex_idx2.p, L->ex_d2.p, d_idx1, d_d1, d_idx2, d_d2, s));
     if (h_nfallback) {
         if (may_fallback) {
@@ -633,8 +635,21 @@ int orb_lindex_set_probe(orb_lindex* L, int nprobe, float fallback)
 {
     if (!L) return lfail(L, ORB_E_ARG, "null argument");
     if (nprobe < 1 || nprobe > L->C) return lfail(L, ORB_E_ARG, "nprobe out of range");
+    if (L->mass < 1.f && nprobe > 64) return lfail(L, ORB_E_ARG, "nprobe above 64 with a probability-mass rule");
     L->prm.nprobe = nprobe;
     L->prm.fallback = fallback;
+    return ORB_OK;
+}
+
+int orb_lindex_set_mass(orb_lindex* L, float mass, float centroid_temperature)
+{
+    if (!L) return lfail(L, ORB_E_ARG, "null argument");
+    if (!(mass > 0.f) || !(mass <= 1.f)) return lfail(L, ORB_E_ARG, "mass must be in (0, 1]");
+    if (!(centroid_temperature > 0.f) || !std::isfinite(centroid_temperature))
+        return lfail(L, ORB_E_ARG, "centroid temperature must be positive");
+    if (mass < 1.f && L->prm.nprobe > 64) return lfail(L, ORB_E_ARG, "a probability-mass rule needs nprobe <= 64");
+    L->mass = mass;
+    L->ctemp = centroid_temperature;
     return ORB_OK;
 }
 
@@ -674,7 +689,8 @@ int orb_lindex_load(orb_lindex* L, const char* path)
     float fb;
     uint32_t cs = 0;
     bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "ORBLIDX1", 8) == 0 && fread(hdr, sizeof(hdr), 1, f) == 1 &&
-              (hdr[0] == 1 || hdr[0] == 2) && hdr[1] == L->C && hdr[2] == L->H && fread(&fb, sizeof(float), 1, f) == 1;
+              (hdr[0] == 1 || hdr[0] == 2) && hdr[1] == L->C && hdr[2] == L->H && hdr[3] >= 1 && hdr[3] <= L->C &&
+              fread(&fb, sizeof(float), 1, f) == 1;
     if (ok && hdr[0] == 2) ok = fread(&cs, sizeof(cs), 1, f) == 1 && cs <= 0xFFFFu;
     std::vector<float> w(nparams(L));
     std::vector<uint8_t> cen((size_t)L->C * 32);
@@ -689,6 +705,7 @@ int orb_lindex_load(orb_lindex* L, const char* path)
     LTRY(L, hipStreamSynchronize(s));
     L->prm.nprobe = hdr[3];
     L->prm.fallback = fb;
+    if (hdr[3] > 64) L->mass = 1.f; /* the mass rule covers at most 64 probes */
     L->cscale16 = (uint16_t)cs;
     L->ndb = -1; /* centroids loaded: orb_lindex_build(db, n, 0) assigns the database to them */
     return ORB_OK;

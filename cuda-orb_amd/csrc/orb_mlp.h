@@ -13,9 +13,10 @@ namespace orbk
 /* C(m, n) = sum_k A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak],
    B(k, n) = B[k*sbk + n*sbn] (bf16 bits); M, N, K multiples of 64 (an error
    otherwise).
-   Epilogue, in order: z += addin[m*ldc + n] if addin; z += bias[n];
-   z *= gelu'(zmul[m*ldc + n]) if zmul; out32[m*ldc + n] = z;
-   out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
+   Epilogue, in order: z += addin[m*ldc + n] if addin; z += cscale * (256 -
+   2 Hamming(qbits row m, cbits row n)) if cbits (32-byte rows: the centroid
+   term, exact in FP32); z += bias[n]; z *= gelu'(zmul[m*ldc + n]) if zmul;
+   out32[m*ldc + n] = z; out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
 struct GemmArgs {
     const uint16_t* A;
     long long sam, sak;
@@ -29,31 +30,30 @@ struct GemmArgs {
     float* out32;
     uint16_t* out16;
     long long ldc;
-    /* optional second K segment (both operands K-contiguous, K2 a multiple of
-       64): z also accumulates sum_k A2[m*sa2m + k] B2[n*sb2n + k], k < K2 */
-    const uint16_t* A2;
-    long long sa2m;
-    const uint16_t* B2;
-    long long sb2n;
-    int K2;
+    /* optional centroid term (forward products only): M rows of query bits,
+       N rows of centroid bits, 32 bytes each */
+    const uint8_t* qbits;
+    const uint8_t* cbits;
+    float cscale;
 };
 
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 /* dst (C x R) = src (R x C) transposed (bf16 bits); R and C multiples of 64 */
 hipError_t launch_transpose16(const uint16_t* src, int R, int C, uint16_t* dst, hipStream_t s);
 hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s);
-/* as launch_bits_pm1 with the two values given as bf16 bits (set bit -> pos, clear bit -> neg) */
-hipError_t launch_bits_pm(const uint8_t* desc, int n, uint16_t pos, uint16_t neg, uint16_t* X, hipStream_t s);
 hipError_t launch_softmax_ce(const float* Z, const int* labels, int n, int C, uint16_t* dZ, float* loss,
                              hipStream_t s);
+/* acc[0] += the sum of loss[0 .. n) (fixed order) */
+hipError_t launch_loss_sum(const float* loss, int n, float* acc, hipStream_t s);
 hipError_t launch_colsum(const uint16_t* X, int n, int C, float* out, hipStream_t s);
 hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
                        float b2, float eps, int step, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t s);
 /* top-k buckets per row of logits (ties to the lower id): best first when
-   sorted, else in bucket-id order (the same set) */
-hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, bool sorted, hipStream_t s);
-
+   sorted, else in bucket-id order (the same set); with mass below 1 (k <= 64)
+   the best-ranked prefix whose softmax mass reaches it, best first, -1 after */
+hipError_t launch_probe(const float* Z, int n, int C, int k, float mass, int* probes, float* pmax, bool sorted,
+                        hipStream_t s);
 /* bucketed database (orb_lindex.hip) */
 /* stable counting sort of n entries by bucket (assign): offsets[0..nb], perm
    (sorted position -> index, index order inside a bucket), db_sorted; H holds
@@ -80,15 +80,16 @@ hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, 
                                 uint32_t* qbound, int nq, uint2* partial, int ncu, hipStream_t s);
 /* per query: merge of the probed buckets, or the exact result when the query
    was sent to the fallback (fb_index >= 0) */
-hipError_t launch_probe_merge(const uint2* partial, int nq, int nprobe, const int* fb_index, const int* ex_idx1,
+hipError_t launch_probe_merge(const uint2* partial, const int* probes, int nq, int nprobe, const int* fb_index,
+                              const int* ex_idx1,
                               const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1, int* d1, int* idx2,
                               int* d2, hipStream_t s);
 /* rows idx[i] (or the first n) of q, with labels lab_src[idx[i]] when given */
 hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, int n, uint8_t* qd, int* lab,
                          hipStream_t s);
 /* the k nearest centroids of each query (Hamming), ties to the lower id */
-hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, int* probes,
-                                 float* pmax, bool sorted, hipStream_t s);
+hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, int nb, int k, float mass, float temp,
+                                 int* probes, float* pmax, bool sorted, hipStream_t s);
 /* compaction of the low-confidence queries for the exact fallback */
 hipError_t launch_fallback_compact(const uint8_t* q, int nq, const float* pmax, float thresh, int* fb_count,
                                    int* fb_index, int* fb_queries_idx, uint8_t* fb_q, hipStream_t s);

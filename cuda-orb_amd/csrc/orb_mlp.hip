@@ -160,6 +160,19 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     }
 }
 
+/* Hamming distance of two 32-byte descriptors */
+__device__ __forceinline__ int ham256(const uint4& a0, const uint4& a1, const uint4& b0, const uint4& b1)
+{
+    return __popc(a0.x ^ b0.x) + __popc(a0.y ^ b0.y) + __popc(a0.z ^ b0.z) + __popc(a0.w ^ b0.w) +
+           __popc(a1.x ^ b1.x) + __popc(a1.y ^ b1.y) + __popc(a1.z ^ b1.z) + __popc(a1.w ^ b1.w);
+}
+
+/* the centroid term of output (m, n): cscale * (256 - 2 d), exact in FP32 */
+__device__ __forceinline__ float centroid_term(const GemmArgs& g, int d)
+{
+    return g.cscale * (float)(256 - 2 * d);
+}
+
 constexpr int kNT = 128; /* workgroup tile of the NT products (M and N) */
 
 /* Epilogue of the NT products for one 128 x 128 tile: acc as the MFMA left it
@@ -170,6 +183,26 @@ __device__ __forceinline__ void nt_epilogue(const GemmArgs& g, f32x4 (&acc)[4][4
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
     const int row = lane & 15, grp = lane >> 4;
+    if (g.cbits) {
+        /* acc += the centroid term: this lane's 4 columns (one per block
+           column) against each of its 16 rows */
+        uint4 c[4][2];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint4* cp = (const uint4*)(g.cbits + (long long)min(n0 + wn + 16 * b + row, g.N - 1) * 32);
+            c[b][0] = cp[0];
+            c[b][1] = cp[1];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint4* qp = (const uint4*)(g.qbits + (long long)min(m0 + wm + 16 * a + 4 * grp + r, g.M - 1) * 32);
+                const uint4 q0 = qp[0], q1 = qp[1];
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b][r] += centroid_term(g, ham256(q0, q1, c[b][0], c[b][1]));
+            }
+    }
     if (!g.addin && !g.zmul) {
         /* z = acc + bias staged as a 128 x 128 fp32 tile in LDS (the loop ended
            on a barrier), then written along rows: 16 B per lane */
@@ -240,8 +273,7 @@ __device__ __forceinline__ void nt_epilogue(const GemmArgs& g, f32x4 (&acc)[4][4
 }
 
 /* C = A B with both operands contiguous along K (A(m, k) = A[m*sam + k],
-   B(k, n) = B[n*sbn + k]), plus the optional second K segment A2 B2 in the
-   same accumulators: 128 x 128 workgroup tiles, four wavefronts of
+   B(k, n) = B[n*sbn + k]): 128 x 128 workgroup tiles, four wavefronts of
    64 x 64 (4 x 4 MFMA blocks), 64-deep K steps double-buffered in LDS with the
    next step's global loads issued before the current step's MFMAs.  Rows
    beyond M or N are read clamped and never stored.  Same epilogue as k_gemm. */
@@ -265,15 +297,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
     const int lr = tid >> 3, lk = (tid & 7) * 8;
     const uint16_t* pa[4];
     const uint16_t* pb[4];
-    const uint16_t* pa2[4];
-    const uint16_t* pb2[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const long long mr = min(m0 + lr + 32 * j, g.M - 1), nr = min(n0 + lr + 32 * j, g.N - 1);
         pa[j] = g.A + mr * g.sam + lk;
         pb[j] = g.B + nr * g.sbn + lk;
-        pa2[j] = g.K2 ? g.A2 + mr * g.sa2m + lk : pa[j];
-        pb2[j] = g.K2 ? g.B2 + nr * g.sb2n + lk : pb[j];
     }
     const int sto = (lr * kNLd + lk) * 2; /* byte offset of this thread's LDS stores */
     char* const lA = (char*)&sA[0][0] + sto;
@@ -281,7 +309,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
     constexpr int kBuf = kNT * kNLd * 2, kRow32 = 32 * kNLd * 2;
     const char* rA = (const char*)&sA[0][0] + ((wm + row) * kNLd + 8 * grp) * 2;
     const char* rB = (const char*)&sB[0][0] + ((wn + row) * kNLd + 8 * grp) * 2;
-    const int nk1 = g.K / kNK, nk = nk1 + g.K2 / kNK;
+    const int nk = g.K / kNK;
     uint4 a0 = *(const uint4*)pa[0], a1 = *(const uint4*)pa[1], a2 = *(const uint4*)pa[2], a3 = *(const uint4*)pa[3];
     uint4 b0 = *(const uint4*)pb[0], b1 = *(const uint4*)pb[1], b2 = *(const uint4*)pb[2], b3 = *(const uint4*)pb[3];
     *(uint4*)(lA) = a0;
@@ -302,17 +330,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
         const int cur = (kt & 1) * kBuf, nxt = kBuf - cur;
         /* the next step's tiles, in flight during this step's MFMAs (the last
            step reloads its own: harmless, never stored over live data) */
-        const int kn = min(kt + 1, nk - 1);
-        const bool seg2 = kn >= nk1;
-        const int k1 = (seg2 ? kn - nk1 : kn) * kNK;
-        a0 = *(const uint4*)((seg2 ? pa2[0] : pa[0]) + k1);
-        a1 = *(const uint4*)((seg2 ? pa2[1] : pa[1]) + k1);
-        a2 = *(const uint4*)((seg2 ? pa2[2] : pa[2]) + k1);
-        a3 = *(const uint4*)((seg2 ? pa2[3] : pa[3]) + k1);
-        b0 = *(const uint4*)((seg2 ? pb2[0] : pb[0]) + k1);
-        b1 = *(const uint4*)((seg2 ? pb2[1] : pb[1]) + k1);
-        b2 = *(const uint4*)((seg2 ? pb2[2] : pb[2]) + k1);
-        b3 = *(const uint4*)((seg2 ? pb2[3] : pb[3]) + k1);
+        const int k1 = min(kt + 1, nk - 1) * kNK;
+        a0 = *(const uint4*)(pa[0] + k1);
+        a1 = *(const uint4*)(pa[1] + k1);
+        a2 = *(const uint4*)(pa[2] + k1);
+        a3 = *(const uint4*)(pa[3] + k1);
+        b0 = *(const uint4*)(pb[0] + k1);
+        b1 = *(const uint4*)(pb[1] + k1);
+        b2 = *(const uint4*)(pb[2] + k1);
+        b3 = *(const uint4*)(pb[3] + k1);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(GemmArgs g)
     nt_epilogue(g, acc, (float*)s_lds, m0, n0);
 }
 
-/* The same product with 128-deep K steps (K and K2 multiples of 128): one
+/* The same product with 128-deep K steps (K a multiple of 128): one
    LDS tile of A and one of B (69.6 KB: two workgroups per CU), the next
    step's 16 B loads (buffer loads: rows beyond M or N read as zeros) in
    registers while the 64 MFMAs of the current step run, two barriers per
@@ -363,30 +389,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt128(GemmArgs g)
     const int lr = tid >> 4, lk = (tid & 15) * 8;
     const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void*)g.A, 0, (int)(g.M * g.sam * 2), 0x00020000);
     const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc((void*)g.B, 0, (int)(g.N * g.sbn * 2), 0x00020000);
-    const __amdgpu_buffer_rsrc_t rA2 =
-        __builtin_amdgcn_make_buffer_rsrc((void*)(g.K2 ? g.A2 : g.A), 0, (int)(g.K2 ? g.M * g.sa2m * 2 : 0), 0x00020000);
-    const __amdgpu_buffer_rsrc_t rB2 =
-        __builtin_amdgcn_make_buffer_rsrc((void*)(g.K2 ? g.B2 : g.B), 0, (int)(g.K2 ? g.N * g.sb2n * 2 : 0), 0x00020000);
     const uint32_t vA = (uint32_t)((lr * g.sam + lk) * 2), vB = (uint32_t)((lr * g.sbn + lk) * 2);
-    const uint32_t vA2 = (uint32_t)((lr * g.sa2m + lk) * 2), vB2 = (uint32_t)((lr * g.sb2n + lk) * 2);
     char* const lA = (char*)s_lds + (lr * kNLd2 + lk) * 2;
     char* const lB = lA + kNT * kNLd2 * 2;
     constexpr int kRow16 = 16 * kNLd2 * 2;
     const char* rdA = (const char*)s_lds + ((wm + row) * kNLd2 + 8 * grp) * 2;
     const char* rdB = (const char*)s_lds + (kNT * kNLd2 + (wn + row) * kNLd2 + 8 * grp) * 2;
-    const int nk1 = g.K / kNK2, nk = nk1 + g.K2 / kNK2;
+    const int nk = g.K / kNK2;
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     u32x4 x[16];
     auto gload = [&](int kt) {
-        const bool seg2 = kt >= nk1;
-        const uint32_t kb = (uint32_t)((seg2 ? kt - nk1 : kt) * kNK2 * 2);
-        const __amdgpu_buffer_rsrc_t ra = seg2 ? rA2 : rA, rb = seg2 ? rB2 : rB;
-        const uint32_t va = seg2 ? vA2 : vA, vb = seg2 ? vB2 : vB;
-        const long long sa = seg2 ? g.sa2m : g.sam, sb = seg2 ? g.sb2n : g.sbn;
+        const uint32_t kb = (uint32_t)(kt * kNK2 * 2);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            x[j] = __builtin_amdgcn_raw_buffer_load_b128(ra, va, (uint32_t)((m0 + 16 * j) * sa * 2) + kb, 0);
-            x[8 + j] = __builtin_amdgcn_raw_buffer_load_b128(rb, vb, (uint32_t)((n0 + 16 * j) * sb * 2) + kb, 0);
+            x[j] = __builtin_amdgcn_raw_buffer_load_b128(rA, vA, (uint32_t)((m0 + 16 * j) * g.sam * 2) + kb, 0);
+            x[8 + j] = __builtin_amdgcn_raw_buffer_load_b128(rB, vB, (uint32_t)((n0 + 16 * j) * g.sbn * 2) + kb, 0);
         }
     };
     auto lstore = [&]() {
@@ -429,6 +446,173 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt128(GemmArgs g)
     nt_epilogue(g, acc, (float*)s_lds, m0, n0);
 }
 
+/* The forward products (no addin / zmul; K a multiple of 32) on 256 x
+   128 workgroup tiles (M x N): four wavefronts of 128 x 64 outputs (8 x 4
+   MFMA blocks, 128 FP32 accumulators per lane), 32-deep K steps.  The tiles
+   travel from memory straight into LDS (buffer loads with the LDS bit, 16
+   bytes per lane, rows past M or N read as zeros) through three stages of
+   24 KB, two steps ahead of the MFMAs, with no staging registers; two
+   workgroups share a CU, so one's epilogue overlaps the other's MFMAs.  In
+   a stage a 32-element row is 64 bytes whose four 16-byte chunks sit at
+   slot chunk ^ ((row >> 2) & 3), so the 16 rows of a fragment read hit 16
+   distinct bank groups.  The MFMA takes the weight rows as its first operand
+   and the activation rows as its second, so a lane holds four consecutive
+   output columns of one row: bias and GELU are applied in registers, each
+   wavefront stages 16 rows at a time in LDS and writes them back as whole
+   row segments (16 bytes per lane).  The K order of the sums is the 128 x
+   128 kernels' (32-deep MFMA steps in K order), so the results are the
+   same. */
+constexpr int kFM = 256, kFN = 128;
+constexpr int kFK = 32;
+constexpr int kFStage = (kFM + kFN) * kFK * 2; /* bytes */
+#define ORB_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
+
+__global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
+{
+    __shared__ __align__(16) char s_st[3 * kFStage];
+    const int tiles_n = (g.N + kFN - 1) / kFN, u = xcd_remap(blockIdx.x, gridDim.x);
+    const int m0 = (u / tiles_n) * kFM, n0 = (u % tiles_n) * kFN;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6); /* uniform: the DMA's LDS address goes to M0 */
+    const int wm = (wid >> 1) * 128, wn = (wid & 1) * 64;
+    const int row = lane & 15, grp = lane >> 4;
+    const __amdgpu_buffer_rsrc_t rA = __builtin_amdgcn_make_buffer_rsrc((void*)g.A, 0, (int)(g.M * g.sam * 2), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc((void*)g.B, 0, (int)(g.N * g.sbn * 2), 0x00020000);
+    /* DMA: instruction j of wavefront wid fills the 16-row block q = wid + 4 j
+       of a stage (blocks 0..15 the activation rows, 16..23 the weight rows);
+       lane -> row lane >> 2 of the block, slot lane & 3, which holds chunk
+       (lane & 3) ^ ((lane >> 4) & 3) of that row */
+    const int drow = lane >> 2, dch = (lane & 3) ^ ((lane >> 4) & 3);
+    const uint32_t vA = (uint32_t)((drow * g.sam + 8 * dch) * 2), vB = (uint32_t)((drow * g.sbn + 8 * dch) * 2);
+    const int nk = g.K / kFK;
+    auto issue = [&](char* st, int kt) {
+        const uint32_t kb = (uint32_t)(kt * kFK * 2);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rA, ORB_LDS_PTR(st + (wid + 4 * j) * 1024), 16, vA,
+                                                     (uint32_t)(m0 + 16 * (wid + 4 * j)) * (uint32_t)g.sam * 2 + kb,
+                                                     0, 0);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rB, ORB_LDS_PTR(st + (16 + wid + 4 * j) * 1024), 16, vB,
+                                                     (uint32_t)(n0 + 16 * (wid + 4 * j)) * (uint32_t)g.sbn * 2 + kb,
+                                                     0, 0);
+    };
+    /* fragment reads: row `row` of a 16-row block, chunk grp at its slot */
+    const int rdo = row * 64 + 16 * (grp ^ ((lane >> 2) & 3));
+    f32x4 acc[4][8];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int a = 0; a < 8; a++) acc[b][a] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    /* the activation fragments in two halves of four (fewer live registers) */
+    auto mfma_step = [&](const char* st) {
+        const char* const px = st + wm * 64 + rdo;
+        const char* const pw = st + kFM * 64 + wn * 64 + rdo;
+        bf16x8 wv[4], xv[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) wv[b] = *(const bf16x8*)(pw + 16 * b * 64);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) xv[a] = *(const bf16x8*)(px + 16 * (4 * h + a) * 64);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[b][4 * h + a] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv[b], xv[a], acc[b][4 * h + a], 0, 0, 0);
+        }
+    };
+    /* step kt: wait for its stage (this wavefront's six loads of it; those of
+       step kt + 1 may stay in flight) and for the reads of the stage about to
+       be refilled, barrier, refill that stage with step kt + 2, MFMAs */
+    auto step = [&](char* cur, char* refill, int kt) {
+        if (kt + 1 < nk) __builtin_amdgcn_s_waitcnt(0x0076); /* vmcnt(6) lgkmcnt(0) */
+        else __builtin_amdgcn_s_waitcnt(0x0070);             /* vmcnt(0) lgkmcnt(0) */
+        __builtin_amdgcn_s_barrier();
+        if (kt + 2 < nk) issue(refill, kt + 2);
+        mfma_step(cur);
+    };
+    issue(s_st, 0);
+    if (nk > 1) issue(s_st + kFStage, 1);
+    for (int kt = 0, c = 0; kt < nk; kt++, c = c == 2 ? 0 : c + 1)
+        step(s_st + c * kFStage, s_st + (c == 0 ? 2 : c - 1) * kFStage, kt);
+    __syncthreads(); /* every load landed and every read done: stage 0 becomes the epilogue's */
+    /* epilogue: lane holds z[m][n .. n+3] of each block (m = the block's row
+       `row`, n = 4 grp); per 16-row block of the wavefront's tile the values go
+       through this wavefront's 16 x 64 stage (the K loop ended on a barrier) */
+    if (g.cbits) {
+        /* acc += the centroid term: per block column this lane's 4 columns
+           against each of its 8 rows */
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            uint4 c[4][2];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint4* cp = (const uint4*)(g.cbits + (long long)min(n0 + wn + 16 * b + 4 * grp + i, g.N - 1) * 32);
+                c[i][0] = cp[0];
+                c[i][1] = cp[1];
+            }
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                const uint4* qp = (const uint4*)(g.qbits + (long long)min(m0 + wm + 16 * a + row, g.M - 1) * 32);
+                const uint4 q0 = qp[0], q1 = qp[1];
+#pragma unroll
+                for (int i = 0; i < 4; i++) acc[b][a][i] += centroid_term(g, ham256(q0, q1, c[i][0], c[i][1]));
+            }
+        }
+    }
+    float* const st = (float*)s_st + wid * 16 * 68;
+    float4 bias[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int n = n0 + wn + 16 * b + 4 * grp;
+        bias[b] = g.bias && n < g.N ? *(const float4*)(g.bias + n) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            *(float4*)(st + row * 68 + 16 * b + 4 * grp) =
+                make_float4(acc[b][a][0] + bias[b].x, acc[b][a][1] + bias[b].y, acc[b][a][2] + bias[b].z,
+                            acc[b][a][3] + bias[b].w);
+        wave_sync_lds();
+        if (g.out32) {
+            /* 16 lanes per row, 4 rows per pass */
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const int rl = 4 * p + (lane >> 4), cl = 4 * (lane & 15);
+                const int m = m0 + wm + 16 * a + rl, n = n0 + wn + cl;
+                if (m < g.M && n < g.N)
+                    *(float4*)(g.out32 + (long long)m * g.ldc + n) = *(const float4*)(st + rl * 68 + cl);
+            }
+        }
+        if (g.out16) {
+            /* 8 lanes per row, 8 rows per pass */
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int rl = 8 * p + (lane >> 3), cl = 8 * (lane & 7);
+                const int m = m0 + wm + 16 * a + rl, n = n0 + wn + cl;
+                if (m < g.M && n < g.N) {
+                    const float4 lo = *(const float4*)(st + rl * 68 + cl);
+                    const float4 hi = *(const float4*)(st + rl * 68 + cl + 4);
+                    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    uint32_t w[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
+                        const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
+                        w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
+                    }
+                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            }
+        }
+        wave_sync_lds(); /* the stage is rewritten for the next block */
+    }
+}
+
 /* dst (C x R) = src (R x C) transposed, bf16 bits; R and C multiples of 64 */
 __global__ __launch_bounds__(256) void k_transpose16(const uint16_t* __restrict__ src, int R, int C,
                                                      uint16_t* __restrict__ dst)
@@ -446,9 +630,9 @@ __global__ __launch_bounds__(256) void k_transpose16(const uint16_t* __restrict_
 
 /* descriptor bits -> +/-1 bf16 (bit k of the descriptor -> column k): one
    thread per descriptor byte, 8 bf16 values in one 16-byte store */
-__global__ void k_bits_pm1(const uint8_t* __restrict__ desc, int n, uint16_t pos, uint16_t neg,
-                           uint16_t* __restrict__ X)
+__global__ void k_bits_pm1(const uint8_t* __restrict__ desc, int n, uint16_t* __restrict__ X)
 {
+    const uint32_t pos = 0x3F80u, neg = 0xBF80u; /* bf16 +1, -1 */
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)n * 32) return;
     const uint32_t b = desc[i];
@@ -459,6 +643,23 @@ __global__ void k_bits_pm1(const uint8_t* __restrict__ desc, int n, uint16_t pos
         w[t] = lo | (hi << 16);
     }
     *(uint4*)(X + i * 8) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+/* acc[0] += loss[0] + .. + loss[n-1]: one workgroup, a fixed summation order
+   (strided per-thread sums, then a tree), so the reported loss of a training
+   run does not depend on scheduling */
+__global__ __launch_bounds__(1024) void k_loss_sum(const float* __restrict__ loss, int n, float* __restrict__ acc)
+{
+    __shared__ float s_part[1024];
+    float t = 0.f;
+    for (int i = threadIdx.x; i < n; i += 1024) t += loss[i];
+    s_part[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s_part[threadIdx.x] += s_part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[0] += s_part[0];
 }
 
 /* one wavefront per row of C logits: loss and bf16 gradient (softmax - onehot) / n */
@@ -541,16 +742,26 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
     if (i < n) y[i] = f2bf(x[i]);
 }
 
-/* Max softmax probability and the top-k logits of query r, one wavefront
-   (ties to the lower bucket id; C <= 1024, a multiple of 64), by descending
-   logit with SORTED, else by bucket id; cand holds CAP candidates.  False:
-   more than CAP candidates, nothing written to the probes. */
+/* candidates one wavefront of the first probe pass holds in LDS (2 KB) */
+constexpr int kProbeCap = 256;
+
+/* Max softmax probability (*pmax_r) and the top-k logits (out[0 .. k)) of
+   the logit row z, one wavefront (ties to the lower bucket id; C <= 1024, a
+   multiple of 64), by descending logit with SORTED, else by bucket id; cand
+   holds CAP candidates.  False: more than CAP candidates, nothing written to
+   the probes.  With mass below 1
+   (k <= 64) only the best-ranked prefix whose softmax mass reaches `mass` is
+   probed: the top-k are ranked (logit descending, ties to the lower id), the
+   weights exp(z - max) of the ranks before a candidate are summed
+   (wave_mass_keep), and the candidate is kept iff that sum is below mass
+   times the row's sum of exp(z - max); the row then holds the kept buckets
+   best first and -1 in the remaining slots, whatever SORTED says.  ids: 64
+   ints of LDS for this wavefront. */
 template <bool SORTED, int CAP>
-__device__ __forceinline__ bool probe_row(const float* __restrict__ Z, int r, int C, int k, int* __restrict__ probes,
-                                          float* __restrict__ pmax, uint2* cand)
+__device__ __forceinline__ bool probe_row(const float* __restrict__ z, int C, int k, float mass, int* __restrict__ out,
+                                          float* __restrict__ pmax_r, uint2* cand, int* ids)
 {
     const int lane = threadIdx.x & 63;
-    const float* z = Z + (long long)r * C;
     /* C is a multiple of 64: slot t is present in every lane or in none */
     float v[16];
 #pragma unroll
@@ -565,56 +776,68 @@ __device__ __forceinline__ bool probe_row(const float* __restrict__ Z, int r, in
     for (int t = 0; t < 16; t++) s += 64 * t < C ? __expf(v[t] - m) : 0.f;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) pmax[r] = 1.f / s;
+    if (lane == 0) *pmax_r = 1.f / s;
     uint32_t key[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) key[t] = 64 * t < C ? float_order_key(v[t]) : 0u;
-    return wave_topk<31, SORTED, CAP>(key, k, cand, probes + (long long)r * k);
+    if (!(mass < 1.f)) return wave_topk<31, SORTED, CAP>(key, k, cand, out);
+    /* the top k best first into LDS, then the rule over lane = rank */
+    if (!wave_topk<31, true, CAP>(key, k, cand, ids)) return false;
+    wave_sync_lds();
+    const int id = lane < k ? ids[lane] : 0;
+    wave_mass_keep(id, __expf(z[id] - m), k, mass * s, out);
+    wave_sync_lds(); /* ids are reused by the wavefront's next query */
+    return true;
 }
 
 /* one wavefront per query.  With CAP below 1024 (LDS for CAP candidates per
    wavefront, which leaves room for more resident wavefronts) a query whose
    candidates do not fit gets -1 as its first probe, for k_probe_fix. */
 template <bool SORTED, int CAP>
-__global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k,
+__global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k, float mass,
                                                int* __restrict__ probes, float* __restrict__ pmax)
 {
     __shared__ uint2 s_cand[4][CAP];
+    __shared__ int s_ids[4][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + wv;
     if (r >= n) return;
-    if (!probe_row<SORTED, CAP>(Z, r, C, k, probes, pmax, s_cand[wv]) && lane == 0) probes[(long long)r * k] = -1;
+    if (!probe_row<SORTED, CAP>(Z + (long long)r * C, C, k, mass, probes + (long long)r * k, pmax + r, s_cand[wv],
+                                s_ids[wv]) &&
+        lane == 0)
+        probes[(long long)r * k] = -1;
 }
 
 /* the queries k_probe marked, redone with room for every candidate: each
    wavefront reads the marks of 64 queries and takes the marked ones in turn */
 template <bool SORTED>
-__global__ __launch_bounds__(256) void k_probe_fix(const float* __restrict__ Z, int n, int C, int k,
+__global__ __launch_bounds__(256) void k_probe_fix(const float* __restrict__ Z, int n, int C, int k, float mass,
                                                    int* __restrict__ probes, float* __restrict__ pmax)
 {
     __shared__ uint2 s_cand[4][1024];
+    __shared__ int s_ids[4][64];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int base = (blockIdx.x * 4 + wv) * 64;
     const bool redo = base + lane < n && probes[(long long)(base + lane) * k] < 0;
     for (uint64_t m = __ballot(redo); m; m &= m - 1) {
         const int r = base + __ffsll((unsigned long long)m) - 1;
-        probe_row<SORTED, 1024>(Z, r, C, k, probes, pmax, s_cand[wv]);
+        probe_row<SORTED, 1024>(Z + (long long)r * C, C, k, mass, probes + (long long)r * k, pmax + r, s_cand[wv],
+                                s_ids[wv]);
         wave_sync_lds(); /* the next query reuses s_cand */
     }
 }
 
-/* candidates one wavefront of the first probe pass holds in LDS (2 KB) */
-constexpr int kProbeCap = 256;
 
 template <bool SORTED>
-static void probe_launch(const float* Z, int n, int C, int k, int* probes, float* pmax, hipStream_t s)
+static void probe_launch(const float* Z, int n, int C, int k, float mass, int* probes, float* pmax, hipStream_t s)
 {
     const dim3 grid((n + 3) / 4);
     if (k <= 64 && C > kProbeCap) {
-        hipLaunchKernelGGL((k_probe<SORTED, kProbeCap>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
-        hipLaunchKernelGGL((k_probe_fix<SORTED>), dim3((n + 255) / 256), dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        hipLaunchKernelGGL((k_probe<SORTED, kProbeCap>), grid, dim3(256), 0, s, Z, n, C, k, mass, probes, pmax);
+        hipLaunchKernelGGL((k_probe_fix<SORTED>), dim3((n + 255) / 256), dim3(256), 0, s, Z, n, C, k, mass, probes,
+                           pmax);
     } else {
-        hipLaunchKernelGGL((k_probe<SORTED, 1024>), grid, dim3(256), 0, s, Z, n, C, k, probes, pmax);
+        hipLaunchKernelGGL((k_probe<SORTED, 1024>), grid, dim3(256), 0, s, Z, n, C, k, mass, probes, pmax);
     }
 }
 
@@ -624,15 +847,19 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.M % kGT || g.N % kGT || g.K % kNK) return hipErrorInvalidValue;
     const bool nt = g.sak == 1 && g.sbk == 1 && g.sam % 8 == 0 && g.sbn % 8 == 0;
-    if (g.K2 && (!nt || g.K2 < 0 || g.K2 % kNK || !g.A2 || !g.B2 || g.sa2m % 8 || g.sb2n % 8))
-        return hipErrorInvalidValue;
+    if (g.cbits && (!g.qbits || g.addin || g.zmul || g.sak != 1 || g.sbk != 1)) return hipErrorInvalidValue;
     if (nt) {
         const int tiles = ((g.M + kNT - 1) / kNT) * ((g.N + kNT - 1) / kNT);
         /* 128-deep steps when both K segments allow them and the operands fit 32-bit buffer offsets */
-        const bool deep = g.K % kNK2 == 0 && g.K2 % kNK2 == 0 && (long long)g.M * g.sam * 2 < (1ll << 31) &&
-                          (long long)g.N * g.sbn * 2 < (1ll << 31) && (long long)g.M * g.sa2m * 2 < (1ll << 31) &&
-                          (long long)g.N * g.sb2n * 2 < (1ll << 31);
-        if (deep) hipLaunchKernelGGL(k_gemm_nt128, dim3(tiles), dim3(256), 0, s, g);
+        const bool deep = g.K % kNK2 == 0 && (long long)g.M * g.sam * 2 < (1ll << 31) &&
+                          (long long)g.N * g.sbn * 2 < (1ll << 31);
+        const bool big = !g.addin && !g.zmul && g.M >= 4 * kFM && g.N >= kFN && g.K % kFK == 0 &&
+                         g.ldc % 8 == 0 && (long long)g.M * g.sam * 2 < (1ll << 31) &&
+                         (long long)g.N * g.sbn * 2 < (1ll << 31);
+        if (big) {
+            const int tiles_big = ((g.M + kFM - 1) / kFM) * ((g.N + kFN - 1) / kFN);
+            hipLaunchKernelGGL(k_gemm_fwd, dim3(tiles_big), dim3(256), 0, s, g);
+        } else if (deep) hipLaunchKernelGGL(k_gemm_nt128, dim3(tiles), dim3(256), 0, s, g);
         else hipLaunchKernelGGL(k_gemm_nt, dim3(tiles), dim3(256), 0, s, g);
     } else {
         hipLaunchKernelGGL(k_gemm, dim3((g.M / kGT) * (g.N / kGT)), dim3(256), 0, s, g);
@@ -647,22 +874,23 @@ hipError_t launch_transpose16(const uint16_t* src, int R, int C, uint16_t* dst, 
     return hipGetLastError();
 }
 
-hipError_t launch_bits_pm(const uint8_t* desc, int n, uint16_t pos, uint16_t neg, uint16_t* X, hipStream_t s)
+hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s)
 {
     const long long tot = (long long)n * 32;
     if (tot > 0)
-        hipLaunchKernelGGL(k_bits_pm1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, desc, n, pos, neg, X);
+        hipLaunchKernelGGL(k_bits_pm1, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, desc, n, X);
     return hipGetLastError();
-}
-
-hipError_t launch_bits_pm1(const uint8_t* desc, int n, uint16_t* X, hipStream_t s)
-{
-    return launch_bits_pm(desc, n, (uint16_t)0x3F80, (uint16_t)0xBF80, X, s);
 }
 
 hipError_t launch_softmax_ce(const float* Z, const int* labels, int n, int C, uint16_t* dZ, float* loss, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, Z, labels, n, C, dZ, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_loss_sum(const float* loss, int n, float* acc, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(1024), 0, s, loss, n, acc);
     return hipGetLastError();
 }
 
@@ -687,12 +915,14 @@ hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_probe(const float* Z, int n, int C, int k, int* probes, float* pmax, bool sorted, hipStream_t s)
+hipError_t launch_probe(const float* Z, int n, int C, int k, float mass, int* probes, float* pmax, bool sorted,
+                        hipStream_t s)
 {
-    if (C <= 0 || C > 1024 || C % 64 || k < 1 || k > C) return hipErrorInvalidValue;
+    if (C <= 0 || C > 1024 || C % 64 || k < 1 || k > C || (mass < 1.f && (k > 64 || !(mass > 0.f))))
+        return hipErrorInvalidValue;
     if (n > 0) {
-        if (sorted) probe_launch<true>(Z, n, C, k, probes, pmax, s);
-        else probe_launch<false>(Z, n, C, k, probes, pmax, s);
+        if (sorted) probe_launch<true>(Z, n, C, k, mass, probes, pmax, s);
+        else probe_launch<false>(Z, n, C, k, mass, probes, pmax, s);
     }
     return hipGetLastError();
 }

@@ -194,4 +194,34 @@ __device__ __forceinline__ bool wave_topk(const uint32_t (&key)[16], int k, uint
     return true;
 }
 
+/* Inclusive prefix sum of a float over the 64 lanes (the pattern of
+   wave_incl_scan, FP32 adds in its tree order).  Every lane must be active. */
+__device__ __forceinline__ float wave_incl_scan_f(float v)
+{
+#define ORB_DPP_ADD(ctrl, rmask, bc)                                                                       \
+    v = __fadd_rn(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, rmask, 0xf, bc)))
+    ORB_DPP_ADD(0x111, 0xf, true);  /* row_shr:1 */
+    ORB_DPP_ADD(0x112, 0xf, true);  /* row_shr:2 */
+    ORB_DPP_ADD(0x114, 0xf, true);  /* row_shr:4 */
+    ORB_DPP_ADD(0x118, 0xf, true);  /* row_shr:8 */
+    ORB_DPP_ADD(0x142, 0xa, false); /* row_bcast:15 */
+    ORB_DPP_ADD(0x143, 0xc, false); /* row_bcast:31 */
+#undef ORB_DPP_ADD
+    return v;
+}
+
+/* Probability-mass rule over the k best candidates, lane r holding the
+   candidate of rank r (r < k <= 64) with weight w: the candidate of rank r is
+   kept iff the weights of ranks 0 .. r-1 (the inclusive FP32 prefix sum in
+   the wave_incl_scan_f order, minus its own weight) sum to less than
+   `limit` -- rank 0 is always kept and the kept set is a prefix of the
+   ranking.  Writes out[r] = its id if kept, else -1, for r < k.  Every lane
+   must be active. */
+__device__ __forceinline__ void wave_mass_keep(int id, float w, int k, float limit, int* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const float before = __fsub_rn(wave_incl_scan_f(lane < k ? w : 0.f), lane < k ? w : 0.f);
+    if (lane < k) out[lane] = before < limit ? id : -1;
+}
+
 } // namespace orbk

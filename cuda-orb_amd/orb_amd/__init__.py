@@ -44,6 +44,7 @@ EXPORTS = [
     "orb_lindex_create", "orb_lindex_destroy", "orb_lindex_build", "orb_lindex_bucket_info", "orb_lindex_train",
     "orb_lindex_logits", "orb_lindex_grads", "orb_lindex_num_params", "orb_lindex_get_weights",
     "orb_lindex_set_weights", "orb_lindex_query", "orb_lindex_probes", "orb_lindex_nprobe", "orb_lindex_save", "orb_lindex_load", "orb_lindex_set_probe",
+    "orb_lindex_set_mass",
 ]
 
 ORB_LINDEX_MLP = 0
@@ -131,6 +132,7 @@ def lib():
         L.orb_lindex_save.argtypes = [P, ctypes.c_char_p]
         L.orb_lindex_load.argtypes = [P, ctypes.c_char_p]
         L.orb_lindex_set_probe.argtypes = [P, I, ctypes.c_float]
+        L.orb_lindex_set_mass.argtypes = [P, ctypes.c_float, ctypes.c_float]
         _lib = L
     return _lib
 
@@ -500,6 +502,11 @@ class LearnedIndex:
 
     def set_probe(self, nprobe, fallback):
         self._c(lib().orb_lindex_set_probe(self.h, nprobe, fallback), "orb_lindex_set_probe")
+
+    def set_mass(self, mass, centroid_temperature=4.0):
+        """Probe only the best-ranked prefix of the nprobe buckets whose probability mass reaches
+        `mass` (1 = every one); centroid-mode weights exp(-distance / centroid_temperature)."""
+        self._c(lib().orb_lindex_set_mass(self.h, mass, centroid_temperature), "orb_lindex_set_mass")
 
     def save(self, path):
         self._c(lib().orb_lindex_save(self.h, str(path).encode()), "orb_lindex_save")

@@ -252,8 +252,21 @@ int orb_lindex_set_weights(orb_lindex* lx, const float* h_w);
 int orb_lindex_probes(orb_lindex* lx, const uint8_t* d_q, int nq, int mode, int* d_probes, float* d_pmax);
 /* Current number of probed buckets (set at creation, by orb_lindex_set_probe or by a load). */
 int orb_lindex_nprobe(const orb_lindex* lx);
-/* Change the number of probed buckets (1..nbuckets) and the fallback threshold. */
+/* Change the number of probed buckets (1..nbuckets; at most 64 while a
+   probability-mass rule is set) and the fallback threshold. */
 int orb_lindex_set_probe(orb_lindex* lx, int nprobe, float fallback);
+/* Probability-mass rule of the probes (mass in (0, 1]; 1, the default, probes
+   all nprobe buckets): of the nprobe best buckets, ranked as
+   orb_lindex_probes ranks them, a query probes the best-ranked prefix whose
+   probability mass reaches `mass` -- a bucket is probed iff the weights of
+   the buckets ranked before it sum to less than mass times the sum of all
+   weights.  Weights: the softmax of the network's logits, or in
+   ORB_LINDEX_CENTROIDS mode exp(-d / centroid_temperature) with d the
+   Hamming distance to the centroid.  orb_lindex_probes then writes the
+   probed buckets best first and -1 in the remaining slots.  Needs
+   nprobe <= 64; a load whose nprobe is above 64 sets mass back to 1.  No
+   counterpart in the reference (the learned matcher is an addition). */
+int orb_lindex_set_mass(orb_lindex* lx, float mass, float centroid_temperature);
 /* Nearest and second-nearest database entries of nq queries (multiple of 64);
    *h_nfallback (may be NULL) receives how many went to the exact search. */
 int orb_lindex_query(orb_lindex* lx, const uint8_t* d_q, int nq, int mode, int* d_idx1, int* d_d1, int* d_idx2,

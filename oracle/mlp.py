@@ -113,10 +113,24 @@ def probe(logits, nprobe=8):
     return order[:, :nprobe], pmax
 
 
+def probe_mass(scores, weights, nprobe, mass):
+    """The probability-mass rule over the top-nprobe buckets: buckets ranked by score (higher
+    first, ties to the lower bucket id); the bucket of rank r is kept iff the weights of ranks
+    0 .. r-1 sum to less than mass times the row's total weight (float64 here).  Returns
+    (n, nprobe) bucket ids, the kept prefix best first and -1 after it."""
+    scores = np.asarray(scores, np.float64)
+    weights = np.asarray(weights, np.float64)
+    order = np.lexsort((np.broadcast_to(np.arange(scores.shape[1]), scores.shape), -scores), axis=1)[:, :nprobe]
+    w = np.take_along_axis(weights, order, axis=1)
+    before = np.cumsum(w, axis=1) - w
+    keep = before < mass * weights.sum(1, keepdims=True)
+    return np.where(keep, order, -1)
+
+
 def bucket_search(q, db, bucket_of, buckets):
     """Exact top-2 of each query restricted to the database entries whose bucket is in its probe
-    list, ties to the lowest database index.  Returns (idx1, d1, idx2, d2); a missing neighbour is
-    -1 / 257 (orb_match_top2's convention)."""
+    list (entries below 0 are empty slots), ties to the lowest database index.  Returns (idx1, d1,
+    idx2, d2); a missing neighbour is -1 / 257 (orb_match_top2's convention)."""
     q64 = np.ascontiguousarray(q, np.uint8).view(np.uint64)
     db64 = np.ascontiguousarray(db, np.uint8).view(np.uint64)
     bucket_of = np.asarray(bucket_of)
@@ -126,6 +140,7 @@ def bucket_search(q, db, bucket_of, buckets):
     out = [np.full(len(q), v, np.int64) for v in (-1, 257, -1, 257)]
     for i in range(len(q)):
         probed = np.unique(buckets[i])
+        probed = probed[probed >= 0]
         cand = np.sort(np.concatenate([members[starts[b]:starts[b + 1]] for b in probed]))
         if len(cand) == 0:
             continue

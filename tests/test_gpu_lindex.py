@@ -43,9 +43,11 @@ def flip_copies(rng, db, n, p=0.05):
     return db[src] ^ flips, src
 
 
-# (rows, hidden, buckets, centroid_scale); 192 x 192 x 320 leaves partial 128-row tiles in every product
+# (rows, hidden, buckets, centroid_scale); 192 x 192 x 320 leaves partial 128-row tiles in every product;
+# 1024 rows and more take the 256 x 256-tile forward kernel (1088 x 320 x 384: partial tiles in M and N)
 @pytest.mark.parametrize("n,hidden,nb,cscale", [(256, 512, 1024, 0.0), (256, 128, 256, 0.0), (256, 128, 256, 0.25),
-                                                (192, 192, 320, 0.0)])
+                                                (192, 192, 320, 0.0), (4096, 512, 1024, 0.125),
+                                                (1088, 320, 384, 0.25)])
 def test_gradients_match_oracle(torch, ctx, n, hidden, nb, cscale):
     import orb_amd
     from oracle import mlp
@@ -77,6 +79,11 @@ def test_gradients_match_oracle(torch, ctx, n, hidden, nb, cscale):
         assert err < GRAD_RTOL, k
     logits = lx.logits(torch.from_numpy(q).cuda()).cpu().numpy()
     assert np.linalg.norm(logits - f["Z3"]) / np.linalg.norm(f["Z3"]) < 1e-2
+    if n >= 1024:
+        # the first 256 rows alone go through the 128 x 128-tile kernels: the same values
+        head = lx.logits(torch.from_numpy(q[:256]).cuda()).cpu().numpy()
+        print(f"  256-row logits bit-identical to the large-batch kernel's: {np.array_equal(head, logits[:256])}")
+        np.testing.assert_allclose(head, logits[:256], rtol=1e-5, atol=1e-5)
     lx.close()
 
 
@@ -191,6 +198,87 @@ def test_probe_selection_with_many_candidates(torch, ctx, case):
                 assert_top2(got, mlp.bucket_search(q, db, bof, probes), (case, mode, nprobe))
     finally:
         lx.close()
+
+
+def assert_mass_probes(got, scores, weights, nprobe, mass, what):
+    """The library decides the mass rule in FP32 (its own exp), the oracle in FP64: a row must
+    equal the oracle's at mass (1 - 1e-4) or at mass (1 + 1e-4), and nine rows in ten must sit
+    far enough from the threshold for the two to agree."""
+    from oracle import mlp
+    lo = mlp.probe_mass(scores, weights, nprobe, mass * (1 - 1e-4))
+    hi = mlp.probe_mass(scores, weights, nprobe, mass * (1 + 1e-4))
+    ok = (got == lo).all(1) | (got == hi).all(1)
+    bad = np.nonzero(~ok)[0]
+    assert len(bad) == 0, f"{what}: {len(bad)} rows differ, first {bad[:3]}: {got[bad[:1]]} vs {lo[bad[:1]]}"
+    assert (lo == hi).all(1).mean() > 0.9, what
+    return (got >= 0).sum(1)
+
+
+@pytest.mark.parametrize("nb,mass", [(256, 0.5), (256, 0.9), (1024, 0.6), (1024, 0.95)])
+def test_probe_mass_rule_matches_oracle(torch, ctx, nb, mass):
+    """The probability-mass rule: of the 32 best buckets a query probes the best-ranked prefix
+    whose softmax mass (network) or exp(-distance / T) mass (centroids) reaches `mass`, -1 after
+    it; the search covers exactly the kept buckets.  1024 buckets also take the probe's
+    two-pass path (at most 256 candidates per wavefront in the first pass)."""
+    import orb_amd
+    from oracle import mlp
+    rng = np.random.default_rng(31 + nb)
+    db = rng.integers(0, 256, (20000, 32), dtype=np.uint8)
+    print(f"mass rule: {nb} buckets, mass {mass}", flush=True)
+    ddb = torch.from_numpy(db).cuda()
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=nb, hidden=128, nprobe=32, fallback=0.0, seed=9, centroid_scale=0.125)
+    try:
+        lx.build(ddb, iters=2)
+        # a few training steps give the logits some spread
+        q_tr, src = flip_copies(rng, db, 4096, p=0.1)
+        _, _, bof = lx.bucket_info()
+        lx.train(torch.from_numpy(q_tr).cuda(), torch.from_numpy(src.astype(np.int32)).cuda(), epochs=2,
+                 batch=1024, lr=3e-3)
+        q, _ = flip_copies(rng, db, 640, p=0.1)
+        dq = torch.from_numpy(q).cuda()
+        bof = bof.cpu().numpy()
+        _, cen, _ = lx.bucket_info()
+        D = (np.unpackbits(q, axis=1)[:, None, :] != np.unpackbits(cen, axis=1)[None, :, :]).sum(-1)
+        T = 2.5
+        lx.set_mass(mass, T)
+        logits = lx.logits(dq).cpu().numpy().astype(np.float64)
+        for mode in (orb_amd.ORB_LINDEX_MLP, orb_amd.ORB_LINDEX_CENTROIDS):
+            if mode == orb_amd.ORB_LINDEX_MLP:
+                scores, w = logits, np.exp(logits - logits.max(1, keepdims=True))
+            else:
+                scores, w = -D.astype(np.float64), np.exp((D.min(1, keepdims=True) - D) / T)
+            got = lx.probes(dq, mode=mode)[0].cpu().numpy()
+            kept = assert_mass_probes(got, scores, w, 32, mass, (nb, mass, mode))
+            print(f"  mode {mode}: probes kept min {kept.min()} mean {kept.mean():.2f} max {kept.max()}", flush=True)
+            res, nfb = lx.query(dq, mode=mode)
+            assert nfb == 0
+            assert_top2(res, mlp.bucket_search(q, db, bof, got), (nb, mass, mode))
+        # the rule off again: every one of the nprobe buckets
+        lx.set_mass(1.0, T)
+        assert (lx.probes(dq)[0].cpu().numpy() >= 0).all()
+    finally:
+        lx.close()
+
+
+def test_probe_mass_rule_arguments(torch, ctx, small_index):
+    """mass outside (0, 1], a non-positive temperature, and nprobe above 64 with the rule set
+    are refused; the rule and nprobe stay as they were."""
+    import orb_amd
+    lx, db, ddb = small_index
+    for mass, temp in ((0.0, 4.0), (1.5, 4.0), (-0.5, 4.0), (0.9, 0.0), (0.9, -1.0), (float("nan"), 4.0)):
+        with pytest.raises(RuntimeError):
+            lx.set_mass(mass, temp)
+    lx.set_probe(100, 0.0)
+    with pytest.raises(RuntimeError):
+        lx.set_mass(0.9, 4.0)
+    lx.set_probe(4, 0.3)
+    lx.set_mass(0.9, 4.0)
+    try:
+        with pytest.raises(RuntimeError):
+            lx.set_probe(65, 0.0)
+        assert orb_amd.lib().orb_lindex_nprobe(lx.h) == 4
+    finally:
+        lx.set_mass(1.0, 4.0)
 
 
 def test_centroid_probe_search_equals_brute_force(torch, ctx, small_index):

@@ -135,3 +135,28 @@ def test_bucketed_top2_restricts_to_probed_buckets():
     i1, d1, i2, d2 = mlp.bucket_search(q, db, bucket_of, probes)
     assert list(i1) == [1, 3, 5, 0] and list(d1) == [1, 3, 0, 0]
     assert list(i2) == [2, -1, 4, -1] and list(d2) == [2, 257, 4, 257]
+
+
+def test_probe_mass_rule_against_brute_force():
+    """The mass rule of the oracle (probe_mass) against a direct restatement over every prefix:
+    for each row, the kept buckets are the shortest best-first prefix of the top nprobe whose
+    weight reaches mass x the row total (all nprobe if none does), ties ranked by bucket id;
+    a hand case with ties fixes the order."""
+    rng = np.random.default_rng(12)
+    for trial in range(200):
+        C = int(rng.integers(4, 40))
+        k = int(rng.integers(1, C + 1))
+        scores = rng.integers(-3, 4, C).astype(np.float64) if trial % 2 else rng.normal(0, 2, C)
+        w = np.exp(scores - scores.max())
+        mass = float(rng.choice([0.3, 0.5, 0.9, 0.99, 1.0]))
+        got = mlp.probe_mass(scores[None, :], w[None, :], k, mass)[0]
+        ranked = sorted(range(C), key=lambda b: (-scores[b], b))[:k]
+        want = []
+        for b in ranked:
+            want.append(b)
+            if sum(w[c] for c in want) >= mass * w.sum():
+                break
+        assert list(got[:len(want)]) == want and (got[len(want):] == -1).all(), (trial, got, want)
+    # ties: equal scores ranked by bucket id; mass 0.5 of four equal weights keeps two
+    got = mlp.probe_mass(np.zeros((1, 4)), np.ones((1, 4)), 4, 0.5)[0]
+    assert list(got) == [0, 1, -1, -1]
