@@ -345,6 +345,16 @@ def learned_match(ctx, torch, dev, log=None):
               "recall_idx": r["centroid_recall_idx"], "mean_probes": r["centroid_mean_probes"],
               "speedup": round(ms_exact / r["centroid_ms"], 3)} for r in mass_rows if r["centroid_recall_idx"] >= 0.99]
     best_c = min(ok_c, key=lambda r: r["ms"]) if ok_c else None
+    # the two winners once more, back to back (network, then centroids): 1 + 3 query batches each, the
+    # last launches of the run, so a kernel trace of it ends with them
+    for row, mode in ((best, orb_amd.ORB_LINDEX_MLP), (best_c, orb_amd.ORB_LINDEX_CENTROIDS)):
+        if row is None:
+            continue
+        lx.set_probe(row["nprobe"], row.get("fallback", 0.0))
+        lx.set_mass(row.get("mass", 1.0), CENTROID_TEMPERATURE)
+        _, row["replayed_ms"] = timed(lambda: lx.query(test_q, mode=mode), reps=3)
+        row["replayed_ms"] = round(row["replayed_ms"], 3)
+    lx.set_mass(1.0, CENTROID_TEMPERATURE)
     res = {
         "database": int(db.shape[0]), "queries": int(test_q.shape[0]), "train_queries": int(train_q.shape[0]),
         "buckets": 1024, "centroid_scale": 0.125,
