@@ -89,6 +89,7 @@ struct orb_lindex {
     int cap_batch = 0;
     DBuf<uint16_t> X, A1, A2, dZ1, dZ2, dZ3;
     DBuf<float> Z1, Z2, Z3, loss, eloss;
+    DBuf<float> ws; /* split-K partial sums of the weight gradients */
     DBuf<int> labels, order;
     /* database */
     int ndb = 0;
@@ -229,6 +230,19 @@ int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbi
 }
 
 
+/* weight gradient out = A^T dZ (A: B x M, dZ: B x N, row-major), the batch
+   split over the split-K workspace */
+int wgrad(orb_lindex* L, const uint16_t* A, int M, const uint16_t* dZ, int N, int B, float* out, hipStream_t s)
+{
+    GemmArgs g = gemm(A, 1, M, dZ, N, 1, M, N, B);
+    g.out32 = out;
+    g.splits = gemm_tn_splits(M, N, B);
+    LALLOC(L, L->ws, (size_t)g.splits * M * N);
+    g.ws = L->ws.p;
+    LTRY(L, launch_gemm(g, s));
+    return ORB_OK;
+}
+
 /* backward into L->g after forward and labels; loss per row in L->loss */
 int backward(orb_lindex* L, int B, hipStream_t s)
 {
@@ -236,25 +250,22 @@ int backward(orb_lindex* L, int B, hipStream_t s)
     const uint16_t* w16 = L->w16.p;
     float* gr = L->g.p;
     LTRY(L, launch_softmax_ce(L->Z3.p, L->labels.p, B, C, L->dZ3.p, L->loss.p, s));
-    GemmArgs g = gemm(L->A2.p, 1, H, L->dZ3.p, C, 1, H, C, B); /* dW3 = A2^T dZ3 */
-    g.out32 = gr + L->off[4];
-    LTRY(L, launch_gemm(g, s));
+    int rc = wgrad(L, L->A2.p, H, L->dZ3.p, C, B, gr + L->off[4], s); /* dW3 = A2^T dZ3 */
+    if (rc != ORB_OK) return rc;
     LTRY(L, launch_colsum(L->dZ3.p, B, C, gr + L->off[5], s));
-    g = gemm(L->dZ3.p, C, 1, w16 + L->off[4], 1, C, B, H, C); /* dZ2 = (dZ3 W3^T) gelu'(Z2) */
+    GemmArgs g = gemm(L->dZ3.p, C, 1, w16 + L->off[4], 1, C, B, H, C); /* dZ2 = (dZ3 W3^T) gelu'(Z2) */
     g.zmul = L->Z2.p;
     g.out16 = L->dZ2.p;
     LTRY(L, launch_gemm(g, s));
-    g = gemm(L->A1.p, 1, H, L->dZ2.p, H, 1, H, H, B); /* dW2 = A1^T dZ2 */
-    g.out32 = gr + L->off[2];
-    LTRY(L, launch_gemm(g, s));
+    rc = wgrad(L, L->A1.p, H, L->dZ2.p, H, B, gr + L->off[2], s); /* dW2 = A1^T dZ2 */
+    if (rc != ORB_OK) return rc;
     LTRY(L, launch_colsum(L->dZ2.p, B, H, gr + L->off[3], s));
     g = gemm(L->dZ2.p, H, 1, w16 + L->off[2], 1, H, B, H, H); /* dZ1 = (dZ2 W2^T) gelu'(Z1) */
     g.zmul = L->Z1.p;
     g.out16 = L->dZ1.p;
     LTRY(L, launch_gemm(g, s));
-    g = gemm(L->X.p, 1, kIn, L->dZ1.p, H, 1, kIn, H, B); /* dW1 = X^T dZ1 */
-    g.out32 = gr + L->off[0];
-    LTRY(L, launch_gemm(g, s));
+    rc = wgrad(L, L->X.p, kIn, L->dZ1.p, H, B, gr + L->off[0], s); /* dW1 = X^T dZ1 */
+    if (rc != ORB_OK) return rc;
     LTRY(L, launch_colsum(L->dZ1.p, B, H, gr + L->off[1], s));
     return ORB_OK;
 }

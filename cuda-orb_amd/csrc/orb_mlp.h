@@ -35,7 +35,14 @@ struct GemmArgs {
     const uint8_t* qbits;
     const uint8_t* cbits;
     float cscale;
+    /* products of operands contiguous along M and N (A^T B: the weight
+       gradients) with only out32: the K range split into up to `splits`
+       parts, ws holding splits x M x N floats of partial sums */
+    float* ws;
+    int splits;
 };
+/* split count and workspace floats of such a product of M x N outputs over K */
+int gemm_tn_splits(int M, int N, int K);
 
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 /* dst (C x R) = src (R x C) transposed (bf16 bits); R and C multiples of 64 */

@@ -160,6 +160,128 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g)
     }
 }
 
+/* C = A^T B for operands contiguous along M and N (A(m, k) = A[k*sak + m],
+   B(k, n) = B[k*sbk + n]): the weight-gradient products, K being the batch.
+   128 x 128 tiles, four wavefronts of 64 x 64 (4 x 4 MFMA blocks); the K
+   range is split into parts of `kper` (one workgroup per tile and part, so a
+   product of a few tiles still fills the chip), each part's sums written to
+   ws[part][m][n] and added up in part order by k_splitk_sum (the result does
+   not depend on scheduling).  The MFMA wants 8 consecutive k per lane and
+   the operands are contiguous along m: a thread loads 8 rows of k x 2
+   columns (8 dwords, 256 contiguous bytes per row across a wavefront) and
+   writes each column's 8 k values as one 16-byte LDS row segment; the next
+   step's dwords are loaded during the current step's MFMAs.  Columns past M
+   or N read zeros (buffer loads) and are not stored. */
+constexpr int kTT = 128, kTK = 32, kTLd = kTK + 8; /* LDS row: 80 B */
+
+__global__ __launch_bounds__(256, 2) void k_gemm_tn(GemmArgs g, int kper)
+{
+    __shared__ __align__(16) uint16_t sA[2][kTT * kTLd];
+    __shared__ __align__(16) uint16_t sB[2][kTT * kTLd];
+    const int tiles_n = (g.N + kTT - 1) / kTT, nt = ((g.M + kTT - 1) / kTT) * tiles_n;
+    const int part = blockIdx.x / nt, t = blockIdx.x % nt;
+    const int m0 = (t / tiles_n) * kTT, n0 = (t % tiles_n) * kTT;
+    const int kb = part * kper, ke = min(g.K, kb + kper);
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = (wid >> 1) * 64, wn = (wid & 1) * 64;
+    const int row = lane & 15, grp = lane >> 4;
+    /* loader: column pair cp (columns 2 cp, 2 cp + 1), k rows 8 ko .. 8 ko + 7 of a step */
+    const int cp = tid & 63, ko = tid >> 6;
+    const __amdgpu_buffer_rsrc_t rA =
+        __builtin_amdgcn_make_buffer_rsrc((void*)g.A, 0, (int)min((long long)g.K * g.sak * 2, 0x7fffffffll), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rB =
+        __builtin_amdgcn_make_buffer_rsrc((void*)g.B, 0, (int)min((long long)g.K * g.sbk * 2, 0x7fffffffll), 0x00020000);
+    const uint32_t oob = 0x80000000u; /* columns past M or N */
+    const uint32_t ca = m0 + 2 * cp < g.M ? (uint32_t)((m0 + 2 * cp) * 2) : oob;
+    const uint32_t cb = n0 + 2 * cp < g.N ? (uint32_t)((n0 + 2 * cp) * 2) : oob;
+    uint32_t xa[8], xb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + 8 * ko + j;
+            const bool in = k < ke;
+            xa[j] = __builtin_amdgcn_raw_buffer_load_b32(rA, in ? ca : oob, (uint32_t)(k * g.sak * 2), 0);
+            xb[j] = __builtin_amdgcn_raw_buffer_load_b32(rB, in ? cb : oob, (uint32_t)(k * g.sbk * 2), 0);
+        }
+    };
+    auto lstore = [&](int buf) {
+        /* column 2 cp: the low halves of the 8 dwords, column 2 cp + 1: the high halves */
+        const uint4 alo = make_uint4(__builtin_amdgcn_perm(xa[1], xa[0], 0x05040100u), __builtin_amdgcn_perm(xa[3], xa[2], 0x05040100u),
+                                     __builtin_amdgcn_perm(xa[5], xa[4], 0x05040100u), __builtin_amdgcn_perm(xa[7], xa[6], 0x05040100u));
+        const uint4 ahi = make_uint4(__builtin_amdgcn_perm(xa[1], xa[0], 0x07060302u), __builtin_amdgcn_perm(xa[3], xa[2], 0x07060302u),
+                                     __builtin_amdgcn_perm(xa[5], xa[4], 0x07060302u), __builtin_amdgcn_perm(xa[7], xa[6], 0x07060302u));
+        const uint4 blo = make_uint4(__builtin_amdgcn_perm(xb[1], xb[0], 0x05040100u), __builtin_amdgcn_perm(xb[3], xb[2], 0x05040100u),
+                                     __builtin_amdgcn_perm(xb[5], xb[4], 0x05040100u), __builtin_amdgcn_perm(xb[7], xb[6], 0x05040100u));
+        const uint4 bhi = make_uint4(__builtin_amdgcn_perm(xb[1], xb[0], 0x07060302u), __builtin_amdgcn_perm(xb[3], xb[2], 0x07060302u),
+                                     __builtin_amdgcn_perm(xb[5], xb[4], 0x07060302u), __builtin_amdgcn_perm(xb[7], xb[6], 0x07060302u));
+        *(uint4*)&sA[buf][(2 * cp) * kTLd + 8 * ko] = alo;
+        *(uint4*)&sA[buf][(2 * cp + 1) * kTLd + 8 * ko] = ahi;
+        *(uint4*)&sB[buf][(2 * cp) * kTLd + 8 * ko] = blo;
+        *(uint4*)&sB[buf][(2 * cp + 1) * kTLd + 8 * ko] = bhi;
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int nk = (ke - kb + kTK - 1) / kTK;
+    if (nk > 0) {
+        gload(kb);
+        lstore(0);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload(kb + (kt + 1) * kTK);
+        __builtin_amdgcn_sched_barrier(0);
+        bf16x8 av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) av[a] = *(const bf16x8*)&sA[cur][(wm + 16 * a + row) * kTLd + 8 * grp];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[b] = *(const bf16x8*)&sB[cur][(wn + 16 * b + row) * kTLd + 8 * grp];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (kt + 1 < nk) lstore(cur ^ 1);
+        __syncthreads();
+    }
+    /* lane holds C[4 grp + r][row] of each 16 x 16 block */
+    float* const ws = g.ws + (long long)part * g.M * g.N;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int n = n0 + wn + 16 * b + row;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + 16 * a + 4 * grp + r;
+                if (m < g.M && n < g.N) ws[(long long)m * g.N + n] = acc[a][b][r];
+            }
+        }
+}
+
+/* out[m*ldc + n] = ws[0][m][n] + ws[1][m][n] + ... (parts in order), four
+   outputs per thread */
+__global__ __launch_bounds__(256) void k_splitk_sum(const float* __restrict__ ws, int parts, int M, int N, long long ldc,
+                                                    float* __restrict__ out)
+{
+    const long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i >= (long long)M * N) return;
+    float4 v = *(const float4*)(ws + i);
+    for (int p = 1; p < parts; p++) {
+        const float4 w = *(const float4*)(ws + (long long)p * M * N + i);
+        v.x += w.x;
+        v.y += w.y;
+        v.z += w.z;
+        v.w += w.w;
+    }
+    const long long m = i / N, n = i % N;
+    *(float4*)(out + m * ldc + n) = v;
+}
+
 /* Hamming distance of two 32-byte descriptors */
 __device__ __forceinline__ int ham256(const uint4& a0, const uint4& a1, const uint4& b0, const uint4& b1)
 {
@@ -843,6 +965,15 @@ static void probe_launch(const float* Z, int n, int C, int k, float mass, int* p
 
 /* ---------------------------------------------------------------- launchers */
 
+int gemm_tn_splits(int M, int N, int K)
+{
+    /* enough parts for about 256 workgroups, each part at least 128 deep */
+    const int tiles = ((M + kTT - 1) / kTT) * ((N + kTT - 1) / kTT);
+    int sp = (256 + tiles - 1) / tiles;
+    sp = min(sp, max(1, K / 128));
+    return max(sp, 1);
+}
+
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.M % kGT || g.N % kGT || g.K % kNK) return hipErrorInvalidValue;
@@ -861,6 +992,16 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
             hipLaunchKernelGGL(k_gemm_fwd, dim3(tiles_big), dim3(256), 0, s, g);
         } else if (deep) hipLaunchKernelGGL(k_gemm_nt128, dim3(tiles), dim3(256), 0, s, g);
         else hipLaunchKernelGGL(k_gemm_nt, dim3(tiles), dim3(256), 0, s, g);
+    } else if (g.ws && g.sam == 1 && g.sbn == 1 && g.splits >= 1 && g.out32 && !g.out16 && !g.addin && !g.bias &&
+               !g.zmul && !g.gelu && g.ldc % 4 == 0 && g.N % 4 == 0 && (long long)g.K * g.sak * 2 < (1ll << 30) &&
+               (long long)g.K * g.sbk * 2 < (1ll << 30)) {
+        const int tiles = ((g.M + kTT - 1) / kTT) * ((g.N + kTT - 1) / kTT);
+        const int kper = ((g.K + g.splits - 1) / g.splits + kTK - 1) / kTK * kTK;
+        const int parts = (g.K + kper - 1) / kper;
+        hipLaunchKernelGGL(k_gemm_tn, dim3(tiles * parts), dim3(256), 0, s, g, kper);
+        const long long n4 = (long long)g.M * g.N / 4;
+        hipLaunchKernelGGL(k_splitk_sum, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, s, (const float*)g.ws, parts,
+                           g.M, g.N, g.ldc, g.out32);
     } else {
         hipLaunchKernelGGL(k_gemm, dim3((g.M / kGT) * (g.N / kGT)), dim3(256), 0, s, g);
     }
