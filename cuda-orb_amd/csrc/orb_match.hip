@@ -37,6 +37,7 @@ namespace orbk
 constexpr int kGranule = 64;          /* database entries per chunk granule (popcount: 16 per wavefront) */
 constexpr int kMinChunkGranules = 2;  /* at least 128 entries per chunk: bounds the merge work */
 constexpr int kPopQ = 64;             /* popcount: queries per wavefront */
+constexpr int kPopTile = 256;         /* popcount: database entries per LDS tile (one per thread) */
 constexpr int kMfmaQ = 512;           /* MFMA: queries per workgroup */
 constexpr int kMfmaRow = 128 + 16;    /* LDS row of one unpacked entry: 8 chunks of 32 FP4 codes, padded */
 /* Split nq queries x ndb entries into qunits x chunks units for `slots`
@@ -115,6 +116,7 @@ __device__ __forceinline__ void match_pop_units(const uint8_t* __restrict__ q, i
                                                 orb::OrbPoint* __restrict__ pts)
 {
     __shared__ uint2 s_k[3][64];
+    __shared__ uint4 s_db[2][2 * kPopTile];
     const Plan P = make_plan(nq, ndb, kPopQ, slots);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (P.chunks == 0) {
@@ -132,8 +134,7 @@ __device__ __forceinline__ void match_pop_units(const uint8_t* __restrict__ q, i
             qa = qp[0];
             qb = qp[1];
         }
-        const int sub = P.chunk / 4;
-        const int j0 = min(ndb, c * P.chunk + wid * sub), j1 = min(ndb, j0 + sub);
+        const int j0 = min(ndb, c * P.chunk), j1 = min(ndb, j0 + P.chunk);
         uint32_t k1 = kNoKey, k2 = kNoKey;
         auto pair = [&](uint4 a, uint4 b, int j) {
             uint32_t d = word_dist<H2>(qa.x ^ a.x);
@@ -146,20 +147,32 @@ __device__ __forceinline__ void match_pop_units(const uint8_t* __restrict__ q, i
             d = word_dist<H2>(qb.w ^ b.w) + d;
             top2_push(k1, k2, (d << kIdxBits) | (uint32_t)j);
         };
-        int j = j0;
-        /* the entry index is wave-uniform: readfirstlane lets the compiler
-           fetch four entries with scalar loads before using them */
-        for (; j + 4 <= j1; j += 4) {
-            const int ju = __builtin_amdgcn_readfirstlane(j);
-            uint4 e[8];
-#pragma unroll
-            for (int t = 0; t < 8; t++) e[t] = db[2 * ju + t];
-#pragma unroll
-            for (int t = 0; t < 4; t++) pair(e[2 * t], e[2 * t + 1], ju + t);
+        /* the chunk through LDS in tiles of 256 entries (one entry per thread,
+           the next tile's loaded during the current one); wavefront wid takes
+           the tile's entries wid, wid + 4, ..., every lane reading the same
+           entry (a broadcast) */
+        const int ntiles = (j1 - j0 + kPopTile - 1) / kPopTile;
+        uint4 ea = make_uint4(0, 0, 0, 0), eb = ea;
+        if (j0 + (int)threadIdx.x < j1) {
+            ea = db[2 * (j0 + threadIdx.x)];
+            eb = db[2 * (j0 + threadIdx.x) + 1];
         }
-        for (; j < j1; j++) {
-            const int ju = __builtin_amdgcn_readfirstlane(j);
-            pair(db[2 * ju], db[2 * ju + 1], ju);
+        for (int t = 0; t < ntiles; t++) {
+            uint4* const tile = s_db[t & 1];
+            tile[2 * threadIdx.x] = ea;
+            tile[2 * threadIdx.x + 1] = eb;
+            __syncthreads();
+            const int jt = j0 + t * kPopTile, n = min(kPopTile, j1 - jt);
+            if (t + 1 < ntiles && jt + kPopTile + (int)threadIdx.x < j1) {
+                ea = db[2 * (jt + kPopTile + threadIdx.x)];
+                eb = db[2 * (jt + kPopTile + threadIdx.x) + 1];
+            }
+            int e = wid;
+            for (; e + 12 < n; e += 16) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) pair(tile[2 * (e + 4 * u)], tile[2 * (e + 4 * u) + 1], jt + e + 4 * u);
+            }
+            for (; e < n; e += 4) pair(tile[2 * e], tile[2 * e + 1], jt + e);
         }
         if (wid > 0) s_k[wid - 1][lane] = make_uint2(k1, k2);
         __syncthreads();
