@@ -282,6 +282,10 @@ def learned_match(ctx, torch, dev, log=None):
     d_ex = exact[1].cpu().numpy()
     rec = lambda o: float((o[1].cpu().numpy() == d_ex).mean())
     rec_idx = lambda o: float((o[0].cpu().numpy() == exact[0].cpu().numpy()).mean())
+    # the network's forward pass alone (descriptor expansion + three BF16 products with the centroid
+    # term, logits written in FP32): 2 (256*512 + 512*512 + 512*1024) FLOP per query on the matrix cores
+    _, ms_fwd = timed(lambda: lx.logits(test_q))
+    fwd_flop = 2.0 * (256 * 512 + 512 * 512 + 512 * 1024) * test_q.shape[0]
     # untrained, the network (last layer zero + centroid term) ranks buckets as the centroids do
     lx.set_probe(8, 0.0)
     (u0, _), _ = timed(lambda: lx.query(test_q, mode=orb_amd.ORB_LINDEX_MLP), reps=1)
@@ -360,6 +364,9 @@ def learned_match(ctx, torch, dev, log=None):
         "buckets": 1024, "centroid_scale": 0.125,
         "bucket_size": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
         "exact_mfma_ms": round(ms_exact, 3),
+        "forward": {"ms": round(ms_fwd, 4), "gflop": round(fwd_flop / 1e9, 1),
+                    "tflops": round(fwd_flop / (ms_fwd * 1e-3) / 1e12, 1),
+                    "frac_of_bf16_peak": round(fwd_flop / (ms_fwd * 1e-3) / 2.5e15, 4)},
         "top8_fallback_0.3": {"ms": round(ms_learned, 3), "recall_at_1": round(rec(lq), 5),
                               "recall_idx": round(rec_idx(lq), 5), "fallback_queries": int(nfb),
                               "speedup_vs_exact": round(ms_exact / ms_learned, 3)},

@@ -189,8 +189,8 @@ GemmArgs gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B
 /* Network forward of B rows of X (the +/-1 expansion of the descriptors
    qbits) into Z3 (logits); keep_z also stores the pre-activations Z1, Z2 the
    backward pass needs.  The centroid term is added to the last layer's
-   products in their epilogue. */
-int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbits)
+   products in their epilogue.  z3: where the logits go (L->Z3 if null). */
+int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbits, float* z3 = nullptr)
 {
     const int H = L->H, C = L->C;
     const float* w = L->w.p;
@@ -224,7 +224,7 @@ int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbi
         g.cscale = bf16_value(L->cscale16);
     }
     g.bias = w + L->off[5];
-    g.out32 = L->Z3.p;
+    g.out32 = z3 ? z3 : L->Z3.p;
     LTRY(L, launch_gemm(g, s));
     return ORB_OK;
 }
@@ -564,10 +564,13 @@ int orb_lindex_logits(orb_lindex* L, const uint8_t* d_q, int n, float* d_logits)
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int B = n - c0 < chunk ? n - c0 : chunk;
         LTRY(L, launch_bits_pm1(d_q + (size_t)c0 * 32, B, L->X.p, s));
-        rc = forward(L, B, s, false, d_q + (size_t)c0 * 32);
+        /* the product stores 16-byte row pieces: an unaligned buffer gets a copy */
+        float* const dst = d_logits + (size_t)c0 * L->C;
+        const bool direct = ((uintptr_t)dst & 15) == 0;
+        rc = forward(L, B, s, false, d_q + (size_t)c0 * 32, direct ? dst : nullptr);
         if (rc != ORB_OK) return rc;
-        LTRY(L, hipMemcpyAsync(d_logits + (size_t)c0 * L->C, L->Z3.p, sizeof(float) * B * L->C,
-                               hipMemcpyDeviceToDevice, s));
+        if (!direct)
+            LTRY(L, hipMemcpyAsync(dst, L->Z3.p, sizeof(float) * B * L->C, hipMemcpyDeviceToDevice, s));
     }
     return ORB_OK;
 }

@@ -99,6 +99,21 @@ def small_index(torch, ctx):
     lx.close()
 
 
+def test_logits_into_an_unaligned_buffer(torch, ctx, small_index):
+    """orb_lindex_logits writes into the caller's buffer directly when it is 16-byte aligned and
+    through a copy otherwise: both give the same values."""
+    import orb_amd
+    lx, db, ddb = small_index
+    q = torch.from_numpy(db[:128]).cuda()
+    want = lx.logits(q)
+    buf = torch.zeros(want.numel() + 4, dtype=torch.float32, device=q.device)
+    lx.ctx._bind_stream()
+    rc = orb_amd.lib().orb_lindex_logits(lx.h, orb_amd._dptr(q), 128, buf.data_ptr() + 4)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert torch.equal(buf[1:1 + want.numel()].view_as(want), want)
+
+
 def test_kmodes_partition_is_consistent(torch, ctx, small_index, oracle):
     lx, db, ddb = small_index
     off, cen, bof = lx.bucket_info()
