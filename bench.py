@@ -283,7 +283,8 @@ def learned_match(ctx, torch, dev, log=None):
     rec = lambda o: float((o[1].cpu().numpy() == d_ex).mean())
     rec_idx = lambda o: float((o[0].cpu().numpy() == exact[0].cpu().numpy()).mean())
     # the network's forward pass alone (descriptor expansion + three BF16 products with the centroid
-    # term, logits written in FP32): 2 (256*512 + 512*512 + 512*1024) FLOP per query on the matrix cores
+    # term, logits rounded to FP16 and written as FP32): 2 (256*512 + 512*512 + 512*1024) FLOP per query
+    # on the matrix cores
     _, ms_fwd = timed(lambda: lx.logits(test_q))
     fwd_flop = 2.0 * (256 * 512 + 512 * 512 + 512 * 1024) * test_q.shape[0]
     # untrained, the network (last layer zero + centroid term) ranks buckets as the centroids do

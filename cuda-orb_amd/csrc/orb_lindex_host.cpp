@@ -89,6 +89,7 @@ struct orb_lindex {
     int cap_batch = 0;
     DBuf<uint16_t> X, A1, A2, dZ1, dZ2, dZ3;
     DBuf<float> Z1, Z2, Z3, loss, eloss;
+    DBuf<uint16_t> Z3h; /* FP16 logits of the query path */
     DBuf<float> ws; /* split-K partial sums of the weight gradients */
     DBuf<int> labels, order;
     /* database */
@@ -186,36 +187,53 @@ GemmArgs gemm(const uint16_t* A, long long sam, long long sak, const uint16_t* B
     return g;
 }
 
-/* Network forward of B rows of X (the +/-1 expansion of the descriptors
-   qbits) into Z3 (logits); keep_z also stores the pre-activations Z1, Z2 the
-   backward pass needs.  The centroid term is added to the last layer's
-   products in their epilogue.  z3: where the logits go (L->Z3 if null). */
-int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbits, float* z3 = nullptr)
+/* the K-contiguous weight copies the forward products read, refreshed after
+   the weights change */
+int ensure_wt(orb_lindex* L, hipStream_t s)
 {
+    if (L->wt_fresh) return ORB_OK;
     const int H = L->H, C = L->C;
-    const float* w = L->w.p;
     const uint16_t* w16 = L->w16.p;
     uint16_t* wt = L->w16t.p;
     const size_t t1 = (size_t)kIn * H, t2 = t1 + (size_t)H * H;
-    if (!L->wt_fresh) {
-        LTRY(L, launch_transpose16(w16 + L->off[0], kIn, H, wt, s));
-        LTRY(L, launch_transpose16(w16 + L->off[2], H, H, wt + t1, s));
-        LTRY(L, launch_transpose16(w16 + L->off[4], H, C, wt + t2, s));
-        L->wt_fresh = true;
-    }
+    LTRY(L, launch_transpose16(w16 + L->off[0], kIn, H, wt, s));
+    LTRY(L, launch_transpose16(w16 + L->off[2], H, H, wt + t1, s));
+    LTRY(L, launch_transpose16(w16 + L->off[4], H, C, wt + t2, s));
+    L->wt_fresh = true;
+    return ORB_OK;
+}
+
+/* Network forward of B rows of X (the +/-1 expansion of the descriptors
+   qbits) into Z3 (logits); keep_z
+   also stores the pre-activations Z1, Z2 the backward pass needs.  The
+   centroid term is added to the last layer's products in their epilogue.
+   Without keep_z (queries) the logits are rounded to FP16: into z3h as FP16
+   bits if given, else as floats into z3 (L->Z3 if null);
+   with keep_z (training) they stay FP32. */
+int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbits, float* z3 = nullptr,
+            uint16_t* z3h = nullptr)
+{
+    const int H = L->H, C = L->C;
+    const float* w = L->w.p;
+    const uint16_t* wt = L->w16t.p;
+    const size_t t1 = (size_t)kIn * H, t2 = t1 + (size_t)H * H;
+    int rc = ensure_wt(L, s);
+    if (rc != ORB_OK) return rc;
+    uint16_t* const a1 = L->A1.p;
+    uint16_t* const a2 = L->A2.p;
     GemmArgs g = gemm(L->X.p, kIn, 1, wt, 1, kIn, B, H, kIn);
     g.bias = w + L->off[1];
     g.gelu = 1;
     g.out32 = keep_z ? L->Z1.p : nullptr;
-    g.out16 = L->A1.p;
+    g.out16 = a1;
     LTRY(L, launch_gemm(g, s));
-    g = gemm(L->A1.p, H, 1, wt + t1, 1, H, B, H, H);
+    g = gemm(a1, H, 1, wt + t1, 1, H, B, H, H);
     g.bias = w + L->off[3];
     g.gelu = 1;
     g.out32 = keep_z ? L->Z2.p : nullptr;
-    g.out16 = L->A2.p;
+    g.out16 = a2;
     LTRY(L, launch_gemm(g, s));
-    g = gemm(L->A2.p, H, 1, wt + t2, 1, H, B, C, H);
+    g = gemm(a2, H, 1, wt + t2, 1, H, B, C, H);
     if (L->cscale16) {
         /* centroid term centroid_scale * <x, c_b> = centroid_scale * (256 - 2 Hamming) */
         if (!L->centroids.p) return lfail(L, ORB_E_STATE, "orb_lindex_build has not been called");
@@ -224,7 +242,9 @@ int forward(orb_lindex* L, int B, hipStream_t s, bool keep_z, const uint8_t* qbi
         g.cscale = bf16_value(L->cscale16);
     }
     g.bias = w + L->off[5];
-    g.out32 = z3 ? z3 : L->Z3.p;
+    g.half = keep_z ? 0 : 1;
+    if (z3h) g.out16 = z3h;
+    else g.out32 = z3 ? z3 : L->Z3.p;
     LTRY(L, launch_gemm(g, s));
     return ORB_OK;
 }
@@ -277,17 +297,18 @@ int compute_probes(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* pro
 {
     const int np = L->prm.nprobe, nb = L->C;
     if (mode == ORB_LINDEX_MLP) {
-        /* network forward in chunks, top-nprobe buckets and max softmax */
+        /* network forward in chunks (FP16 logits), top-nprobe buckets and max softmax */
         const int chunk = 65536;
         int rc = ensure_batch(L, nq < chunk ? nq : chunk);
         if (rc != ORB_OK) return rc;
+        LALLOC(L, L->Z3h, (size_t)(nq < chunk ? nq : chunk) * nb);
         for (int c0 = 0; c0 < nq; c0 += chunk) {
             const int B = nq - c0 < chunk ? nq - c0 : chunk;
             const uint8_t* qb = d_q + (size_t)c0 * 32;
             LTRY(L, launch_bits_pm1(qb, B, L->X.p, s));
-            rc = forward(L, B, s, false, qb);
+            rc = forward(L, B, s, false, qb, nullptr, L->Z3h.p);
             if (rc != ORB_OK) return rc;
-            LTRY(L, launch_probe(L->Z3.p, B, nb, np, L->mass, probes + (size_t)c0 * np, pmax + c0, sorted, s));
+            LTRY(L, launch_probe(L->Z3h.p, B, nb, np, L->mass, probes + (size_t)c0 * np, pmax + c0, sorted, s));
         }
     } else {
         LTRY(L, launch_centroid_probe(d_q, nq, L->centroids.p, nb, np, L->mass, L->ctemp, probes, pmax, sorted, s));

@@ -16,7 +16,9 @@ namespace orbk
    Epilogue, in order: z += addin[m*ldc + n] if addin; z += cscale * (256 -
    2 Hamming(qbits row m, cbits row n)) if cbits (32-byte rows: the centroid
    term, exact in FP32); z += bias[n]; z *= gelu'(zmul[m*ldc + n]) if zmul;
-   out32[m*ldc + n] = z; out16[m*ldc + n] = bf16(gelu ? gelu(z) : z). */
+   out32[m*ldc + n] = z; out16[m*ldc + n] = bf16(gelu ? gelu(z) : z).  With
+   half (products of operands contiguous along K, no addin, zmul or gelu) z
+   is first rounded to FP16, and out16 takes the FP16 bits instead. */
 struct GemmArgs {
     const uint16_t* A;
     long long sam, sak;
@@ -40,6 +42,7 @@ struct GemmArgs {
        parts, ws holding splits x M x N floats of partial sums */
     float* ws;
     int splits;
+    int half;
 };
 /* split count and workspace floats of such a product of M x N outputs over K */
 int gemm_tn_splits(int M, int N, int K);
@@ -56,10 +59,10 @@ hipError_t launch_colsum(const uint16_t* X, int n, int C, float* out, hipStream_
 hipError_t launch_adam(float* w, uint16_t* w16, const float* g, float* m, float* v, long long n, float lr, float b1,
                        float b2, float eps, int step, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t s);
-/* top-k buckets per row of logits (ties to the lower id): best first when
+/* top-k buckets per row of FP16 logits (ties to the lower id): best first when
    sorted, else in bucket-id order (the same set); with mass below 1 (k <= 64)
    the best-ranked prefix whose softmax mass reaches it, best first, -1 after */
-hipError_t launch_probe(const float* Z, int n, int C, int k, float mass, int* probes, float* pmax, bool sorted,
+hipError_t launch_probe(const uint16_t* Z, int n, int C, int k, float mass, int* probes, float* pmax, bool sorted,
                         hipStream_t s);
 /* bucketed database (orb_lindex.hip) */
 /* stable counting sort of n entries by bucket (assign): offsets[0..nb], perm

@@ -24,6 +24,7 @@
  *   k_probe     softmax max and the exact top-k buckets of each query
  */
 #include "orb_internal.h"
+#include "orb_match_common.h"
 #include "orb_mlp.h"
 #include "orb_wave.h"
 
@@ -46,6 +47,22 @@ __device__ __forceinline__ float bf2f(uint16_t h)
     return __uint_as_float((uint32_t)h << 16);
 }
 
+/* x rounded to FP16 (nearest even), and FP16 bits <-> float */
+__device__ __forceinline__ float round_h(float x)
+{
+    return (float)(_Float16)x;
+}
+
+__device__ __forceinline__ uint32_t h_bits(float x)
+{
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)x);
+}
+
+__device__ __forceinline__ float h2f(uint32_t h)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)h);
+}
+
 /* erf(x) in the form 1 - exp(-t P(t)), t = min(|x|, 4), sign restored: P is a
    degree-7 polynomial fitted for this kernel by weighted least squares on
    [0, 4] against the float64 erf (weights t exp(-t P), the sensitivity of erf
@@ -66,14 +83,48 @@ __device__ __forceinline__ float erf_fit(float x)
     return copysignf(1.0f - e, x);
 }
 
+/* gelu(z) = z (1 + erf(z / sqrt 2)) / 2 for the BF16 activations: with
+   u = min(|z|, 4 sqrt 2) and h = erfc(u / sqrt 2) / 2 = 2^(-u Q(u) - 1),
+   gelu = z - z h (z >= 0) or z h (z < 0).  Q is a degree-5 polynomial fitted
+   by weighted least squares on (0, 4 sqrt 2] to -log2(erfc(u / sqrt 2)) / u
+   (weights u erfc(u / sqrt 2), the sensitivity of erfc to Q; Chebyshev
+   nodes).  Evaluated in float32 the error is at most 7.3e-7 absolute and
+   0.26 % relative where |gelu| > 1e-4, under half a BF16 step. */
 __device__ __forceinline__ float gelu(float z)
 {
-    return z * 0.5f * (1.0f + erf_fit(z * 0.70710678118654752f));
+    const float u = fminf(fabsf(z), 5.65685425f);
+    float q = -0x1.4d320ep-16f;
+    q = fmaf(q, u, 0x1.5b3d9ep-11f);
+    q = fmaf(q, u, -0x1.fc8a3ap-8f);
+    q = fmaf(q, u, 0x1.b1e27cp-5f);
+    q = fmaf(q, u, 0x1.d61574p-2f);
+    q = fmaf(q, u, 0x1.26afbep+0f);
+    const float h = __builtin_amdgcn_exp2f(fmaf(-u, q, -1.0f));
+    const float zh = z * h;
+    return z >= 0.f ? z - zh : zh;
 }
 
 __device__ __forceinline__ float gelu_grad(float z)
 {
     return 0.5f * (1.0f + erf_fit(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+
+/* 8 staged values -> 8 output halves: FP16 bits (values already rounded)
+   with half, else BF16 of gelu(v) or v */
+__device__ __forceinline__ uint4 pack_out16(const GemmArgs& g, const float (&v)[8])
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if (g.half) {
+            w[t] = h_bits(v[2 * t]) | (h_bits(v[2 * t + 1]) << 16);
+        } else {
+            const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
+            const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
+            w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
+        }
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 /* ------------------------------------------------------------------ GEMM */
@@ -336,7 +387,10 @@ __device__ __forceinline__ void nt_epilogue(const GemmArgs& g, f32x4 (&acc)[4][4
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) st[(wm + 16 * a + 4 * grp + r) * kOLd + nl] = acc[a][b][r] + bias;
+                for (int r = 0; r < 4; r++) {
+                    const float z = acc[a][b][r] + bias;
+                    st[(wm + 16 * a + 4 * grp + r) * kOLd + nl] = g.half ? round_h(z) : z;
+                }
         }
         __syncthreads();
         if (g.out32) {
@@ -356,15 +410,8 @@ __device__ __forceinline__ void nt_epilogue(const GemmArgs& g, f32x4 (&acc)[4][4
                 if (m < g.M && n < g.N) {
                     const float4 lo = *(const float4*)(st + rl * kOLd + cl);
                     const float4 hi = *(const float4*)(st + rl * kOLd + cl + 4);
-                    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                    uint32_t w[4];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
-                        const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
-                        w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
-                    }
-                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = make_uint4(w[0], w[1], w[2], w[3]);
+                    const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = pack_out16(g, v);
                 }
             }
         }
@@ -665,23 +712,33 @@ __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
        `row`, n = 4 grp); per 16-row block of the wavefront's tile the values go
        through this wavefront's 16 x 64 stage (the K loop ended on a barrier) */
     if (g.cbits) {
-        /* acc += the centroid term: per block column this lane's 4 columns
-           against each of its 8 rows */
+        /* acc += the centroid term on the matrix cores: descriptor bits as FP4
+           codes +1 (set) / -1 (clear), so a 256-deep FP4 contraction of a
+           centroid row with a query row is 256 - 2 d exactly (the same value
+           the popcount form gives; the same FMA then adds it).  Codes of every
+           byte value from a table in the third stage's LDS (free after the
+           loop); lane (grp, row) takes words grp and 4 + grp of its centroid
+           (A operand, the output column) and of its query (B, the row). */
+        uint32_t* const tab = (uint32_t*)(s_st + 2 * kFStage);
+        tab[tid] = fp4_pm1_8((uint32_t)tid);
+        __syncthreads();
+        v4i cc[4][2];
 #pragma unroll
         for (int b = 0; b < 4; b++) {
-            uint4 c[4][2];
+            const uint32_t* cp = (const uint32_t*)(g.cbits + (long long)min(n0 + wn + 16 * b + row, g.N - 1) * 32);
+            cc[b][0] = fp4_word_lut(cp[grp], tab);
+            cc[b][1] = fp4_word_lut(cp[4 + grp], tab);
+        }
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint4* cp = (const uint4*)(g.cbits + (long long)min(n0 + wn + 16 * b + 4 * grp + i, g.N - 1) * 32);
-                c[i][0] = cp[0];
-                c[i][1] = cp[1];
-            }
+        for (int a = 0; a < 8; a++) {
+            const uint32_t* qp = (const uint32_t*)(g.qbits + (long long)min(m0 + wm + 16 * a + row, g.M - 1) * 32);
+            const v4i q0 = fp4_word_lut(qp[grp], tab), q1 = fp4_word_lut(qp[4 + grp], tab);
 #pragma unroll
-            for (int a = 0; a < 8; a++) {
-                const uint4* qp = (const uint4*)(g.qbits + (long long)min(m0 + wm + 16 * a + row, g.M - 1) * 32);
-                const uint4 q0 = qp[0], q1 = qp[1];
+            for (int b = 0; b < 4; b++) {
+                v4f t = mfma_fp4(cc[b][0], q0, (v4f){0.f, 0.f, 0.f, 0.f});
+                t = mfma_fp4(cc[b][1], q1, t);
 #pragma unroll
-                for (int i = 0; i < 4; i++) acc[b][a][i] += centroid_term(g, ham256(q0, q1, c[i][0], c[i][1]));
+                for (int i = 0; i < 4; i++) acc[b][a][i] += g.cscale * t[i];
             }
         }
     }
@@ -695,10 +752,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
 #pragma unroll
     for (int a = 0; a < 8; a++) {
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            *(float4*)(st + row * 68 + 16 * b + 4 * grp) =
-                make_float4(acc[b][a][0] + bias[b].x, acc[b][a][1] + bias[b].y, acc[b][a][2] + bias[b].z,
-                            acc[b][a][3] + bias[b].w);
+        for (int b = 0; b < 4; b++) {
+            float4 z = make_float4(acc[b][a][0] + bias[b].x, acc[b][a][1] + bias[b].y, acc[b][a][2] + bias[b].z,
+                                   acc[b][a][3] + bias[b].w);
+            if (g.half) z = make_float4(round_h(z.x), round_h(z.y), round_h(z.z), round_h(z.w));
+            *(float4*)(st + row * 68 + 16 * b + 4 * grp) = z;
+        }
         wave_sync_lds();
         if (g.out32) {
             /* 16 lanes per row, 4 rows per pass */
@@ -719,15 +778,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
                 if (m < g.M && n < g.N) {
                     const float4 lo = *(const float4*)(st + rl * 68 + cl);
                     const float4 hi = *(const float4*)(st + rl * 68 + cl + 4);
-                    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                    uint32_t w[4];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        const float e0 = g.gelu ? gelu(v[2 * t]) : v[2 * t];
-                        const float e1 = g.gelu ? gelu(v[2 * t + 1]) : v[2 * t + 1];
-                        w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
-                    }
-                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = make_uint4(w[0], w[1], w[2], w[3]);
+                    const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = pack_out16(g, v);
                 }
             }
         }
@@ -868,8 +920,8 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
 constexpr int kProbeCap = 256;
 
 /* Max softmax probability (*pmax_r) and the top-k logits (out[0 .. k)) of
-   the logit row z, one wavefront (ties to the lower bucket id; C <= 1024, a
-   multiple of 64), by descending logit with SORTED, else by bucket id; cand
+   the FP16 logit row z, one wavefront (ties to the lower bucket id; C <= 1024,
+   a multiple of 64), by descending logit with SORTED, else by bucket id; cand
    holds CAP candidates.  False: more than CAP candidates, nothing written to
    the probes.  With mass below 1
    (k <= 64) only the best-ranked prefix whose softmax mass reaches `mass` is
@@ -879,15 +931,31 @@ constexpr int kProbeCap = 256;
    times the row's sum of exp(z - max); the row then holds the kept buckets
    best first and -1 in the remaining slots, whatever SORTED says.  ids: 64
    ints of LDS for this wavefront. */
+/* order-preserving 16-bit key of FP16 bits (-0 and +0 share one key, above
+   0: every value has a key of at least 1) */
+__device__ __forceinline__ uint32_t half_order_key(uint32_t h)
+{
+    return (h & 0x7fffu) == 0u ? 0x8000u : (h & 0x8000u) ? (~h & 0xffffu) : (h | 0x8000u);
+}
+
 template <bool SORTED, int CAP>
-__device__ __forceinline__ bool probe_row(const float* __restrict__ z, int C, int k, float mass, int* __restrict__ out,
-                                          float* __restrict__ pmax_r, uint2* cand, int* ids)
+__device__ __forceinline__ bool probe_row(const uint16_t* __restrict__ z, int C, int k, float mass,
+                                          int* __restrict__ out, float* __restrict__ pmax_r, uint2* cand, int* ids)
 {
     const int lane = threadIdx.x & 63;
-    /* C is a multiple of 64: slot t is present in every lane or in none */
+    /* one dword per lane and slot pair t: buckets 128 t + 2 lane and + 1 (C
+       is even, so a pair is present whole or not at all) */
+    const uint32_t* const z2 = (const uint32_t*)z;
+    uint32_t hb[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) hb[t] = 128 * t + 2 * lane < C ? z2[lane + 64 * t] : 0u;
     float v[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) v[t] = 64 * t < C ? z[lane + 64 * t] : -INFINITY;
+    for (int t = 0; t < 8; t++) {
+        const bool present = 128 * t + 2 * lane < C;
+        v[2 * t] = present ? h2f(hb[t] & 0xffffu) : -INFINITY;
+        v[2 * t + 1] = present ? h2f(hb[t] >> 16) : -INFINITY;
+    }
     float m = -INFINITY;
 #pragma unroll
     for (int t = 0; t < 16; t++) m = fmaxf(m, v[t]);
@@ -895,19 +963,23 @@ __device__ __forceinline__ bool probe_row(const float* __restrict__ z, int C, in
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     float s = 0.f;
 #pragma unroll
-    for (int t = 0; t < 16; t++) s += 64 * t < C ? __expf(v[t] - m) : 0.f;
+    for (int t = 0; t < 16; t++) s += 128 * (t >> 1) + 2 * lane < C ? __expf(v[t] - m) : 0.f;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) *pmax_r = 1.f / s;
     uint32_t key[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) key[t] = 64 * t < C ? float_order_key(v[t]) : 0u;
-    if (!(mass < 1.f)) return wave_topk<31, SORTED, CAP>(key, k, cand, out);
+    for (int t = 0; t < 8; t++) {
+        const bool present = 128 * t + 2 * lane < C;
+        key[2 * t] = present ? half_order_key(hb[t] & 0xffffu) : 0u;
+        key[2 * t + 1] = present ? half_order_key(hb[t] >> 16) : 0u;
+    }
+    if (!(mass < 1.f)) return wave_topk<15, SORTED, CAP, true>(key, k, cand, out);
     /* the top k best first into LDS, then the rule over lane = rank */
-    if (!wave_topk<31, true, CAP>(key, k, cand, ids)) return false;
+    if (!wave_topk<15, true, CAP, true>(key, k, cand, ids)) return false;
     wave_sync_lds();
     const int id = lane < k ? ids[lane] : 0;
-    wave_mass_keep(id, __expf(z[id] - m), k, mass * s, out);
+    wave_mass_keep(id, __expf(h2f(z[id]) - m), k, mass * s, out);
     wave_sync_lds(); /* ids are reused by the wavefront's next query */
     return true;
 }
@@ -916,7 +988,7 @@ __device__ __forceinline__ bool probe_row(const float* __restrict__ z, int C, in
    wavefront, which leaves room for more resident wavefronts) a query whose
    candidates do not fit gets -1 as its first probe, for k_probe_fix. */
 template <bool SORTED, int CAP>
-__global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int n, int C, int k, float mass,
+__global__ __launch_bounds__(256) void k_probe(const uint16_t* __restrict__ Z, int n, int C, int k, float mass,
                                                int* __restrict__ probes, float* __restrict__ pmax)
 {
     __shared__ uint2 s_cand[4][CAP];
@@ -933,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_probe(const float* __restrict__ Z, int 
 /* the queries k_probe marked, redone with room for every candidate: each
    wavefront reads the marks of 64 queries and takes the marked ones in turn */
 template <bool SORTED>
-__global__ __launch_bounds__(256) void k_probe_fix(const float* __restrict__ Z, int n, int C, int k, float mass,
+__global__ __launch_bounds__(256) void k_probe_fix(const uint16_t* __restrict__ Z, int n, int C, int k, float mass,
                                                    int* __restrict__ probes, float* __restrict__ pmax)
 {
     __shared__ uint2 s_cand[4][1024];
@@ -951,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_probe_fix(const float* __restrict__ Z, 
 
 
 template <bool SORTED>
-static void probe_launch(const float* Z, int n, int C, int k, float mass, int* probes, float* pmax, hipStream_t s)
+static void probe_launch(const uint16_t* Z, int n, int C, int k, float mass, int* probes, float* pmax, hipStream_t s)
 {
     const dim3 grid((n + 3) / 4);
     if (k <= 64 && C > kProbeCap) {
@@ -979,6 +1051,7 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
     if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.M % kGT || g.N % kGT || g.K % kNK) return hipErrorInvalidValue;
     const bool nt = g.sak == 1 && g.sbk == 1 && g.sam % 8 == 0 && g.sbn % 8 == 0;
     if (g.cbits && (!g.qbits || g.addin || g.zmul || g.sak != 1 || g.sbk != 1)) return hipErrorInvalidValue;
+    if (g.half && (g.addin || g.zmul || g.gelu || !nt)) return hipErrorInvalidValue;
     if (nt) {
         const int tiles = ((g.M + kNT - 1) / kNT) * ((g.N + kNT - 1) / kNT);
         /* 128-deep steps when both K segments allow them and the operands fit 32-bit buffer offsets */
@@ -1056,7 +1129,7 @@ hipError_t launch_to_bf16(const float* x, uint16_t* y, long long n, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_probe(const float* Z, int n, int C, int k, float mass, int* probes, float* pmax, bool sorted,
+hipError_t launch_probe(const uint16_t* Z, int n, int C, int k, float mass, int* probes, float* pmax, bool sorted,
                         hipStream_t s)
 {
     if (C <= 0 || C > 1024 || C % 64 || k < 1 || k > C || (mass < 1.f && (k > 64 || !(mass > 0.f))))

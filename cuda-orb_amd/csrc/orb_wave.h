@@ -99,8 +99,9 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
     return T;
 }
 
-/* Top-k of up to 1024 entries, entry lane + 64 t holding key[t] (key 0 marks
-   an absent entry, every key below 2^(HIBIT+1); at least k entries present,
+/* Top-k of up to 1024 entries, entry lane + 64 t holding key[t] (with PAIRS
+   entry 128 (t / 2) + 2 lane + t % 2: a lane's slot pairs are adjacent
+   entries) (key 0 marks an absent entry, every key below 2^(HIBIT+1); at least k entries present,
    1 <= k <= 1024): writes the k entry numbers of the largest keys, equal keys
    taken by ascending entry number, to out[0 .. k) -- with SORTED by
    descending key (equal keys by ascending entry number), otherwise in entry
@@ -114,7 +115,7 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
       (fewer than k): SORTED ranks them against each other, otherwise they
       keep entry order; entries equal to T take the remaining places in entry
       order. */
-template <int HIBIT, bool SORTED, int CAP = 1024>
+template <int HIBIT, bool SORTED, int CAP = 1024, bool PAIRS = false>
 __device__ __forceinline__ bool wave_topk(const uint32_t (&key)[16], int k, uint2* cand, int* __restrict__ out)
 {
     static_assert(CAP % 64 == 0 && CAP >= 64 && CAP <= 1024, "whole slots of 64 candidates");
@@ -128,13 +129,27 @@ __device__ __forceinline__ bool wave_topk(const uint32_t (&key)[16], int k, uint
         L = wave_kth_key<1, HIBIT>(lm, k);
     }
     int nc = 0;
+    if (PAIRS) {
+        /* entry order: a lane's pair after the pairs of the lanes below it */
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const bool p = key[t] >= L;
-        const uint64_t m = __ballot(p);
-        const int pos = nc + lanes_below(m);
-        if (p && pos < CAP) cand[pos] = make_uint2(key[t], (uint32_t)(lane + 64 * t));
-        nc += __popcll(m);
+        for (int t = 0; t < 8; t++) {
+            const bool p0 = key[2 * t] >= L, p1 = key[2 * t + 1] >= L;
+            const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+            const int pos = nc + lanes_below(m0) + lanes_below(m1);
+            const uint32_t id = (uint32_t)(128 * t + 2 * lane);
+            if (p0 && pos < CAP) cand[pos] = make_uint2(key[2 * t], id);
+            if (p1 && pos + (int)p0 < CAP) cand[pos + (int)p0] = make_uint2(key[2 * t + 1], id + 1u);
+            nc += __popcll(m0) + __popcll(m1);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const bool p = key[t] >= L;
+            const uint64_t m = __ballot(p);
+            const int pos = nc + lanes_below(m);
+            if (p && pos < CAP) cand[pos] = make_uint2(key[t], (uint32_t)(lane + 64 * t));
+            nc += __popcll(m);
+        }
     }
     if (nc > CAP) return false;
     wave_sync_lds();

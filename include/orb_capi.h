@@ -234,7 +234,9 @@ int orb_lindex_bucket_info(orb_lindex* lx, int* h_offsets, uint8_t* h_centroids,
    receives the mean loss of each epoch's last batch.  Synchronous. */
 int orb_lindex_train(orb_lindex* lx, const uint8_t* d_q, const int* d_target, int n, int epochs, int batch,
                      float lr, float* h_epoch_loss);
-/* Bucket logits of n queries (multiple of 64) into d_logits (n x nbuckets floats). */
+/* Bucket logits of n queries (multiple of 64) into d_logits (n x nbuckets
+   floats), rounded to FP16 (nearest even): the values the query path ranks
+   its buckets by. */
 int orb_lindex_logits(orb_lindex* lx, const uint8_t* d_q, int n, float* d_logits);
 /* Gradients of the mean cross-entropy for n queries (multiple of 64) with
    labels d_labels, without updating: all parameters (W1 b1 W2 b2 W3 b3,
