@@ -275,6 +275,50 @@ def test_probe_mass_rule_matches_oracle(torch, ctx, nb, mass):
         lx.close()
 
 
+@pytest.mark.parametrize("nb", [64, 192, 320, 960])
+def test_probe_selection_bucket_counts_off_128(torch, ctx, nb):
+    """The query path ranks FP16 logits read one dword (two adjacent buckets) per lane: bucket
+    counts that are multiples of 64 but not of 128 leave the last pair slot to half the lanes.
+    The logits are FP16 values; the probed buckets (best first, ties to the lower id), the max
+    softmax and the probability-mass rule equal the oracle's on them, and the search covers
+    the probed buckets exactly."""
+    import orb_amd
+    from oracle import mlp
+    rng = np.random.default_rng(57 + nb)
+    db = rng.integers(0, 256, (6000, 32), dtype=np.uint8)
+    print(f"probe selection: {nb} buckets", flush=True)
+    ddb = torch.from_numpy(db).cuda()
+    lx = orb_amd.LearnedIndex(ctx, nbuckets=nb, hidden=128, nprobe=8, fallback=0.0, seed=12, centroid_scale=0.125)
+    try:
+        lx.build(ddb, iters=1)
+        q_tr, src = flip_copies(rng, db, 2048, p=0.1)
+        lx.train(torch.from_numpy(q_tr).cuda(), torch.from_numpy(src.astype(np.int32)).cuda(), epochs=1,
+                 batch=1024, lr=3e-3)
+        q, _ = flip_copies(rng, db, 1088, p=0.1)
+        dq = torch.from_numpy(q).cuda()
+        logits32 = lx.logits(dq).cpu().numpy()
+        assert np.array_equal(logits32, logits32.astype(np.float16).astype(np.float32))
+        logits = logits32.astype(np.float64)
+        _, _, bof = lx.bucket_info()
+        bof = bof.cpu().numpy()
+        for nprobe in sorted({1, 8, min(nb, 64), nb}):
+            lx.set_probe(nprobe, 0.0)
+            pr, pm = lx.probes(dq)
+            want, wpm = mlp.probe(logits, nprobe)
+            assert np.array_equal(pr.cpu().numpy(), want), (nb, nprobe)
+            np.testing.assert_allclose(pm.cpu().numpy(), wpm, rtol=1e-5)
+        lx.set_probe(min(nb, 32), 0.0)
+        lx.set_mass(0.9, 4.0)
+        got = lx.probes(dq)[0].cpu().numpy()
+        w = np.exp(logits - logits.max(1, keepdims=True))
+        assert_mass_probes(got, logits, w, min(nb, 32), 0.9, (nb, 0.9))
+        res, nfb = lx.query(dq)
+        assert nfb == 0
+        assert_top2(res, mlp.bucket_search(q, db, bof, got), (nb, "mass"))
+    finally:
+        lx.close()
+
+
 def test_probe_mass_rule_arguments(torch, ctx, small_index):
     """mass outside (0, 1], a non-positive temperature, and nprobe above 64 with the rule set
     are refused; the rule and nprobe stay as they were."""
