@@ -17,7 +17,7 @@
 namespace orbk
 {
 
-constexpr int kLxRow = kTileRow;  /* LDS row of one unpacked entry */
+constexpr int kLxRow = 128 + 16;  /* LDS row of one unpacked entry: 8 chunks of 32 FP4 codes, padded */
 constexpr int kLxTile = 64;      /* entries per LDS tile */
 constexpr int kLxQ = 256;        /* queries per unit */
 
@@ -228,8 +228,10 @@ __global__ __launch_bounds__(256) void k_probe_fill(const int* __restrict__ prob
 }
 
 /* Each unit: up to 256 list entries (query, slot) of one bucket x all of the
-   bucket's entries.  A wave holds 64 queries as +/-1 B operands; each 64-entry
-   tile of the bucket is unpacked into LDS once per workgroup.  Units are
+   bucket's entries.  A wave holds 64 queries as negated +/-1 FP4 B operands;
+   each 64-entry tile of the bucket is unpacked into LDS once per workgroup as
+   +/-1 FP4 codes (a 256-entry byte table in LDS), so a 16 x 16 block of
+   -dot values is two 128-deep FP4 MFMAs, exact in FP32 (|-dot| <= 256).  Units are
    numbered along order[] (buckets by decreasing size) and handed out through
    the counter *next, so the largest come first and the workgroups finish
    together. */
@@ -245,10 +247,14 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                                                           uint32_t* __restrict__ qbound, uint2* __restrict__ partial)
 {
     __shared__ __align__(16) int8_t s_db[kLxTile * kLxRow];
+    __shared__ uint32_t s_code[256];
     __shared__ int s_u;
     const int total_units = unit_offsets[nb];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int g = lane >> 4, col = lane & 15;
+    s_code[tid] = fp4_pm1_8((uint32_t)tid); /* +1 where a bit is set; read after the loop's first barrier */
+    /* the accumulators start at 256, so they hold 256 - dot >= 0 */
+    const v4f c256 = {256.f, 256.f, 256.f, 256.f};
     for (;;) {
         __syncthreads();
         if (tid == 0) s_u = atomicAdd(next, 1);
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
         const int lo = unit_pos[u]; /* position of unit u in the bucket order */
         const int b = order[lo];
         const int e0 = list_offsets[b] + (u - unit_offsets[lo]) * kLxQ, e1 = min(list_offsets[b + 1], e0 + kLxQ);
-        v4i bq[4][4];
+        v4i bq[4][2];
         /* A query's second distance found so far in any of its buckets
            (qbound) bounds its final second distance: entries farther than it
            cannot enter the final top-2 and are kept out by starting the
@@ -278,7 +284,9 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                 d1 = qp[1];
                 bd = __hip_atomic_load(qbound + qi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            query_operands_neg(d0, d1, g, bq[qb]);
+            const uint32_t w[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) bq[qb][ks] = fp4_word(w[4 * ks + g], fp4_neg_pm1_8);
             k1[qb] = k2[qb] = bd <= 256u ? (bd << kIdxBits) | kIdxMask : kNoKey;
         }
         const int j0 = offsets[b], j1 = offsets[b + 1];
@@ -294,25 +302,24 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
             constexpr bool PART = decltype(part)::value;
 #pragma unroll
             for (int ab = 0; ab < 4; ab++) {
-                v4i a[4];
+                v4i a[2];
 #pragma unroll
-                for (int ks = 0; ks < 4; ks++) a[ks] = *(const v4i*)(s_db + lds_chunk(ab * 16 + col, 4 * ks + g));
+                for (int ks = 0; ks < 2; ks++) a[ks] = *(const v4i*)(s_db + (ab * 16 + col) * kLxRow + 16 * (4 * ks + g));
                 const int jb = jt + ab * 16 + 4 * g;
 #pragma unroll
                 for (int qb = 0; qb < 4; qb++) {
-                    v4i acc = {0, 0, 0, 0};
-#pragma unroll
-                    for (int ks = 0; ks < 4; ks++)
-                        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[ks], bq[qb][ks], acc, 0, 0, 0);
-                    /* acc holds -dot: the key of the smallest with index 0
-                       bounds the 4 keys from below: above k2, none can enter
-                       the top-2 */
-                    const int m = min(min(acc[0], acc[1]), min(acc[2], acc[3]));
-                    if (pm1_neg_dot_key(m, 0u) <= k2[qb]) {
+                    v4f acc = mfma_fp4(a[0], bq[qb][0], c256);
+                    acc = mfma_fp4(a[1], bq[qb][1], acc);
+                    /* acc holds 256 - dot (float bits order like the values):
+                       the key of the smallest with index 0 bounds the 4 keys
+                       from below: above k2, none can enter the top-2 */
+                    const uint32_t mb = min(min(__float_as_uint(acc[0]), __float_as_uint(acc[1])),
+                                            min(__float_as_uint(acc[2]), __float_as_uint(acc[3])));
+                    if (pm1_neg_dot_key((int)__uint_as_float(mb) - 256, 0u) <= k2[qb]) {
 #pragma unroll
                         for (int r = 0; r < 4; r++)
                             if (!PART || jb + r < j1)
-                                top2_push(k1[qb], k2[qb], pm1_neg_dot_key(acc[r], (uint32_t)(jb + r)));
+                                top2_push(k1[qb], k2[qb], pm1_neg_dot_key((int)acc[r] - 256, (uint32_t)(jb + r)));
                     }
                 }
             }
@@ -320,12 +327,9 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
         for (int jt = j0; jt < j1; jt += kLxTile) {
             __syncthreads();
             {
-                const uint32_t lo2 = nv.x, hi2 = nv.y;
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t hw = k < 2 ? (lo2 >> (16 * k)) & 0xffffu : (hi2 >> (16 * (k - 2))) & 0xffffu;
-                    *(v4i*)(s_db + lds_chunk(e, 4 * qq + k)) = pm1_16(hw);
-                }
+                /* descriptor words 2 qq and 2 qq + 1 of entry e */
+                *(v4i*)(s_db + e * kLxRow + 16 * (2 * qq)) = fp4_word_lut(nv.x, s_code);
+                *(v4i*)(s_db + e * kLxRow + 16 * (2 * qq + 1)) = fp4_word_lut(nv.y, s_code);
                 const int jn = jt + kLxTile + e;
                 nv = make_uint2(0u, 0u);
                 if (jn < j1) nv = *(const uint2*)(db_sorted + (long long)jn * 32 + qq * 8);

@@ -6,22 +6,32 @@
  * of recall.
  *
  *   k_gemm      C = A B on v_mfma_f32_16x16x32_bf16, 64 x 64 workgroup tiles,
- *               A and B addressed through strides (so A^T B and A B^T need no
- *               transposed copies), epilogue fused: + bias, GELU, or the
- *               GELU-derivative product of the backward pass, FP32 and/or
- *               BF16 stores
+ *               A and B addressed through strides, epilogue fused: + bias,
+ *               GELU, or the GELU-derivative product of the backward pass,
+ *               FP32 and/or BF16 stores
+ *   k_gemm_fwd  the forward products of 1024 rows and more: 256 x 128 tiles
+ *               in an XCD-aware order, 32-deep K steps in three LDS stages
+ *               filled by buffer loads with the LDS bit, epilogue with the
+ *               centroid term on FP4 matrix cores, bias, GELU, BF16, FP32 or
+ *               FP16 (query logits) stores through an LDS stage
  *   k_gemm_nt128, k_gemm_nt
- *               the same product when both operands are contiguous along K
- *               (forward pass and the data-gradient products): 128 x 128
- *               tiles in an XCD-aware order, 128-deep K steps with buffer
- *               loads (K a multiple of 128) or 64-deep steps double-buffered
- *               in LDS, an optional second K segment, LDS-staged epilogue
- *   k_transpose16  bf16 transposes (K-contiguous weight copies for k_gemm_nt)
+ *               products with both operands contiguous along K below 1024
+ *               rows and the data gradients: 128 x 128 tiles, 128-deep K
+ *               steps with buffer loads (K a multiple of 128) or 64-deep
+ *               steps double-buffered in LDS, LDS-staged epilogue
+ *   k_gemm_tn, k_splitk_sum
+ *               weight gradients A^T B (both contiguous along M and N): the
+ *               batch split over ~256 workgroups, parts summed in order
+ *   k_transpose16  bf16 transposes (K-contiguous weight copies)
  *   k_bits_pm1  descriptor bits -> +/-1 BF16 inputs
- *   k_softmax_ce  row softmax, cross-entropy and its gradient (scaled 1/n)
+ *   k_softmax_ce, k_loss_sum
+ *               row softmax, cross-entropy and its gradient (scaled 1/n); the
+ *               epoch loss summed on the device in a fixed order
  *   k_colsum    bias gradients
  *   k_adam      Adam on FP32 master weights, BF16 copies refreshed
- *   k_probe     softmax max and the exact top-k buckets of each query
+ *   k_probe, k_probe_fix
+ *               softmax max and the exact top-k buckets of each query's FP16
+ *               logits, or the probability-mass prefix of them
  */
 #include "orb_internal.h"
 #include "orb_match_common.h"
