@@ -99,6 +99,27 @@ __device__ __forceinline__ uint32_t wave_kth_key(const uint32_t* key, int k)
     return T;
 }
 
+/* The wavefront's 64 values sorted in descending order across the lanes
+   (lane 0 the largest): a bitonic network of 21 compare-exchange steps, the
+   partner's value by a lane shuffle.  Every lane must be active. */
+__device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t o = (uint32_t)__shfl_xor((int)v, j, 64);
+            /* blocks of k lanes alternate direction, the whole wave descending
+               at k = 64; the lower lane of a pair keeps the larger value in a
+               descending block */
+            const bool keep_max = ((lane & j) == 0) == ((lane & k) == 0);
+            v = keep_max ? max(v, o) : min(v, o);
+        }
+    }
+    return v;
+}
+
 /* Top-k of up to 1024 entries, entry lane + 64 t holding key[t] (with PAIRS
    entry 128 (t / 2) + 2 lane + t % 2: a lane's slot pairs are adjacent
    entries) (key 0 marks an absent entry, every key below 2^(HIBIT+1); at least k entries present,
@@ -171,6 +192,41 @@ __device__ __forceinline__ bool wave_topk(const uint32_t (&key)[16], int k, uint
                        : S <= 8 ? wave_kth_key<N8, HIBIT>(ck, k)
                                 : wave_kth_key<NSL, HIBIT>(ck, k);
     wave_sync_lds();
+    if (SORTED && k <= 64) {
+        /* the k entries (those above T, then the first ones equal to T, each
+           group in entry order) one per lane, ranked by a sort of
+           key << 10 | (1023 - entry): descending key, ties by ascending entry */
+        static_assert(HIBIT <= 21, "key and entry number share 32 bits");
+        int base = 0;
+#pragma unroll
+        for (int s = 0; s < NSL; s++) {
+            if (s < S) {
+                const bool p = ck[s] > T;
+                const uint64_t m = __ballot(p);
+                if (p) cand[base + lanes_below(m)] = make_uint2(ck[s], (uint32_t)ci[s]);
+                base += __popcll(m);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NSL; s++) {
+            if (s < S && base < k) {
+                const bool p = ck[s] == T;
+                const uint64_t m = __ballot(p);
+                const int rank = base + lanes_below(m);
+                if (p && rank < k) cand[rank] = make_uint2(ck[s], (uint32_t)ci[s]);
+                base += __popcll(m);
+            }
+        }
+        wave_sync_lds();
+        uint32_t v = 0;
+        if (lane < k) {
+            const uint2 e = cand[lane];
+            v = (e.x << 10) | (1023u - e.y);
+        }
+        v = wave_sort_desc(v);
+        if (lane < k) out[lane] = (int)(1023u - (v & 1023u));
+        return true;
+    }
     int g = 0;
 #pragma unroll
     for (int s = 0; s < NSL; s++) {
