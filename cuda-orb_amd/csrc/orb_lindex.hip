@@ -393,10 +393,19 @@ __global__ __launch_bounds__(256) void k_probe_merge(const uint2* __restrict__ p
                                                      const int* __restrict__ ex_idx1, const int* __restrict__ ex_d1,
                                                      const int* __restrict__ ex_idx2, const int* __restrict__ ex_d2,
                                                      int* __restrict__ idx1, int* __restrict__ d1,
-                                                     int* __restrict__ idx2, int* __restrict__ d2)
+                                                     int* __restrict__ idx2, int* __restrict__ d2,
+                                                     int* __restrict__ reset_count, int ncount,
+                                                     uint32_t* __restrict__ reset_qbound)
 {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 3, j = threadIdx.x & 7;
     const bool live = i < nq;
+    if (reset_count) {
+        /* the probe-list counts and the search's bounds of this batch are no
+           longer read: left clean for the next batch */
+        if (blockIdx.x == 0)
+            for (int t = threadIdx.x; t < ncount; t += blockDim.x) reset_count[t] = 0;
+        if (live && j == 0) reset_qbound[i] = 0xffffffffu;
+    }
     uint32_t k1 = kNoKey, k2 = kNoKey;
     if (live) {
         for (int s = j; s < nprobe; s += 8) {
@@ -590,10 +599,10 @@ hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, in
 
 hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, const int* order, int* list_count,
                               int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, int* unit_pos,
-                              hipStream_t s)
+                              bool zeroed, hipStream_t s)
 {
     const long long n = (long long)nq * nprobe;
-    hipLaunchKernelGGL(k_zero_int, grid1(nb + 1), dim3(256), 0, s, list_count, nb + 1);
+    if (!zeroed) hipLaunchKernelGGL(k_zero_int, grid1(nb + 1), dim3(256), 0, s, list_count, nb + 1);
     if (nb > 1024) return hipErrorInvalidValue;
     const dim3 chunks((unsigned)((n + kPlChunk - 1) / kPlChunk));
     if (n > 0) hipLaunchKernelGGL(k_probe_count, chunks, dim3(256), 0, s, probes, (int)n, nb, list_count);
@@ -606,10 +615,10 @@ hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, con
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
                                 const int* unit_offsets, const int* unit_pos, const int* order, int nb,
                                 const uint8_t* db_sorted, const int* perm, const int* offsets, int* next,
-                                uint32_t* qbound, int nq, uint2* partial, int ncu, hipStream_t s)
+                                uint32_t* qbound, int nq, uint2* partial, int ncu, bool filled, hipStream_t s)
 {
     /* no bound yet for any query */
-    hipLaunchKernelGGL(k_fill_u32, grid1(nq), dim3(256), 0, s, qbound, nq, 0xffffffffu);
+    if (!filled) hipLaunchKernelGGL(k_fill_u32, grid1(nq), dim3(256), 0, s, qbound, nq, 0xffffffffu);
     /* as many workgroups as stay resident: they take units until none are left */
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_bucket_search, 256, 0) != hipSuccess || per < 1) per = 1;
@@ -631,11 +640,13 @@ hipError_t launch_fallback_compact(const uint8_t* q, int nq, const float* pmax, 
 
 hipError_t launch_probe_merge(const uint2* partial, const int* probes, int nq, int nprobe, const int* fb_index,
                               const int* ex_idx1, const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1,
-                              int* d1, int* idx2, int* d2, hipStream_t s)
+                              int* d1, int* idx2, int* d2, int* reset_count, int ncount, uint32_t* reset_qbound,
+                              hipStream_t s)
 {
     if (nq > 0)
         hipLaunchKernelGGL(k_probe_merge, dim3((unsigned)((nq + 31) / 32)), dim3(256), 0, s, partial, probes, nq,
-                           nprobe, fb_index, ex_idx1, ex_d1, ex_idx2, ex_d2, idx1, d1, idx2, d2);
+                           nprobe, fb_index, ex_idx1, ex_d1, ex_idx2, ex_d2, idx1, d1, idx2, d2, reset_count, ncount,
+                           reset_qbound);
     return hipGetLastError();
 }
 

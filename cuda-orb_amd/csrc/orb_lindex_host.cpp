@@ -105,6 +105,9 @@ struct orb_lindex {
     DBuf<uint8_t> fb_q;
     DBuf<uint2> partial;
     DBuf<uint32_t> qbound; /* per query: the smallest second distance found so far (bucket search) */
+    /* list_count is zero and qbound[0 .. clean_rows) holds "no bound": the
+       state each query batch's merge leaves for the next one (0: unknown) */
+    int clean_rows = 0;
 };
 
 namespace
@@ -604,6 +607,8 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     if (mode != ORB_LINDEX_MLP && mode != ORB_LINDEX_CENTROIDS) return lfail(L, ORB_E_ARG, "unknown mode");
     hipStream_t s = stream_of(L);
     const int np = L->prm.nprobe, nb = L->C;
+    const int* const count_before = L->list_count.p;
+    const uint32_t* const bound_before = L->qbound.p;
     LALLOC(L, L->probes, (size_t)nq * np);
     LALLOC(L, L->pmax, nq);
     LALLOC(L, L->list_count, nb + 1);
@@ -622,14 +627,19 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
     LALLOC(L, L->ex_d1, nq);
     LALLOC(L, L->ex_idx2, nq);
     LALLOC(L, L->ex_d2, nq);
+    /* counts and bounds left clean by the previous batch's merge need no
+       clearing launches; a new buffer or a batch that stopped early does */
+    if (L->list_count.p != count_before || L->qbound.p != bound_before) L->clean_rows = 0;
+    const bool clean = L->clean_rows >= nq;
+    L->clean_rows = 0;
     /* the search needs the probed set only: its order is left as it comes */
     int rc = compute_probes(L, d_q, nq, mode, L->probes.p, L->pmax.p, false, s);
     if (rc != ORB_OK) return rc;
     LTRY(L, launch_probe_lists(L->probes.p, nq, np, nb, L->border.p, L->list_count.p, L->list_offsets.p,
-                               L->list_cursor.p, L->lists.p, L->unit_offsets.p, L->unit_pos.p, s));
+                               L->list_cursor.p, L->lists.p, L->unit_offsets.p, L->unit_pos.p, clean, s));
     LTRY(L, launch_bucket_search(d_q, np, L->lists.p, L->list_offsets.p, L->unit_offsets.p, L->unit_pos.p, L->border.p, nb,
                                  L->db_sorted.p, L->perm.p, L->offsets.p, L->list_count.p + nb, L->qbound.p, nq,
-                                 L->partial.p, L->ncu, s));
+                                 L->partial.p, L->ncu, clean, s));
     const float thresh = mode == ORB_LINDEX_MLP ? L->prm.fallback : -1.f;
     /* the max softmax probability is above 0: a threshold at or below 0 (and
        centroid mode) sends no query to the exact search, which is skipped */
@@ -643,7 +653,9 @@ int orb_lindex_query(orb_lindex* L, const uint8_t* d_q, int nq, int mode, int* d
         if (rc != ORB_OK) return rc;
     }
     LTRY(L, launch_probe_merge(L->partial.p, L->probes.p, nq, np, may_fallback ? L->fb_index.p : nullptr, L->ex_idx1.p,
-                               L->ex_d1.p, L->ex_idx2.p, L->ex_d2.p, d_idx1, d_d1, d_idx2, d_d2, s));
+                               L->ex_d1.p, L->ex_idx2.p, L->ex_d2.p, d_idx1, d_d1, d_idx2, d_d2, L->list_count.p,
+                               nb + 1, L->qbound.p, s));
+    L->clean_rows = nq;
     if (h_nfallback) {
         if (may_fallback) {
             LTRY(L, hipMemcpyAsync(h_nfallback, L->fb_count.p, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -74,26 +74,30 @@ hipError_t launch_kmodes_update(const uint8_t* db_sorted, const int* offsets, in
                                 hipStream_t s);
 
 /* probe lists: queries x nprobe (bucket ids) -> per-bucket lists of q*nprobe+slot;
-   list_count has nb + 1 ints (the last is the search's unit counter, zeroed);
+   list_count has nb + 1 ints (the last is the search's unit counter), zeroed
+   first unless `zeroed` (already zero: the previous query's merge reset it);
    unit_offsets counts search units along order[] (buckets by decreasing size)
    and unit_pos[u] (nq * nprobe / 256 + nb ints) is the order position of unit u */
 hipError_t launch_probe_lists(const int* probes, int nq, int nprobe, int nb, const int* order, int* list_count,
                               int* list_offsets, int* list_cursor, int* lists, int* unit_offsets, int* unit_pos,
-                              hipStream_t s);
+                              bool zeroed, hipStream_t s);
 /* bucketed exact search: for every (query, probed bucket) the top-2 keys
    (distance << 23 | original index) over that bucket's entries, leaving out
    entries farther than the query's second distance in its other buckets
-   (qbound: nq words of scratch); units taken from the zeroed counter *next */
+   (qbound: nq words, set to "no bound" first unless `filled`); units taken
+   from the zeroed counter *next */
 hipError_t launch_bucket_search(const uint8_t* q, int nprobe, const int* lists, const int* list_offsets,
                                 const int* unit_offsets, const int* unit_pos, const int* order, int nb,
                                 const uint8_t* db_sorted, const int* perm, const int* offsets, int* next,
-                                uint32_t* qbound, int nq, uint2* partial, int ncu, hipStream_t s);
+                                uint32_t* qbound, int nq, uint2* partial, int ncu, bool filled, hipStream_t s);
 /* per query: merge of the probed buckets, or the exact result when the query
-   was sent to the fallback (fb_index >= 0) */
+   was sent to the fallback (fb_index >= 0); with reset_count, it also zeroes
+   reset_count[0 .. ncount) and sets reset_qbound[0 .. nq) to "no bound",
+   ready for the next query batch */
 hipError_t launch_probe_merge(const uint2* partial, const int* probes, int nq, int nprobe, const int* fb_index,
                               const int* ex_idx1,
                               const int* ex_d1, const int* ex_idx2, const int* ex_d2, int* idx1, int* d1, int* idx2,
-                              int* d2, hipStream_t s);
+                              int* d2, int* reset_count, int ncount, uint32_t* reset_qbound, hipStream_t s);
 /* rows idx[i] (or the first n) of q, with labels lab_src[idx[i]] when given */
 hipError_t launch_gather(const uint8_t* q, const int* idx, const int* lab_src, int n, uint8_t* qd, int* lab,
                          hipStream_t s);
