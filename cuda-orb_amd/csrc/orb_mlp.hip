@@ -120,7 +120,25 @@ __device__ __forceinline__ float gelu_grad(float z)
 }
 
 /* 8 staged values -> 8 output halves: FP16 bits (values already rounded)
-   with half, else BF16 of gelu(v) or v */
+   with HALF, else BF16 of gelu(v) (GELU) or v */
+template <bool GELU, bool HALF>
+__device__ __forceinline__ uint4 pack_out16_t(const float (&v)[8])
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        if constexpr (HALF) {
+            w[t] = h_bits(v[2 * t]) | (h_bits(v[2 * t + 1]) << 16);
+        } else {
+            const float e0 = GELU ? gelu(v[2 * t]) : v[2 * t];
+            const float e1 = GELU ? gelu(v[2 * t + 1]) : v[2 * t + 1];
+            w[t] = (uint32_t)f2bf(e0) | ((uint32_t)f2bf(e1) << 16);
+        }
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+/* the same with the flags of g */
 __device__ __forceinline__ uint4 pack_out16(const GemmArgs& g, const float (&v)[8])
 {
     uint32_t w[4];
@@ -646,6 +664,7 @@ constexpr int kFK = 32;
 constexpr int kFStage = (kFM + kFN) * kFK * 2; /* bytes */
 #define ORB_LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
 
+template <bool GELU, bool HALF>
 __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
 {
     __shared__ __align__(16) char s_st[3 * kFStage];
@@ -765,7 +784,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
         for (int b = 0; b < 4; b++) {
             float4 z = make_float4(acc[b][a][0] + bias[b].x, acc[b][a][1] + bias[b].y, acc[b][a][2] + bias[b].z,
                                    acc[b][a][3] + bias[b].w);
-            if (g.half) z = make_float4(round_h(z.x), round_h(z.y), round_h(z.z), round_h(z.w));
+            if constexpr (HALF) z = make_float4(round_h(z.x), round_h(z.y), round_h(z.z), round_h(z.w));
             *(float4*)(st + row * 68 + 16 * b + 4 * grp) = z;
         }
         wave_sync_lds();
@@ -789,7 +808,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
                     const float4 lo = *(const float4*)(st + rl * 68 + cl);
                     const float4 hi = *(const float4*)(st + rl * 68 + cl + 4);
                     const float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = pack_out16(g, v);
+                    *(uint4*)(g.out16 + (long long)m * g.ldc + n) = pack_out16_t<GELU, HALF>(v);
                 }
             }
         }
@@ -1072,7 +1091,9 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
                          (long long)g.N * g.sbn * 2 < (1ll << 31);
         if (big) {
             const int tiles_big = ((g.M + kFM - 1) / kFM) * ((g.N + kFN - 1) / kFN);
-            hipLaunchKernelGGL(k_gemm_fwd, dim3(tiles_big), dim3(256), 0, s, g);
+            if (g.half) hipLaunchKernelGGL((k_gemm_fwd<false, true>), dim3(tiles_big), dim3(256), 0, s, g);
+            else if (g.gelu) hipLaunchKernelGGL((k_gemm_fwd<true, false>), dim3(tiles_big), dim3(256), 0, s, g);
+            else hipLaunchKernelGGL((k_gemm_fwd<false, false>), dim3(tiles_big), dim3(256), 0, s, g);
         } else if (deep) hipLaunchKernelGGL(k_gemm_nt128, dim3(tiles), dim3(256), 0, s, g);
         else hipLaunchKernelGGL(k_gemm_nt, dim3(tiles), dim3(256), 0, s, g);
     } else if (g.ws && g.sam == 1 && g.sbn == 1 && g.splits >= 1 && g.out32 && !g.out16 && !g.addin && !g.bias &&
