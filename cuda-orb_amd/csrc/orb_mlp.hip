@@ -784,7 +784,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_fwd(GemmArgs g)
         for (int b = 0; b < 4; b++) {
             float4 z = make_float4(acc[b][a][0] + bias[b].x, acc[b][a][1] + bias[b].y, acc[b][a][2] + bias[b].z,
                                    acc[b][a][3] + bias[b].w);
-            if constexpr (HALF) z = make_float4(round_h(z.x), round_h(z.y), round_h(z.z), round_h(z.w));
+            /* FP32 stores of the rounded values; FP16 stores round in the pack */
+            if constexpr (HALF)
+                if (g.out32) z = make_float4(round_h(z.x), round_h(z.y), round_h(z.z), round_h(z.w));
             *(float4*)(st + row * 68 + 16 * b + 4 * grp) = z;
         }
         wave_sync_lds();
