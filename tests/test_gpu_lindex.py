@@ -99,6 +99,31 @@ def small_index(torch, ctx):
     lx.close()
 
 
+def test_query_batches_in_sequence(torch, ctx, small_index):
+    """A query batch leaves the probe-list counts and the search bounds clean for the next one
+    (the clearing launches run only for a new buffer or a larger batch): batches of growing and
+    shrinking size, in both probe modes, each equal the oracle."""
+    import orb_amd
+    from oracle import mlp
+    lx, db, ddb = small_index
+    rng = np.random.default_rng(77)
+    _, _, bof = lx.bucket_info()
+    bof = bof.cpu().numpy()
+    lx.set_probe(6, 0.0)
+    try:
+        for nq, mode in ((128, orb_amd.ORB_LINDEX_MLP), (640, orb_amd.ORB_LINDEX_MLP), (64, orb_amd.ORB_LINDEX_CENTROIDS),
+                         (640, orb_amd.ORB_LINDEX_CENTROIDS), (1024, orb_amd.ORB_LINDEX_MLP), (192, orb_amd.ORB_LINDEX_MLP)):
+            print(f"batch {nq} mode {mode}", flush=True)
+            q, _ = flip_copies(rng, db, nq, p=0.1)
+            dq = torch.from_numpy(q).cuda()
+            probes = lx.probes(dq, mode=mode)[0].cpu().numpy()
+            got, nfb = lx.query(dq, mode=mode)
+            assert nfb == 0
+            assert_top2(got, mlp.bucket_search(q, db, bof, probes), (nq, mode))
+    finally:
+        lx.set_probe(4, 0.3)
+
+
 def test_logits_into_an_unaligned_buffer(torch, ctx, small_index):
     """orb_lindex_logits writes into the caller's buffer directly when it is 16-byte aligned and
     through a copy otherwise: both give the same values."""
