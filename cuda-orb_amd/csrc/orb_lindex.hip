@@ -271,6 +271,11 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
            top-2 at the phantom key (bound << 23 | kIdxMask), above every
            real key of that distance; phantoms are dropped at the end. */
         uint32_t k1[4], k2[4];
+        /* acc = 256 - dot = 2 d for distance d, so a block's results can enter
+           the top-2 only if its smallest acc is at most 2 (k2 >> 23): bb holds
+           the float bits of that bound (non-negative floats order like their
+           bits) */
+        uint32_t bb[4];
 #pragma unroll
         for (int qb = 0; qb < 4; qb++) {
             const int e = e0 + wid * 64 + qb * 16 + col;
@@ -288,6 +293,7 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) bq[qb][ks] = fp4_word(w[4 * ks + g], fp4_neg_pm1_8);
             k1[qb] = k2[qb] = bd <= 256u ? (bd << kIdxBits) | kIdxMask : kNoKey;
+            bb[qb] = __float_as_uint((float)(2u * (k2[qb] >> kIdxBits)));
         }
         const int j0 = offsets[b], j1 = offsets[b + 1];
         const bool wave_busy = e0 + wid * 64 < e1;
@@ -310,16 +316,15 @@ __global__ __launch_bounds__(256, 4) void k_bucket_search(const uint8_t* __restr
                 for (int qb = 0; qb < 4; qb++) {
                     v4f acc = mfma_fp4(a[0], bq[qb][0], c256);
                     acc = mfma_fp4(a[1], bq[qb][1], acc);
-                    /* acc holds 256 - dot (float bits order like the values):
-                       the key of the smallest with index 0 bounds the 4 keys
-                       from below: above k2, none can enter the top-2 */
+                    /* acc = 2 d: the key d << 23 | position is acc << 22 + position */
                     const uint32_t mb = min(min(__float_as_uint(acc[0]), __float_as_uint(acc[1])),
                                             min(__float_as_uint(acc[2]), __float_as_uint(acc[3])));
-                    if (pm1_neg_dot_key((int)__uint_as_float(mb) - 256, 0u) <= k2[qb]) {
+                    if (mb <= bb[qb]) {
 #pragma unroll
                         for (int r = 0; r < 4; r++)
                             if (!PART || jb + r < j1)
-                                top2_push(k1[qb], k2[qb], pm1_neg_dot_key((int)acc[r] - 256, (uint32_t)(jb + r)));
+                                top2_push(k1[qb], k2[qb], ((uint32_t)(int)acc[r] << (kIdxBits - 1)) + (uint32_t)(jb + r));
+                        bb[qb] = __float_as_uint((float)(2u * (k2[qb] >> kIdxBits)));
                     }
                 }
             }
