@@ -1,16 +1,16 @@
 /*
- * Device helpers shared by the exact matchers (orb_match.hip) and the
- * bucketed search of the learned matcher (orb_lindex.hip).
+ * Device helpers shared by the exact matchers (orb_match.hip), the bucketed
+ * search of the learned matcher (orb_lindex.hip) and the forward pass's
+ * centroid term (orb_mlp.hip).
  *
  * A (query, entry) pair is the packed key (distance << 23 | index): the
  * minimum key is the nearest entry with ties to the lowest index, and a
- * running top-2 is (min(k1, key), med3(k1, key, k2)).  The INT8 MFMA paths
- * expand the query's bits to +/-1 bytes and the database entry's either to
- * 0/1 bytes (dot = (both set) - (entry set, query clear), distance =
- * popcount(query) - dot: fewer unpack instructions) or to +/-1 bytes
- * (distance = (256 - dot) / 2: no per-query term, fewer registers).  The
- * query operand is negated, so the accumulators hold -dot and a key is a
- * single shift-add.
+ * running top-2 is (min(k1, key), med3(k1, key, k2)).  The matrix-core
+ * paths contract descriptor bits as FP4 codes, one per bit: the query's as
+ * negated +/-1 codes and the entry's as 0/1 codes (dot = (entry set, query
+ * clear) - (both set), distance = popcount(query) + that: the exact
+ * matchers) or as +/-1 codes (256 - 2 distance, no per-query term: the
+ * bucketed search and the centroid term).
  */
 #pragma once
 
@@ -102,18 +102,6 @@ __device__ __forceinline__ v4f mfma_fp4(v4i a, v4i b, v4f c)
     return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, 127, 0, 127);
 }
 
-/* An unpacked 64-entry database tile in LDS: one row per entry, 16 chunks
-   of 16 bytes (the expansion of 16 descriptor bits each), rows padded to
-   288 bytes.  (Measured on the dense 65536 x 262144 match: rows of 272, 288
-   or 304 bytes run alike; 320-byte rows and every XOR swizzle of the chunks
-   within 256-byte rows tried ran 7-28 % slower.) */
-constexpr int kTileRow = 256 + 32;
-
-__device__ __forceinline__ int lds_chunk(int r, int c)
-{
-    return r * kTileRow + 16 * c;
-}
-
 /* k2 = med3(k1, key, k2) (k1 <= k2 always holds) and k1 = min(k1, key).
    Written as min/max so the compiler selects v_med3_u32 and v_min_u32
    itself and places any wait states the MFMA results they read need. */
@@ -124,64 +112,12 @@ __device__ __forceinline__ void top2_push(uint32_t& k1, uint32_t& k2, uint32_t k
     k1 = lo;
 }
 
-/* 16 bits -> 16 bytes of 1 (bit set) / 0 (bit clear) */
-__device__ __forceinline__ v4i b01_16(uint32_t h)
-{
-    v4i r;
-    r.x = (int)(((h & 15u) * 0x00204081u) & 0x01010101u);
-    r.y = (int)((((h >> 4) & 15u) * 0x00204081u) & 0x01010101u);
-    r.z = (int)((((h >> 8) & 15u) * 0x00204081u) & 0x01010101u);
-    r.w = (int)((((h >> 12) & 15u) * 0x00204081u) & 0x01010101u);
-    return r;
-}
-
-/* 4 bits -> 4 bytes of +1 (bit set) / -1 (bit clear): the multiply spreads
-   bit i to bit 2 of byte i, so each selector byte is 4 (a byte of the first
-   source, 0x01) or 0 (a byte of the second, 0xFF) */
-__device__ __forceinline__ int pm1_nibble(uint32_t n)
-{
-    return (int)__builtin_amdgcn_perm(0x01010101u, 0xFFFFFFFFu, (n * 0x00810204u) & 0x04040404u);
-}
-
-/* 16 bits -> 16 +/-1 bytes */
-__device__ __forceinline__ v4i pm1_16(uint32_t h)
-{
-    v4i r;
-    r.x = pm1_nibble(h & 15u);
-    r.y = pm1_nibble((h >> 4) & 15u);
-    r.z = pm1_nibble((h >> 8) & 15u);
-    r.w = pm1_nibble((h >> 12) & 15u);
-    return r;
-}
-
-/* B operands of one query for the four 64-deep K steps of a 256-bit
-   descriptor, negated: lane k-group g takes bits 64*ks + 16*g .. +15 and
-   expands their complement (-1 where the bit is set, +1 where clear), so the
-   MFMA accumulates -dot and a key is one shift-add of the accumulator */
-__device__ __forceinline__ void query_operands_neg(uint4 d0, uint4 d1, int g, v4i* bq)
-{
-    const uint32_t w[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++) {
-        const int hw = 4 * ks + g;
-        bq[ks] = pm1_16(~(w[hw >> 1] >> (16 * (hw & 1))) & 0xffffu);
-    }
-}
-
 /* key of a (+/-1 query, 0/1 entry) pair from the accumulated -dot:
    (popcount(query) - dot) << 23 | idx = (-dot << 23) + base_idx with
    base_idx = popcount(query) << 23 | idx (one v_lshl_add_u32) */
 __device__ __forceinline__ uint32_t neg_dot_key(int negdot, uint32_t base_idx)
 {
     return ((uint32_t)negdot << kIdxBits) + base_idx;
-}
-
-/* key of a (+/-1 query, +/-1 entry) pair from the accumulated -dot: the
-   distance is (256 - dot) / 2 (dot is even), so the key is
-   (-dot << 22) + (128 << 23) + idx modulo 2^32 */
-__device__ __forceinline__ uint32_t pm1_neg_dot_key(int negdot, uint32_t idx)
-{
-    return ((uint32_t)negdot << (kIdxBits - 1)) + ((128u << kIdxBits) + idx);
 }
 
 /* For the 0/1-entry matcher: the largest -dot whose key can still be at or
