@@ -464,14 +464,12 @@ __global__ void k_gather(const uint8_t* __restrict__ q, const int* __restrict__ 
    nearest centroid, wave_mass_keep), best first and -1 after.  The
    centroids are staged
    in LDS once per workgroup and each wavefront takes kCenQ queries in turn.
-   CAP and FIX as for the network's probe: with CAP below 1024 a query whose
-   candidates do not fit gets -1 as its first probe, and the FIX launch
-   (CAP 1024) redoes those queries, its workgroups without one leaving before
-   staging the centroids. */
+   CAP as for the network's probe: with CAP below 1024 a query whose
+   candidates do not fit the LDS buffer is finished from its slots. */
 constexpr int kCenQ = 8;
 constexpr int kCenCap = 256;
 
-template <bool SORTED, int CAP, bool FIX>
+template <bool SORTED, int CAP>
 __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restrict__ q, int nq,
                                                         const uint8_t* __restrict__ cen, int nb, int k, float mass,
                                                         float inv_temp, int* __restrict__ probes,
@@ -480,18 +478,12 @@ __global__ __launch_bounds__(256) void k_centroid_probe(const uint8_t* __restric
     __shared__ uint4 s_c[1024 * 2];
     __shared__ uint2 s_cand[4][CAP];
     __shared__ int s_ids[4][64];
-    if (FIX) {
-        const int r = blockIdx.x * 4 * kCenQ + (int)threadIdx.x;
-        const bool redo = threadIdx.x < 4 * kCenQ && r < nq && probes[(long long)r * k] < 0;
-        if (!__syncthreads_or(redo)) return;
-    }
     for (int i = threadIdx.x; i < nb * 2; i += 256) s_c[i] = ((const uint4*)cen)[i];
     __syncthreads();
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int t = 0; t < kCenQ; t++) {
         const int r = (blockIdx.x * 4 + wv) * kCenQ + t;
         if (r >= nq) return;
-        if (FIX && probes[(long long)r * k] >= 0) continue;
         const uint4 qa = ((const uint4*)(q + (long long)r * 32))[0], qb = ((const uint4*)(q + (long long)r * 32))[1];
         /* key 0x10000 - distance: nearer centroids rank higher; 0 marks no centroid */
         uint32_t key[16];
@@ -558,20 +550,13 @@ hipError_t launch_centroid_probe(const uint8_t* q, int nq, const uint8_t* cen, i
     const dim3 grid((nq + 4 * kCenQ - 1) / (4 * kCenQ));
     if (nq <= 0) return hipGetLastError();
     const bool split = k <= 64 && nb > kCenCap;
-#define CENTROID_PROBE(S, CAPV, FIXV)                                                                          \
-    hipLaunchKernelGGL((k_centroid_probe<S, CAPV, FIXV>), grid, dim3(256), 0, s, q, nq, cen, nb, k, mass, it, \
-                       probes, pmax)
-    if (sorted && split) {
-        CENTROID_PROBE(true, kCenCap, false);
-        CENTROID_PROBE(true, 1024, true);
-    } else if (sorted) {
-        CENTROID_PROBE(true, 1024, false);
-    } else if (split) {
-        CENTROID_PROBE(false, kCenCap, false);
-        CENTROID_PROBE(false, 1024, true);
-    } else {
-        CENTROID_PROBE(false, 1024, false);
-    }
+#define CENTROID_PROBE(S, CAPV)                                                                          \
+    hipLaunchKernelGGL((k_centroid_probe<S, CAPV>), grid, dim3(256), 0, s, q, nq, cen, nb, k, mass, it, probes, \
+                       pmax)
+    if (sorted && split) CENTROID_PROBE(true, kCenCap);
+    else if (sorted) CENTROID_PROBE(true, 1024);
+    else if (split) CENTROID_PROBE(false, kCenCap);
+    else CENTROID_PROBE(false, 1024);
 #undef CENTROID_PROBE
     return hipGetLastError();
 }

@@ -29,8 +29,7 @@
  *               epoch loss summed on the device in a fixed order
  *   k_colsum    bias gradients
  *   k_adam      Adam on FP32 master weights, BF16 copies refreshed
- *   k_probe, k_probe_fix
- *               softmax max and the exact top-k buckets of each query's FP16
+ *   k_probe     softmax max and the exact top-k buckets of each query's FP16
  *               logits, or the probability-mass prefix of them
  */
 #include "orb_internal.h"
@@ -953,8 +952,9 @@ constexpr int kProbeCap = 256;
 /* Max softmax probability (*pmax_r) and the top-k logits (out[0 .. k)) of
    the FP16 logit row z, one wavefront (ties to the lower bucket id; C <= 1024,
    a multiple of 64), by descending logit with SORTED, else by bucket id; cand
-   holds CAP candidates.  False: more than CAP candidates, nothing written to
-   the probes.  With mass below 1
+   holds CAP candidates (more are handled from the slots for k <= 64; false,
+   nothing written, only for k > 64 with CAP below 1024, never launched).
+   With mass below 1
    (k <= 64) only the best-ranked prefix whose softmax mass reaches `mass` is
    probed: the top-k are ranked (logit descending, ties to the lower id), the
    weights exp(z - max) of the ranks before a candidate are summed
@@ -1017,7 +1017,7 @@ __device__ __forceinline__ bool probe_row(const uint16_t* __restrict__ z, int C,
 
 /* one wavefront per query.  With CAP below 1024 (LDS for CAP candidates per
    wavefront, which leaves room for more resident wavefronts) a query whose
-   candidates do not fit gets -1 as its first probe, for k_probe_fix. */
+   candidates do not fit is finished from its slots (probe_row). */
 template <bool SORTED, int CAP>
 __global__ __launch_bounds__(256) void k_probe(const uint16_t* __restrict__ Z, int n, int C, int k, float mass,
                                                int* __restrict__ probes, float* __restrict__ pmax)
@@ -1033,34 +1033,14 @@ __global__ __launch_bounds__(256) void k_probe(const uint16_t* __restrict__ Z, i
         probes[(long long)r * k] = -1;
 }
 
-/* the queries k_probe marked, redone with room for every candidate: each
-   wavefront reads the marks of 64 queries and takes the marked ones in turn */
-template <bool SORTED>
-__global__ __launch_bounds__(256) void k_probe_fix(const uint16_t* __restrict__ Z, int n, int C, int k, float mass,
-                                                   int* __restrict__ probes, float* __restrict__ pmax)
-{
-    __shared__ uint2 s_cand[4][1024];
-    __shared__ int s_ids[4][64];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int base = (blockIdx.x * 4 + wv) * 64;
-    const bool redo = base + lane < n && probes[(long long)(base + lane) * k] < 0;
-    for (uint64_t m = __ballot(redo); m; m &= m - 1) {
-        const int r = base + __ffsll((unsigned long long)m) - 1;
-        probe_row<SORTED, 1024>(Z + (long long)r * C, C, k, mass, probes + (long long)r * k, pmax + r, s_cand[wv],
-                                s_ids[wv]);
-        wave_sync_lds(); /* the next query reuses s_cand */
-    }
-}
-
-
 template <bool SORTED>
 static void probe_launch(const uint16_t* Z, int n, int C, int k, float mass, int* probes, float* pmax, hipStream_t s)
 {
     const dim3 grid((n + 3) / 4);
     if (k <= 64 && C > kProbeCap) {
+        /* a query whose candidates overflow the LDS buffer is finished from
+           its slots inside the kernel: no second pass */
         hipLaunchKernelGGL((k_probe<SORTED, kProbeCap>), grid, dim3(256), 0, s, Z, n, C, k, mass, probes, pmax);
-        hipLaunchKernelGGL((k_probe_fix<SORTED>), dim3((n + 255) / 256), dim3(256), 0, s, Z, n, C, k, mass, probes,
-                           pmax);
     } else {
         hipLaunchKernelGGL((k_probe<SORTED, 1024>), grid, dim3(256), 0, s, Z, n, C, k, mass, probes, pmax);
     }

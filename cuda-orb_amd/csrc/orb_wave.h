@@ -127,8 +127,9 @@ __device__ __forceinline__ uint32_t wave_sort_desc(uint32_t v)
    taken by ascending entry number, to out[0 .. k) -- with SORTED by
    descending key (equal keys by ascending entry number), otherwise in entry
    order.  cand is this wavefront's LDS buffer of CAP entries; with CAP below
-   1024, more than CAP candidates (ties at the k-th lane maximum, or k > 64)
-   make it return false having written nothing, otherwise it returns true.
+   1024, more than CAP candidates (ties at the k-th lane maximum) are handled
+   from the slots directly (k <= 64), and only k > 64 makes it return false
+   having written nothing; otherwise it returns true.
 
    1. For k <= 64 the k-th largest lane maximum L bounds the answer from below
       (k lanes hold a key >= L); entries >= L are compacted into cand in entry
@@ -172,7 +173,58 @@ __device__ __forceinline__ bool wave_topk(const uint32_t (&key)[16], int k, uint
             nc += __popcll(m);
         }
     }
-    if (nc > CAP) return false;
+    if (nc > CAP) {
+        if (k > 64) return false;
+        /* more candidates than fit (ties at the bound): the exact k-th key
+           over every slot, then the entries above it and the first ones
+           equal to it (at most k <= 64, so they fit), each group in entry
+           order, taken straight from the slots */
+        const uint32_t T = wave_kth_key<16, HIBIT>(key, k);
+        wave_sync_lds();
+        int base = 0;
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            auto take = [&](uint32_t kv, uint32_t id, int before) {
+                const int pos = base + before;
+                const bool p = pass == 0 ? kv > T : kv == T;
+                if (p && pos < k) cand[pos] = make_uint2(kv, id);
+            };
+            if (PAIRS) {
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    const bool p0 = pass == 0 ? key[2 * t] > T : key[2 * t] == T;
+                    const bool p1 = pass == 0 ? key[2 * t + 1] > T : key[2 * t + 1] == T;
+                    const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+                    const int before = lanes_below(m0) + lanes_below(m1);
+                    const uint32_t id = (uint32_t)(128 * t + 2 * lane);
+                    take(key[2 * t], id, before);
+                    take(key[2 * t + 1], id + 1u, before + (int)p0);
+                    base += __popcll(m0) + __popcll(m1);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 16; t++) {
+                    const bool p = pass == 0 ? key[t] > T : key[t] == T;
+                    const uint64_t m = __ballot(p);
+                    take(key[t], (uint32_t)(lane + 64 * t), lanes_below(m));
+                    base += __popcll(m);
+                }
+            }
+        }
+        wave_sync_lds();
+        if (SORTED) {
+            uint32_t v = 0;
+            if (lane < k) {
+                const uint2 e = cand[lane];
+                v = (e.x << 10) | (1023u - e.y);
+            }
+            v = wave_sort_desc(v);
+            if (lane < k) out[lane] = (int)(1023u - (v & 1023u));
+        } else if (lane < k) {
+            out[lane] = (int)cand[lane].y;
+        }
+        return true;
+    }
     wave_sync_lds();
     const int S = (nc + 63) >> 6;
     uint32_t ck[NSL];
